@@ -1,0 +1,166 @@
+/*
+ * par2gpu.h -- C ABI of the MI355X PAR2 create engine (libpar2gpu.so).
+ *
+ * The reference (javi11/par2rust) has no FFI on this path: its encode is a
+ * set of internal Rust calls.  Each entry point below replaces one of those
+ * internal call sites, cited as file:line of the reference tree.  Plain
+ * pointers and sizes only; every call returns a p2g status (0 = success);
+ * p2g_last_error() gives the message of the last failure on a context.
+ *
+ * Threading: a context is used by one host thread at a time.  All device
+ * work of a context runs on that context's HIP streams; the host-buffer
+ * entry points return after their results are in host memory.
+ */
+#ifndef PAR2GPU_H
+#define PAR2GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes.  1..9 map one-to-one onto the Par2Error variants of
+ * src/error.rs:7-41; 10.. are conditions the reference does not have. */
+enum {
+    P2G_OK = 0,
+    P2G_ERR_IO = 1,                      /* Par2Error::Io */
+    P2G_ERR_INVALID_SLICE_SIZE = 2,      /* Par2Error::InvalidSliceSize */
+    P2G_ERR_INVALID_FILE_NAME = 3,       /* Par2Error::InvalidFileName */
+    P2G_ERR_NO_INPUT_FILES = 4,          /* Par2Error::NoInputFiles */
+    P2G_ERR_EMPTY_FILE = 5,              /* Par2Error::EmptyFile */
+    P2G_ERR_TOO_MANY_FILES = 6,          /* Par2Error::TooManyFiles */
+    P2G_ERR_TOO_MANY_RECOVERY_BLOCKS = 7,/* Par2Error::TooManyRecoveryBlocks */
+    P2G_ERR_TOO_MANY_INPUT_BLOCKS = 8,   /* Par2Error::TooManyInputBlocks */
+    P2G_ERR_INVALID_VOLUME_SCHEME = 9,   /* Par2Error::InvalidVolumeScheme */
+    P2G_ERR_INVALID_ARGUMENT = 10,       /* bad pointer / size at this ABI */
+    P2G_ERR_DEVICE = 11,                 /* HIP runtime or kernel failure */
+    P2G_ERR_OUT_OF_MEMORY = 12,          /* device or pinned allocation failed */
+};
+
+/* PAR2 limits, src/creator.rs:102-108. */
+#define P2G_MAX_FILES 32768u
+#define P2G_MAX_INPUT_BLOCKS 32768u
+#define P2G_MAX_RECOVERY_BLOCKS 65535u
+
+typedef struct p2g_ctx p2g_ctx;
+typedef struct p2g_job p2g_job;
+
+/* Opens `device`, builds the GF(2^16) log/antilog tables on it and creates
+ * the context's streams.  No reference counterpart: the reference's
+ * equivalent state is gf16_tables() (src/galois.rs:28-31) and the global
+ * rayon pool. */
+int p2g_open(int device, p2g_ctx **out);
+void p2g_close(p2g_ctx *ctx);
+const char *p2g_last_error(const p2g_ctx *ctx);
+/* Version string of the library, e.g. "par2gpu 0.1.0 gfx950". */
+const char *p2g_version(void);
+
+/* ---------------------------------------------------------------------
+ * Host-buffer entry points (the drop-in cut points)
+ * ------------------------------------------------------------------- */
+
+/* Per-slice MD5 and CRC-32 of one file's bytes, the last slice zero padded
+ * to slice_size: ceil(len / slice_size) entries into md5_out (16 bytes each)
+ * and crc_out.  Replaces the per-slice closure of scan_via_mmap,
+ * src/source.rs:165-220 (called from scan_with_progress, :100-109). */
+int p2g_slice_checksums(p2g_ctx *ctx, const uint8_t *data, uint64_t len,
+                        uint64_t slice_size, uint8_t *md5_out, uint32_t *crc_out);
+
+/* Streaming recovery encode.  Replaces encode_all_recovery_blocks,
+ * src/creator.rs:883-1018 (called at :252-257), and, through push with a
+ * running first_input_index, encode_file_slices_into_buffers, :728-843.
+ *   begin:  total_inputs = N of the whole set (fixes the RS input bases),
+ *           recovery_count = R, first exponent 0 (src/creator.rs:915).
+ *   push:   `len` bytes holding consecutive input slices starting at global
+ *           input index first_input_index; the last slice may be partial
+ *           and is zero padded.  Pushes may come in any order.
+ *   finish: recovery block r (exponent r) into rec_out + r * slice_size.
+ *           With set_id non-NULL, pkt_md5_out[16 r ..] also receives the
+ *           packet hash of recovery packet r, MD5(set_id || "PAR 2.0\0RecvSlic"
+ *           || le32(r) || block r), as build_packet computes it
+ *           (src/format.rs:40-44 via src/packet/recovery.rs:11-19). */
+int p2g_encode_begin(p2g_ctx *ctx, uint64_t slice_size, uint32_t total_inputs,
+                     uint32_t recovery_count, p2g_job **out);
+int p2g_encode_push(p2g_job *job, uint32_t first_input_index, const uint8_t *data,
+                    uint64_t len);
+int p2g_encode_finish(p2g_job *job, uint8_t *rec_out, const uint8_t *set_id,
+                      uint8_t *pkt_md5_out);
+void p2g_encode_free(p2g_job *job);
+
+/* Packet hashes of R recovery packets whose blocks are in host memory
+ * (block r at rec + r * slice_size, exponent first_exponent + r).  Replaces
+ * the R calls of build_packet made by build_volume_packet,
+ * src/creator.rs:1031-1041. */
+int p2g_recovery_packet_md5(p2g_ctx *ctx, const uint8_t *rec, uint32_t count,
+                            uint64_t slice_size, uint32_t first_exponent,
+                            const uint8_t set_id[16], uint8_t *pkt_md5_out);
+
+/* ---------------------------------------------------------------------
+ * Device-buffer entry points (inputs already resident in HBM).  The same
+ * kernels as above, without host copies; `stream` is a hipStream_t or NULL
+ * for the context's own stream.  They return once the work is queued;
+ * p2g_synchronize waits for it.
+ * ------------------------------------------------------------------- */
+
+/* One input file resident on the device. */
+typedef struct {
+    const uint8_t *data; /* device pointer, 16-byte aligned */
+    uint64_t len;        /* bytes; ceil(len / slice_size) slices */
+} p2g_dev_file;
+
+/* Encode every slice of `files` (file order, slice order) into rec
+ * (device, recovery_count * slice_size bytes, exponent-major).
+ * first_input_index is the global index of the first slice (0 for a whole
+ * set) and total_inputs the N that fixes the input bases.  accumulate = 0
+ * overwrites rec, 1 XORs into it. */
+int p2g_dev_encode(p2g_ctx *ctx, const p2g_dev_file *files, uint32_t nfiles,
+                   uint64_t slice_size, uint32_t first_input_index, uint32_t total_inputs,
+                   uint32_t recovery_count, uint8_t *rec, int accumulate, void *stream);
+
+/* Per-slice MD5 (16 B each) and CRC-32 of every slice of `files`, written to
+ * device arrays in file order then slice order. */
+int p2g_dev_slice_checksums(p2g_ctx *ctx, const p2g_dev_file *files, uint32_t nfiles,
+                            uint64_t slice_size, uint8_t *md5_out, uint32_t *crc_out,
+                            void *stream);
+
+/* Packet MD5s of `count` recovery packets whose blocks are on the device. */
+int p2g_dev_recovery_packet_md5(p2g_ctx *ctx, const uint8_t *rec, uint32_t count,
+                                uint64_t slice_size, uint32_t first_exponent,
+                                const uint8_t set_id[16], uint8_t *pkt_md5_out,
+                                void *stream);
+
+int p2g_synchronize(p2g_ctx *ctx);
+
+/* ---------------------------------------------------------------------
+ * Whole-pipeline entry point: SourceFile::scan for every input, then
+ * run_create.  Replaces the library caller of examples/create_from_lib.rs:
+ * 12-41 (scan, src/source.rs:53, then run_create, src/creator.rs:182).
+ * ------------------------------------------------------------------- */
+
+enum { P2G_SCHEME_SINGLE = 0, P2G_SCHEME_EXPONENTIAL = 1, P2G_SCHEME_UNIFORM = 2,
+       P2G_SCHEME_EXPLICIT = 3 };
+
+/* VolumeScheme, src/creator.rs:110-139.  Limited wraps the inner scheme. */
+typedef struct {
+    int kind;                        /* P2G_SCHEME_* */
+    uint32_t uniform_count;          /* Uniform { count } */
+    int limited;                     /* Limited { max_blocks_per_volume, inner } */
+    uint32_t limit_cap;              /* max_blocks_per_volume */
+    const uint32_t *explicit_sizes;  /* Explicit(sizes) */
+    uint32_t n_explicit;
+} p2g_volume_scheme;
+
+/* Writes the index file `output` and its volume files.  names_out receives
+ * the NUL-separated paths written, index first (capacity names_cap). */
+int p2g_run_create(p2g_ctx *ctx, const char *output, uint64_t slice_size,
+                   uint32_t recovery_count, const p2g_volume_scheme *scheme,
+                   uint32_t nfiles, const char *const *paths,
+                   const uint8_t *const *names, const uint32_t *name_lens,
+                   char *names_out, size_t names_cap);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

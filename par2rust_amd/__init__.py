@@ -1,0 +1,306 @@
+"""ctypes binding of libpar2gpu.so, the MI355X PAR2 create engine.
+
+Mirrors the reference's library surface (javi11/par2rust src/lib.rs:42-61):
+``CreateOptions``, ``VolumeScheme``, ``run_create`` and ``SourceFile``-style
+scanning, each a call through the C ABI declared in include/par2gpu.h.  The
+library must have been built (python par2rust_amd/build.py); importing this
+package without it raises, and there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass, field
+from typing import List, Optional, Sequence, Tuple
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpar2gpu.so")
+
+# status codes, include/par2gpu.h
+P2G_OK = 0
+ERRORS = {
+    1: "Io",
+    2: "InvalidSliceSize",
+    3: "InvalidFileName",
+    4: "NoInputFiles",
+    5: "EmptyFile",
+    6: "TooManyFiles",
+    7: "TooManyRecoveryBlocks",
+    8: "TooManyInputBlocks",
+    9: "InvalidVolumeScheme",
+    10: "InvalidArgument",
+    11: "Device",
+    12: "OutOfMemory",
+}
+
+SCHEME_SINGLE, SCHEME_EXPONENTIAL, SCHEME_UNIFORM, SCHEME_EXPLICIT = 0, 1, 2, 3
+
+EXPORTED_SYMBOLS = [
+    "p2g_open", "p2g_close", "p2g_last_error", "p2g_version",
+    "p2g_slice_checksums", "p2g_encode_begin", "p2g_encode_push", "p2g_encode_finish",
+    "p2g_encode_free", "p2g_recovery_packet_md5", "p2g_dev_encode",
+    "p2g_dev_slice_checksums", "p2g_dev_recovery_packet_md5", "p2g_synchronize",
+    "p2g_run_create",
+]
+
+
+class Par2Error(RuntimeError):
+    """A failed p2g call; ``kind`` names the Par2Error variant (src/error.rs:7-41)."""
+
+    def __init__(self, code: int, message: str = ""):
+        self.code = code
+        self.kind = ERRORS.get(code, f"Unknown({code})")
+        super().__init__(f"{self.kind}: {message}" if message else self.kind)
+
+
+class _DevFile(ctypes.Structure):
+    _fields_ = [("data", ctypes.c_void_p), ("len", ctypes.c_uint64)]
+
+
+class _Scheme(ctypes.Structure):
+    _fields_ = [
+        ("kind", ctypes.c_int),
+        ("uniform_count", ctypes.c_uint32),
+        ("limited", ctypes.c_int),
+        ("limit_cap", ctypes.c_uint32),
+        ("explicit_sizes", ctypes.POINTER(ctypes.c_uint32)),
+        ("n_explicit", ctypes.c_uint32),
+    ]
+
+
+def _load() -> ctypes.CDLL:
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is missing; build it with `python par2rust_amd/build.py`")
+    lib = ctypes.CDLL(LIB_PATH)
+    P, u8p, u32p = ctypes.c_void_p, ctypes.c_char_p, ctypes.POINTER(ctypes.c_uint32)
+    sig = {
+        "p2g_open": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(P)]),
+        "p2g_close": (None, [P]),
+        "p2g_last_error": (ctypes.c_char_p, [P]),
+        "p2g_version": (ctypes.c_char_p, []),
+        "p2g_slice_checksums": (ctypes.c_int, [P, P, ctypes.c_uint64, ctypes.c_uint64, P, P]),
+        "p2g_encode_begin": (ctypes.c_int, [P, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32,
+                                            ctypes.POINTER(P)]),
+        "p2g_encode_push": (ctypes.c_int, [P, ctypes.c_uint32, P, ctypes.c_uint64]),
+        "p2g_encode_finish": (ctypes.c_int, [P, P, P, P]),
+        "p2g_encode_free": (None, [P]),
+        "p2g_recovery_packet_md5": (ctypes.c_int, [P, P, ctypes.c_uint32, ctypes.c_uint64,
+                                                   ctypes.c_uint32, P, P]),
+        "p2g_dev_encode": (ctypes.c_int, [P, P, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint32,
+                                          ctypes.c_uint32, ctypes.c_uint32, P, ctypes.c_int, P]),
+        "p2g_dev_slice_checksums": (ctypes.c_int, [P, P, ctypes.c_uint32, ctypes.c_uint64, P, P,
+                                                   P]),
+        "p2g_dev_recovery_packet_md5": (ctypes.c_int, [P, P, ctypes.c_uint32, ctypes.c_uint64,
+                                                       ctypes.c_uint32, P, P, P]),
+        "p2g_synchronize": (ctypes.c_int, [P]),
+        "p2g_run_create": (ctypes.c_int, [P, ctypes.c_char_p, ctypes.c_uint64, ctypes.c_uint32,
+                                          ctypes.POINTER(_Scheme), ctypes.c_uint32,
+                                          ctypes.POINTER(ctypes.c_char_p),
+                                          ctypes.POINTER(ctypes.c_char_p), u32p,
+                                          ctypes.c_char_p, ctypes.c_size_t]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+lib = _load()
+
+
+@dataclass
+class VolumeScheme:
+    """VolumeScheme, src/creator.rs:110-139 (Limited wraps the inner kind)."""
+
+    kind: int = SCHEME_SINGLE
+    uniform_count: int = 0
+    limited: bool = False
+    limit_cap: int = 0
+    explicit_sizes: List[int] = field(default_factory=list)
+
+    @staticmethod
+    def single() -> "VolumeScheme":
+        return VolumeScheme(SCHEME_SINGLE)
+
+    @staticmethod
+    def exponential() -> "VolumeScheme":
+        return VolumeScheme(SCHEME_EXPONENTIAL)
+
+    @staticmethod
+    def uniform(count: int) -> "VolumeScheme":
+        return VolumeScheme(SCHEME_UNIFORM, uniform_count=count)
+
+    @staticmethod
+    def explicit(sizes: Sequence[int]) -> "VolumeScheme":
+        return VolumeScheme(SCHEME_EXPLICIT, explicit_sizes=list(sizes))
+
+    def capped(self, cap: int) -> "VolumeScheme":
+        return VolumeScheme(self.kind, self.uniform_count, True, cap, list(self.explicit_sizes))
+
+    def _c(self):
+        arr = (ctypes.c_uint32 * max(1, len(self.explicit_sizes)))(*self.explicit_sizes)
+        s = _Scheme(self.kind, self.uniform_count, int(self.limited), self.limit_cap,
+                    ctypes.cast(arr, ctypes.POINTER(ctypes.c_uint32)) if self.explicit_sizes
+                    else None, len(self.explicit_sizes))
+        return s, arr
+
+
+@dataclass
+class CreateOptions:
+    """CreateOptions, src/creator.rs:142-173 (library default scheme: Single)."""
+
+    output: str
+    slice_size: int = 4096
+    recovery_block_count: int = 0
+    volume_scheme: VolumeScheme = field(default_factory=VolumeScheme.single)
+
+
+def _buf(data) -> Tuple[ctypes.c_void_p, int, object]:
+    """(pointer, length, keep-alive) for bytes / bytearray / numpy arrays."""
+    if isinstance(data, (bytes, bytearray)):
+        b = data if isinstance(data, bytearray) else bytearray(data)
+        c = (ctypes.c_char * max(1, len(b))).from_buffer(b) if len(b) else None
+        return (ctypes.cast(c, ctypes.c_void_p) if c is not None else None), len(b), (b, c)
+    import numpy as np
+    a = np.ascontiguousarray(data, dtype=np.uint8)
+    return ctypes.c_void_p(a.ctypes.data), a.nbytes, a
+
+
+class Context:
+    """One p2g context on one GPU (p2g_open / p2g_close)."""
+
+    def __init__(self, device: int = 0):
+        h = ctypes.c_void_p()
+        rc = lib.p2g_open(device, ctypes.byref(h))
+        if rc != P2G_OK:
+            raise Par2Error(rc, "p2g_open failed")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib.p2g_close(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc: int):
+        if rc != P2G_OK:
+            raise Par2Error(rc, lib.p2g_last_error(self.h).decode(errors="replace"))
+
+    # --- host-buffer entry points -------------------------------------
+    def slice_checksums(self, data, slice_size: int) -> Tuple[List[bytes], List[int]]:
+        p, n, keep = _buf(data)
+        count = (n + slice_size - 1) // slice_size if slice_size else 0
+        md5 = ctypes.create_string_buffer(16 * max(1, count))
+        crc = (ctypes.c_uint32 * max(1, count))()
+        self._check(lib.p2g_slice_checksums(self.h, p, n, slice_size, md5, crc))
+        return [md5.raw[16 * i:16 * i + 16] for i in range(count)], [crc[i] for i in range(count)]
+
+    def encode(self, files: Sequence, slice_size: int, recovery_count: int,
+               set_id: Optional[bytes] = None, push_bytes: int = 64 << 20):
+        """Recovery blocks (R * S bytes, exponent-major) of the slices of
+        `files` in order, through p2g_encode_begin / push / finish.  With
+        set_id, also returns the R recovery-packet MD5s."""
+        total = sum((len(f) + slice_size - 1) // slice_size for f in files)
+        job = ctypes.c_void_p()
+        self._check(lib.p2g_encode_begin(self.h, slice_size, total, recovery_count,
+                                         ctypes.byref(job)))
+        try:
+            idx = 0
+            chunk = max(1, push_bytes // slice_size) * slice_size
+            for f in files:
+                p, n, keep = _buf(f)
+                off = 0
+                while off < n:
+                    take = min(chunk, n - off)
+                    self._check(lib.p2g_encode_push(job, idx, ctypes.c_void_p(p.value + off),
+                                                    take))
+                    idx += (take + slice_size - 1) // slice_size
+                    off += take
+            rec = ctypes.create_string_buffer(max(1, recovery_count * slice_size))
+            md5 = ctypes.create_string_buffer(16 * max(1, recovery_count))
+            sid = ctypes.create_string_buffer(bytes(set_id), 16) if set_id is not None else None
+            self._check(lib.p2g_encode_finish(job, rec, sid, md5 if set_id is not None else None))
+        finally:
+            lib.p2g_encode_free(job)
+        out = rec.raw[:recovery_count * slice_size]
+        if set_id is None:
+            return out
+        return out, [md5.raw[16 * i:16 * i + 16] for i in range(recovery_count)]
+
+    def recovery_packet_md5(self, rec, count: int, slice_size: int, first_exponent: int,
+                            set_id: bytes) -> List[bytes]:
+        p, n, keep = _buf(rec)
+        out = ctypes.create_string_buffer(16 * max(1, count))
+        sid = ctypes.create_string_buffer(bytes(set_id), 16)
+        self._check(lib.p2g_recovery_packet_md5(self.h, p, count, slice_size, first_exponent,
+                                                sid, out))
+        return [out.raw[16 * i:16 * i + 16] for i in range(count)]
+
+    # --- device-buffer entry points (torch tensors on the GPU) -----------
+    def dev_encode(self, files, slice_size: int, recovery_count: int, rec,
+                   first_input: int = 0, total_inputs: Optional[int] = None,
+                   accumulate: bool = False, stream: Optional[int] = None):
+        arr = (_DevFile * max(1, len(files)))(*[_DevFile(t.data_ptr(), t.numel() * t.element_size())
+                                               for t in files])
+        if total_inputs is None:
+            total_inputs = first_input + sum(
+                (t.numel() * t.element_size() + slice_size - 1) // slice_size for t in files)
+        self._check(lib.p2g_dev_encode(self.h, arr, len(files), slice_size, first_input,
+                                       total_inputs, recovery_count,
+                                       ctypes.c_void_p(rec.data_ptr()), int(accumulate),
+                                       ctypes.c_void_p(stream) if stream else None))
+
+    def dev_slice_checksums(self, files, slice_size: int, md5_out, crc_out,
+                            stream: Optional[int] = None):
+        arr = (_DevFile * max(1, len(files)))(*[_DevFile(t.data_ptr(), t.numel() * t.element_size())
+                                               for t in files])
+        self._check(lib.p2g_dev_slice_checksums(self.h, arr, len(files), slice_size,
+                                                ctypes.c_void_p(md5_out.data_ptr()),
+                                                ctypes.c_void_p(crc_out.data_ptr()),
+                                                ctypes.c_void_p(stream) if stream else None))
+
+    def dev_recovery_packet_md5(self, rec, count: int, slice_size: int, first_exponent: int,
+                                set_id: bytes, md5_out, stream: Optional[int] = None):
+        sid = ctypes.create_string_buffer(bytes(set_id), 16)
+        self._check(lib.p2g_dev_recovery_packet_md5(self.h, ctypes.c_void_p(rec.data_ptr()),
+                                                    count, slice_size, first_exponent, sid,
+                                                    ctypes.c_void_p(md5_out.data_ptr()),
+                                                    ctypes.c_void_p(stream) if stream else None))
+
+    def synchronize(self):
+        self._check(lib.p2g_synchronize(self.h))
+
+    # --- whole pipeline --------------------------------------------------
+    def run_create(self, opts: CreateOptions, inputs: Sequence[Tuple[str, bytes]]) -> List[str]:
+        """SourceFile::scan of each (path, display name), then run_create
+        (src/creator.rs:182).  Returns the paths written, index first."""
+        scheme, keep = opts.volume_scheme._c()
+        n = len(inputs)
+        paths = (ctypes.c_char_p * max(1, n))(*[os.fsencode(p) for p, _ in inputs])
+        names = (ctypes.c_char_p * max(1, n))(*[bytes(nm) for _, nm in inputs])
+        lens = (ctypes.c_uint32 * max(1, n))(*[len(nm) for _, nm in inputs])
+        cap = 1 << 20
+        out = ctypes.create_string_buffer(cap)
+        rc = lib.p2g_run_create(self.h, os.fsencode(opts.output), opts.slice_size,
+                                opts.recovery_block_count, ctypes.byref(scheme), n, paths, names,
+                                lens, out, cap)
+        if rc != P2G_OK:
+            raise Par2Error(rc, lib.p2g_last_error(self.h).decode(errors="replace"))
+        return [s.decode() for s in out.raw.split(b"\0") if s][: 1 + 65536]
+
+
+def version() -> str:
+    return lib.p2g_version().decode()

@@ -1,0 +1,520 @@
+// p2g_abi.hip -- context, device memory and the extern "C" entry points of
+// include/par2gpu.h (p2g_run_create lives in par2_create.cpp).
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "p2g_context.hpp"
+#include "p2g_internal.hpp"
+
+using namespace p2g;
+
+namespace p2g {
+
+// ---------------------------------------------------------------------
+// GF(2^16) tables on the host: build_tables, src/galois.rs:33-53.
+// ---------------------------------------------------------------------
+static void build_gf_tables(std::vector<uint16_t> &log, std::vector<uint16_t> &antilog) {
+    log.assign(65536, 0);
+    antilog.assign(65536, 0);
+    uint32_t b = 1;
+    for (uint32_t l = 0; l < kFieldLimit; l++) {
+        log[b] = (uint16_t)l;
+        antilog[l] = (uint16_t)b;
+        b <<= 1;
+        if (b & 0x10000u) b ^= kFieldGenerator;
+    }
+    log[0] = (uint16_t)kFieldLimit;
+    antilog[kFieldLimit] = 0;
+}
+
+// Logs of the RS input bases: the non-negative integers coprime to 65535 in
+// increasing order, generate_input_bases, src/reedsolomon.rs:130-149
+// (gcd(65535, 0) = 65535, so 0 is not one of them).
+static std::vector<uint16_t> input_base_logs() {
+    std::vector<uint16_t> out;
+    out.reserve(kMaxInputs);
+    for (uint32_t l = 0; l < kFieldLimit && out.size() < kMaxInputs; l++) {
+        uint32_t a = kFieldLimit, b = l;
+        while (b) { uint32_t t = a % b; a = b; b = t; }
+        if (a == 1) out.push_back((uint16_t)l);
+    }
+    return out;
+}
+
+int Ctx::fail(int code, const char *what, hipError_t e) {
+    char buf[512];
+    if (e != hipSuccess)
+        snprintf(buf, sizeof buf, "%s: %s", what, hipGetErrorString(e));
+    else
+        snprintf(buf, sizeof buf, "%s", what);
+    err = buf;
+    return code;
+}
+
+int DevBuf::reserve(size_t n) {
+    if (n <= cap) return P2G_OK;
+    // Work queued earlier may still read the old allocation.
+    if (ptr) { hipDeviceSynchronize(); hipFree(ptr); }
+    ptr = nullptr;
+    cap = 0;
+    const size_t want = std::max<size_t>(n, 256);
+    if (hipMalloc(&ptr, want) != hipSuccess) { ptr = nullptr; return P2G_ERR_OUT_OF_MEMORY; }
+    cap = want;
+    return P2G_OK;
+}
+
+void DevBuf::release() {
+    if (ptr) hipFree(ptr);
+    ptr = nullptr;
+    cap = 0;
+}
+
+int PinBuf::reserve(size_t n) {
+    if (n <= cap) return P2G_OK;
+    // An asynchronous copy queued earlier may still read the old allocation.
+    if (ptr) { hipDeviceSynchronize(); hipHostFree(ptr); }
+    ptr = nullptr;
+    cap = 0;
+    const size_t want = std::max<size_t>(n, 256);
+    if (hipHostMalloc(&ptr, want, hipHostMallocDefault) != hipSuccess) {
+        ptr = nullptr;
+        return P2G_ERR_OUT_OF_MEMORY;
+    }
+    cap = want;
+    return P2G_OK;
+}
+
+void PinBuf::release() {
+    if (ptr) hipHostFree(ptr);
+    ptr = nullptr;
+    cap = 0;
+}
+
+int Ctx::open(int dev) {
+    device = dev;
+    hipError_t e = hipSetDevice(dev);
+    if (e != hipSuccess) return fail(P2G_ERR_DEVICE, "hipSetDevice", e);
+    e = hipStreamCreateWithFlags(&stream, hipStreamNonBlocking);
+    if (e != hipSuccess) return fail(P2G_ERR_DEVICE, "hipStreamCreate", e);
+    e = hipStreamCreateWithFlags(&side, hipStreamNonBlocking);
+    if (e != hipSuccess) return fail(P2G_ERR_DEVICE, "hipStreamCreate", e);
+    e = hipEventCreateWithFlags(&enc_descs.free, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&sum_descs.free, hipEventDisableTiming);
+    if (e != hipSuccess) return fail(P2G_ERR_DEVICE, "hipEventCreate", e);
+    build_gf_tables(h_log, h_antilog);
+    base_logs = input_base_logs();
+    if (d_antilog.reserve(65536 * 2) != P2G_OK) return fail(P2G_ERR_OUT_OF_MEMORY, "antilog table");
+    e = hipMemcpy(d_antilog.ptr, h_antilog.data(), 65536 * 2, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return fail(P2G_ERR_DEVICE, "upload antilog", e);
+    if (d_base_logs.reserve(base_logs.size() * 2) != P2G_OK)
+        return fail(P2G_ERR_OUT_OF_MEMORY, "base logs");
+    e = hipMemcpy(d_base_logs.ptr, base_logs.data(), base_logs.size() * 2, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return fail(P2G_ERR_DEVICE, "upload base logs", e);
+    return P2G_OK;
+}
+
+Ctx::~Ctx() {
+    hipSetDevice(device);
+    if (stream) hipStreamSynchronize(stream);
+    if (side) hipStreamSynchronize(side);
+    d_antilog.release();
+    d_base_logs.release();
+    d_mats.release();
+    d_md5.release();
+    d_crc.release();
+    d_stage.release();
+    for (DescSlot *sl : {&enc_descs, &sum_descs}) {
+        sl->h.release();
+        sl->d.release();
+        if (sl->free) hipEventDestroy(sl->free);
+    }
+    if (stream) hipStreamDestroy(stream);
+    if (side) hipStreamDestroy(side);
+}
+
+int check_slice_size(Ctx *c, uint64_t S) {
+    if (S == 0 || S % 4 != 0)
+        return c->fail(P2G_ERR_INVALID_SLICE_SIZE, "slice size must be > 0 and a multiple of 4");
+    if (S > kMaxSliceBytes)
+        return c->fail(P2G_ERR_INVALID_ARGUMENT, "slice size above the 2 GiB kernel limit");
+    return P2G_OK;
+}
+
+int append_descs(Ctx *c, const p2g_dev_file *files, uint32_t nfiles, uint64_t S,
+                 std::vector<SliceDesc> &out) {
+    for (uint32_t f = 0; f < nfiles; f++) {
+        if (files[f].len && !files[f].data)
+            return c->fail(P2G_ERR_INVALID_ARGUMENT, "null device file pointer");
+        if (reinterpret_cast<uintptr_t>(files[f].data) % 16 != 0)
+            return c->fail(P2G_ERR_INVALID_ARGUMENT, "device file pointers must be 16-byte aligned");
+        const uint64_t n = (files[f].len + S - 1) / S;
+        if (out.size() + n > 0x7FFFFFFFull)
+            return c->fail(P2G_ERR_INVALID_ARGUMENT, "too many slices in one call");
+        for (uint64_t s = 0; s < n; s++) {
+            const uint64_t have = files[f].len - s * S;
+            out.push_back({files[f].data + s * S, (uint32_t)(have < S ? have : S), 0});
+        }
+    }
+    return P2G_OK;
+}
+
+int upload_descs(Ctx *c, DescSlot &slot, const std::vector<SliceDesc> &descs, hipStream_t st) {
+    const size_t bytes = descs.size() * sizeof(SliceDesc);
+    if (bytes == 0) return P2G_OK;
+    // The previous upload out of slot.h must have been consumed first.
+    hipError_t e = hipEventSynchronize(slot.free);
+    if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "descriptor staging wait", e);
+    if (slot.h.reserve(bytes) != P2G_OK || slot.d.reserve(bytes) != P2G_OK)
+        return c->fail(P2G_ERR_OUT_OF_MEMORY, "descriptor buffers");
+    memcpy(slot.h.ptr, descs.data(), bytes);
+    e = hipMemcpyAsync(slot.d.ptr, slot.h.ptr, bytes, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipEventRecord(slot.free, st);
+    if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "upload slice descriptors", e);
+    return P2G_OK;
+}
+
+int dev_encode(Ctx *c, const SliceDesc *d_descs, uint32_t n, uint64_t S, uint32_t first_input,
+               uint32_t total_inputs, uint32_t R, uint8_t *d_rec, int accumulate,
+               hipStream_t st) {
+    if (R > P2G_MAX_RECOVERY_BLOCKS)
+        return c->fail(P2G_ERR_TOO_MANY_RECOVERY_BLOCKS, "recovery count above 65535");
+    if (total_inputs > kMaxInputs)
+        return c->fail(P2G_ERR_TOO_MANY_INPUT_BLOCKS, "more than 32768 input slices");
+    if ((uint64_t)first_input + n > total_inputs)
+        return c->fail(P2G_ERR_INVALID_ARGUMENT, "input index range exceeds total_inputs");
+    if (R == 0) return P2G_OK;
+    hipError_t e;
+    if (n == 0) {
+        if (!accumulate) {
+            e = hipMemsetAsync(d_rec, 0, (size_t)R * S, st);
+            if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "hipMemsetAsync", e);
+        }
+        return P2G_OK;
+    }
+    // Inputs go in chunks so the coefficient matrices (32 B per (r, i))
+    // stay within kMatsBudget.
+    uint64_t ch = kMatsBudget / ((uint64_t)R * 32);
+    ch = std::max<uint64_t>(1, std::min<uint64_t>(ch, n));
+    if (c->d_mats.reserve((size_t)ch * R * 32) != P2G_OK)
+        return c->fail(P2G_ERR_OUT_OF_MEMORY, "coefficient matrices");
+    const uint16_t *dlog = static_cast<const uint16_t *>(c->d_base_logs.ptr);
+    for (uint64_t c0 = 0; c0 < n; c0 += ch) {
+        const uint32_t cn = (uint32_t)std::min<uint64_t>(ch, n - c0);
+        e = launch_coeff_mats(static_cast<const uint16_t *>(c->d_antilog.ptr),
+                              dlog + first_input + c0, cn, R,
+                              static_cast<uint16_t *>(c->d_mats.ptr), st);
+        if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "coefficient kernel", e);
+        e = launch_encode(d_descs + c0, cn, static_cast<const uint16_t *>(c->d_mats.ptr), R,
+                          (uint32_t)S, d_rec, (c0 > 0 || accumulate) ? 1 : 0, st);
+        if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "encode kernel", e);
+    }
+    return P2G_OK;
+}
+
+}  // namespace p2g
+
+// =====================================================================
+// extern "C"
+// =====================================================================
+
+extern "C" {
+
+const char *p2g_version(void) { return "par2gpu 0.1.0 gfx950"; }
+
+int p2g_open(int device, p2g_ctx **out) {
+    if (!out) return P2G_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    Ctx *c = new (std::nothrow) Ctx();
+    if (!c) return P2G_ERR_OUT_OF_MEMORY;
+    int rc = c->open(device);
+    if (rc != P2G_OK) {
+        fprintf(stderr, "p2g_open: %s\n", c->err.c_str());
+        delete c;
+        return rc;
+    }
+    *out = reinterpret_cast<p2g_ctx *>(c);
+    return P2G_OK;
+}
+
+void p2g_close(p2g_ctx *ctx) { delete reinterpret_cast<Ctx *>(ctx); }
+
+const char *p2g_last_error(const p2g_ctx *ctx) {
+    return ctx ? reinterpret_cast<const Ctx *>(ctx)->err.c_str() : "null context";
+}
+
+int p2g_synchronize(p2g_ctx *ctx) {
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (!c) return P2G_ERR_INVALID_ARGUMENT;
+    hipSetDevice(c->device);
+    hipError_t e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->side);
+    if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "synchronize", e);
+    return P2G_OK;
+}
+
+int p2g_dev_encode(p2g_ctx *ctx, const p2g_dev_file *files, uint32_t nfiles, uint64_t slice_size,
+                   uint32_t first_input_index, uint32_t total_inputs, uint32_t recovery_count,
+                   uint8_t *rec, int accumulate, void *stream) {
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (!c) return P2G_ERR_INVALID_ARGUMENT;
+    if ((nfiles && !files) || (recovery_count && !rec))
+        return c->fail(P2G_ERR_INVALID_ARGUMENT, "null argument");
+    if (reinterpret_cast<uintptr_t>(rec) % 4 != 0)
+        return c->fail(P2G_ERR_INVALID_ARGUMENT, "recovery buffer must be 4-byte aligned");
+    int rc = check_slice_size(c, slice_size);
+    if (rc != P2G_OK) return rc;
+    hipSetDevice(c->device);
+    std::vector<SliceDesc> descs;
+    rc = append_descs(c, files, nfiles, slice_size, descs);
+    if (rc != P2G_OK) return rc;
+    if (descs.size() > kMaxInputs)
+        return c->fail(P2G_ERR_TOO_MANY_INPUT_BLOCKS, "more than 32768 input slices");
+    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c->stream;
+    rc = upload_descs(c, c->enc_descs, descs, st);
+    if (rc != P2G_OK) return rc;
+    return dev_encode(c, static_cast<const SliceDesc *>(c->enc_descs.d.ptr), (uint32_t)descs.size(),
+                      slice_size, first_input_index, total_inputs, recovery_count, rec,
+                      accumulate, st);
+}
+
+int p2g_dev_slice_checksums(p2g_ctx *ctx, const p2g_dev_file *files, uint32_t nfiles,
+                            uint64_t slice_size, uint8_t *md5_out, uint32_t *crc_out,
+                            void *stream) {
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (!c) return P2G_ERR_INVALID_ARGUMENT;
+    if ((nfiles && !files) || !md5_out || !crc_out)
+        return c->fail(P2G_ERR_INVALID_ARGUMENT, "null argument");
+    if (reinterpret_cast<uintptr_t>(md5_out) % 4 != 0 || reinterpret_cast<uintptr_t>(crc_out) % 4 != 0)
+        return c->fail(P2G_ERR_INVALID_ARGUMENT, "checksum outputs must be 4-byte aligned");
+    int rc = check_slice_size(c, slice_size);
+    if (rc != P2G_OK) return rc;
+    hipSetDevice(c->device);
+    std::vector<SliceDesc> descs;
+    rc = append_descs(c, files, nfiles, slice_size, descs);
+    if (rc != P2G_OK) return rc;
+    if (descs.empty()) return P2G_OK;
+    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c->stream;
+    rc = upload_descs(c, c->sum_descs, descs, st);
+    if (rc != P2G_OK) return rc;
+    const SliceDesc *dd = static_cast<const SliceDesc *>(c->sum_descs.d.ptr);
+    const uint32_t n = (uint32_t)descs.size();
+    hipError_t e = launch_slice_md5(dd, n, (uint32_t)slice_size, md5_out, st);
+    if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "slice md5 kernel", e);
+    e = launch_slice_crc32(dd, n, (uint32_t)slice_size, crc_out, st);
+    if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "slice crc kernel", e);
+    return P2G_OK;
+}
+
+int p2g_dev_recovery_packet_md5(p2g_ctx *ctx, const uint8_t *rec, uint32_t count,
+                                uint64_t slice_size, uint32_t first_exponent,
+                                const uint8_t set_id[16], uint8_t *pkt_md5_out, void *stream) {
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (!c) return P2G_ERR_INVALID_ARGUMENT;
+    if (count && (!rec || !set_id || !pkt_md5_out))
+        return c->fail(P2G_ERR_INVALID_ARGUMENT, "null argument");
+    if (reinterpret_cast<uintptr_t>(rec) % 4 != 0 || reinterpret_cast<uintptr_t>(pkt_md5_out) % 4 != 0)
+        return c->fail(P2G_ERR_INVALID_ARGUMENT, "recovery blocks and digests must be 4-byte aligned");
+    int rc = check_slice_size(c, slice_size);
+    if (rc != P2G_OK) return rc;
+    if ((uint64_t)first_exponent + count > 65536)
+        return c->fail(P2G_ERR_TOO_MANY_RECOVERY_BLOCKS, "recovery exponent above 65535");
+    hipSetDevice(c->device);
+    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c->stream;
+    hipError_t e = launch_recovery_md5(rec, count, (uint32_t)slice_size, first_exponent, set_id,
+                                       pkt_md5_out, st);
+    if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "recovery md5 kernel", e);
+    return P2G_OK;
+}
+
+// ---- host-buffer entry points ----------------------------------------
+
+int p2g_slice_checksums(p2g_ctx *ctx, const uint8_t *data, uint64_t len, uint64_t slice_size,
+                        uint8_t *md5_out, uint32_t *crc_out) {
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (!c) return P2G_ERR_INVALID_ARGUMENT;
+    if (len && (!data || !md5_out || !crc_out))
+        return c->fail(P2G_ERR_INVALID_ARGUMENT, "null argument");
+    int rc = check_slice_size(c, slice_size);
+    if (rc != P2G_OK) return rc;
+    if (len == 0) return P2G_OK;
+    hipSetDevice(c->device);
+    const uint64_t n = (len + slice_size - 1) / slice_size;
+    if (c->d_stage.reserve(len) != P2G_OK || c->d_md5.reserve(n * 16) != P2G_OK ||
+        c->d_crc.reserve(n * 4) != P2G_OK)
+        return c->fail(P2G_ERR_OUT_OF_MEMORY, "checksum buffers");
+    hipError_t e = hipMemcpyAsync(c->d_stage.ptr, data, len, hipMemcpyHostToDevice, c->stream);
+    if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "upload file bytes", e);
+    p2g_dev_file f{static_cast<const uint8_t *>(c->d_stage.ptr), len};
+    rc = p2g_dev_slice_checksums(ctx, &f, 1, slice_size, static_cast<uint8_t *>(c->d_md5.ptr),
+                                 static_cast<uint32_t *>(c->d_crc.ptr), c->stream);
+    if (rc != P2G_OK) return rc;
+    e = hipMemcpyAsync(md5_out, c->d_md5.ptr, n * 16, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(crc_out, c->d_crc.ptr, n * 4, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "slice checksums", e);
+    return P2G_OK;
+}
+
+int p2g_recovery_packet_md5(p2g_ctx *ctx, const uint8_t *rec, uint32_t count, uint64_t slice_size,
+                            uint32_t first_exponent, const uint8_t set_id[16],
+                            uint8_t *pkt_md5_out) {
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (!c) return P2G_ERR_INVALID_ARGUMENT;
+    if (count && (!rec || !set_id || !pkt_md5_out))
+        return c->fail(P2G_ERR_INVALID_ARGUMENT, "null argument");
+    int rc = check_slice_size(c, slice_size);
+    if (rc != P2G_OK) return rc;
+    if (count == 0) return P2G_OK;
+    hipSetDevice(c->device);
+    const size_t bytes = (size_t)count * slice_size;
+    if (c->d_stage.reserve(bytes) != P2G_OK || c->d_md5.reserve((size_t)count * 16) != P2G_OK)
+        return c->fail(P2G_ERR_OUT_OF_MEMORY, "packet md5 buffers");
+    hipError_t e = hipMemcpyAsync(c->d_stage.ptr, rec, bytes, hipMemcpyHostToDevice, c->stream);
+    if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "upload recovery blocks", e);
+    rc = p2g_dev_recovery_packet_md5(ctx, static_cast<const uint8_t *>(c->d_stage.ptr), count,
+                                     slice_size, first_exponent, set_id,
+                                     static_cast<uint8_t *>(c->d_md5.ptr), c->stream);
+    if (rc != P2G_OK) return rc;
+    e = hipMemcpyAsync(pkt_md5_out, c->d_md5.ptr, (size_t)count * 16, hipMemcpyDeviceToHost,
+                       c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "recovery packet md5", e);
+    return P2G_OK;
+}
+
+// ---- streaming encode job ----------------------------------------------
+
+int p2g_encode_begin(p2g_ctx *ctx, uint64_t slice_size, uint32_t total_inputs,
+                     uint32_t recovery_count, p2g_job **out) {
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (!c || !out) return P2G_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    int rc = check_slice_size(c, slice_size);
+    if (rc != P2G_OK) return rc;
+    if (recovery_count > P2G_MAX_RECOVERY_BLOCKS)
+        return c->fail(P2G_ERR_TOO_MANY_RECOVERY_BLOCKS, "recovery count above 65535");
+    if (total_inputs > kMaxInputs)
+        return c->fail(P2G_ERR_TOO_MANY_INPUT_BLOCKS, "more than 32768 input slices");
+    hipSetDevice(c->device);
+    Job *j = new (std::nothrow) Job();
+    if (!j) return P2G_ERR_OUT_OF_MEMORY;
+    j->ctx = c;
+    j->S = slice_size;
+    j->N = total_inputs;
+    j->R = recovery_count;
+    const size_t rec_bytes = (size_t)recovery_count * slice_size;
+    const size_t desc_bytes = (size_t)std::max<uint32_t>(total_inputs, 1) * sizeof(SliceDesc);
+    if ((rec_bytes && j->d_rec.reserve(rec_bytes) != P2G_OK) ||
+        j->d_descs.reserve(desc_bytes) != P2G_OK || j->h_descs.reserve(desc_bytes) != P2G_OK) {
+        p2g_encode_free(reinterpret_cast<p2g_job *>(j));
+        return c->fail(P2G_ERR_OUT_OF_MEMORY, "encode job buffers");
+    }
+    if (rec_bytes) {
+        hipError_t e = hipMemsetAsync(j->d_rec.ptr, 0, rec_bytes, c->stream);
+        if (e != hipSuccess) {
+            p2g_encode_free(reinterpret_cast<p2g_job *>(j));
+            return c->fail(P2G_ERR_DEVICE, "clear accumulator", e);
+        }
+    }
+    *out = reinterpret_cast<p2g_job *>(j);
+    return P2G_OK;
+}
+
+int p2g_encode_push(p2g_job *job, uint32_t first_input_index, const uint8_t *data, uint64_t len) {
+    Job *j = reinterpret_cast<Job *>(job);
+    if (!j) return P2G_ERR_INVALID_ARGUMENT;
+    Ctx *c = j->ctx;
+    if (len && !data) return c->fail(P2G_ERR_INVALID_ARGUMENT, "null argument");
+    if (len == 0 || j->R == 0) return P2G_OK;
+    hipSetDevice(c->device);
+    const uint64_t n = (len + j->S - 1) / j->S;
+    if ((uint64_t)first_input_index + n > j->N)
+        return c->fail(P2G_ERR_INVALID_ARGUMENT, "pushed slices exceed total_inputs");
+    // Double-buffered staging: this push's copy runs on the side stream while
+    // the previous push's encode runs on the main stream.  A staging buffer
+    // is reused only after the encode that read it has finished, and the
+    // call returns once `data` has been copied, so the caller may reuse it.
+    Job::Stage &sg = j->stage[j->next_stage];
+    j->next_stage ^= 1;
+    hipError_t e;
+    if (sg.done) {
+        e = hipEventSynchronize(sg.done);
+        if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "staging wait", e);
+    } else {
+        e = hipEventCreateWithFlags(&sg.done, hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&sg.copied, hipEventDisableTiming);
+        if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "event create", e);
+    }
+    if (sg.buf.reserve(len) != P2G_OK) return c->fail(P2G_ERR_OUT_OF_MEMORY, "staging buffer");
+    e = hipMemcpyAsync(sg.buf.ptr, data, len, hipMemcpyHostToDevice, c->side);
+    if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "upload slices", e);
+    // Descriptors of this push go to their own range of the job's tables,
+    // which no other push touches.
+    SliceDesc *hd = static_cast<SliceDesc *>(j->h_descs.ptr) + first_input_index;
+    const uint8_t *base = static_cast<const uint8_t *>(sg.buf.ptr);
+    for (uint64_t s = 0; s < n; s++) {
+        const uint64_t have = len - s * j->S;
+        hd[s] = {base + s * j->S, (uint32_t)(have < j->S ? have : j->S), 0};
+    }
+    SliceDesc *dd = static_cast<SliceDesc *>(j->d_descs.ptr) + first_input_index;
+    e = hipMemcpyAsync(dd, hd, n * sizeof(SliceDesc), hipMemcpyHostToDevice, c->side);
+    if (e == hipSuccess) e = hipEventRecord(sg.copied, c->side);
+    if (e == hipSuccess) e = hipStreamWaitEvent(c->stream, sg.copied, 0);
+    if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "upload slice descriptors", e);
+    int rc = dev_encode(c, dd, (uint32_t)n, j->S, first_input_index, j->N, j->R,
+                        static_cast<uint8_t *>(j->d_rec.ptr), 1, c->stream);
+    if (rc != P2G_OK) return rc;
+    e = hipEventRecord(sg.done, c->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(sg.copied);
+    if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "event record", e);
+    return P2G_OK;
+}
+
+int p2g_encode_finish(p2g_job *job, uint8_t *rec_out, const uint8_t *set_id, uint8_t *pkt_md5_out) {
+    Job *j = reinterpret_cast<Job *>(job);
+    if (!j) return P2G_ERR_INVALID_ARGUMENT;
+    Ctx *c = j->ctx;
+    if (j->R == 0) return P2G_OK;
+    if (!rec_out || (set_id && !pkt_md5_out))
+        return c->fail(P2G_ERR_INVALID_ARGUMENT, "null argument");
+    hipSetDevice(c->device);
+    const size_t bytes = (size_t)j->R * j->S;
+    hipError_t e;
+    if (set_id) {
+        if (c->d_md5.reserve((size_t)j->R * 16) != P2G_OK)
+            return c->fail(P2G_ERR_OUT_OF_MEMORY, "packet md5 buffer");
+        e = launch_recovery_md5(static_cast<const uint8_t *>(j->d_rec.ptr), j->R, (uint32_t)j->S,
+                                0, set_id, static_cast<uint8_t *>(c->d_md5.ptr), c->stream);
+        if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "recovery md5 kernel", e);
+    }
+    e = hipMemcpyAsync(rec_out, j->d_rec.ptr, bytes, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess && set_id)
+        e = hipMemcpyAsync(pkt_md5_out, c->d_md5.ptr, (size_t)j->R * 16, hipMemcpyDeviceToHost,
+                           c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "encode finish", e);
+    return P2G_OK;
+}
+
+void p2g_encode_free(p2g_job *job) {
+    Job *j = reinterpret_cast<Job *>(job);
+    if (!j) return;
+    hipSetDevice(j->ctx->device);
+    hipStreamSynchronize(j->ctx->stream);
+    for (auto &sg : j->stage) {
+        if (sg.done) hipEventDestroy(sg.done);
+        if (sg.copied) hipEventDestroy(sg.copied);
+        sg.buf.release();
+    }
+    j->d_rec.release();
+    j->d_descs.release();
+    j->h_descs.release();
+    delete j;
+}
+
+}  // extern "C"

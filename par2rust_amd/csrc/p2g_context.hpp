@@ -1,0 +1,92 @@
+// p2g_context.hpp -- the state behind p2g_ctx and p2g_job.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "p2g_internal.hpp"
+
+namespace p2g {
+
+constexpr uint32_t kMaxInputs = 32768;            // MAX_INPUT_BLOCKS, src/creator.rs:106
+constexpr uint64_t kMaxSliceBytes = 1ull << 31;   // kernels index slices with u32
+constexpr uint64_t kMatsBudget = 256ull << 20;    // coefficient matrices per launch chunk
+
+// Grow-only device allocation.
+struct DevBuf {
+    void *ptr = nullptr;
+    size_t cap = 0;
+    int reserve(size_t n);
+    void release();
+};
+
+// Grow-only pinned host allocation.
+struct PinBuf {
+    void *ptr = nullptr;
+    size_t cap = 0;
+    int reserve(size_t n);
+    void release();
+};
+
+// Slice descriptors staged through pinned memory into a device table.  One
+// slot per kind of launch, so launches on different streams never share one.
+struct DescSlot {
+    PinBuf h;
+    DevBuf d;
+    hipEvent_t free = nullptr;  // `h` may be rewritten once this fires
+};
+
+struct Ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;  // kernels and copies of this context
+    hipStream_t side = nullptr;    // work overlapped with `stream`
+    std::string err;
+
+    std::vector<uint16_t> h_log, h_antilog;
+    std::vector<uint16_t> base_logs;  // L_i of every possible input (32768)
+
+    DevBuf d_antilog, d_base_logs;
+    DevBuf d_mats, d_md5, d_crc, d_stage;
+    DescSlot enc_descs;  // encode launches
+    DescSlot sum_descs;  // slice checksum launches
+
+    int open(int dev);
+    ~Ctx();
+    int fail(int code, const char *what, hipError_t e = hipSuccess);
+};
+
+struct Job {
+    Ctx *ctx = nullptr;
+    uint64_t S = 0;
+    uint32_t N = 0, R = 0;
+    DevBuf d_rec;
+    DevBuf d_descs;   // one descriptor per global input index
+    PinBuf h_descs;
+    struct Stage {
+        DevBuf buf;
+        hipEvent_t done = nullptr;    // the encode that read `buf` has finished
+        hipEvent_t copied = nullptr;  // host bytes and descriptors are on the device
+    } stage[2];
+    int next_stage = 0;
+};
+
+int check_slice_size(Ctx *c, uint64_t S);
+
+// Appends the descriptors of every slice of `files` (file order, slice order).
+int append_descs(Ctx *c, const p2g_dev_file *files, uint32_t nfiles, uint64_t S,
+                 std::vector<SliceDesc> &out);
+
+// Copies descriptors into a slot's device table through its pinned staging
+// buffer, in stream order.
+int upload_descs(Ctx *c, DescSlot &slot, const std::vector<SliceDesc> &descs, hipStream_t st);
+
+// rec (^)= encode of the n slices described by d_descs (device), whose global
+// input indices start at first_input.
+int dev_encode(Ctx *c, const SliceDesc *d_descs, uint32_t n, uint64_t S, uint32_t first_input,
+               uint32_t total_inputs, uint32_t R, uint8_t *d_rec, int accumulate,
+               hipStream_t st);
+
+}  // namespace p2g

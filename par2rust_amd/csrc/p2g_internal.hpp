@@ -1,0 +1,57 @@
+// p2g_internal.hpp -- types shared by the kernels, the C ABI and the host
+// pipeline of libpar2gpu.so.  Not part of the public ABI (include/par2gpu.h).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "../../include/par2gpu.h"
+
+namespace p2g {
+
+// One input slice as the encode and checksum kernels see it: `ptr` points at
+// the slice's first byte on the device, bytes [valid, slice_size) read as
+// zero (the zero padding of a file's trailing partial slice,
+// src/creator.rs:79-85 and src/source.rs:174-183).
+struct SliceDesc {
+    const uint8_t *ptr;
+    uint32_t valid;
+    uint32_t pad_;
+};
+static_assert(sizeof(SliceDesc) == 16, "SliceDesc must stay 16 bytes");
+
+// GF(2^16) field constants, src/galois.rs:15-18.
+constexpr uint32_t kFieldLimit = 65535u;
+constexpr uint32_t kFieldGenerator = 0x1100Bu;
+
+// Launch wrappers implemented in p2g_kernels.hip.  All are asynchronous on
+// `stream` and return hipSuccess or the launch error.
+
+// mats[(i * R + r) * 16 + j] = row j of the 16x16 GF(2) matrix of
+// multiplication by coef(r, first_input + i) = base^r, base = antilog[L].
+hipError_t launch_coeff_mats(const uint16_t *d_antilog, const uint16_t *d_logbase,
+                             uint32_t n_inputs, uint32_t R, uint16_t *d_mats,
+                             hipStream_t stream);
+
+// rec[r][*] (^)= XOR_i coef(r, i) * slice_i over the n_inputs descriptors.
+hipError_t launch_encode(const SliceDesc *d_inputs, uint32_t n_inputs, const uint16_t *d_mats,
+                         uint32_t R, uint32_t S, uint8_t *d_rec, int accumulate,
+                         hipStream_t stream);
+
+// MD5 of each zero-padded S-byte slice (16 bytes per slice into d_md5).
+hipError_t launch_slice_md5(const SliceDesc *d_slices, uint32_t n, uint32_t S,
+                            uint8_t *d_md5, hipStream_t stream);
+
+// CRC-32 of each zero-padded S-byte slice.
+hipError_t launch_slice_crc32(const SliceDesc *d_slices, uint32_t n, uint32_t S,
+                              uint32_t *d_crc, hipStream_t stream);
+
+// Packet MD5 of recovery packets r = 0..count-1: MD5(set_id || "PAR 2.0\0RecvSlic"
+// || le32(first_exp + r) || rec[r * S .. r * S + S)).
+hipError_t launch_recovery_md5(const uint8_t *d_rec, uint32_t count, uint32_t S,
+                               uint32_t first_exp, const uint8_t set_id[16],
+                               uint8_t *d_md5, hipStream_t stream);
+
+}  // namespace p2g

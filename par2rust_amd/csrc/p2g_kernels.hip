@@ -1,0 +1,449 @@
+// p2g_kernels.hip -- gfx950 kernels of the PAR2 create path.
+//
+//   k_coeff_mats     RS coefficients base_i^r from the log/antilog tables
+//                    (src/reedsolomon.rs:56-60, src/galois.rs:74-87) expanded
+//                    to the 16x16 GF(2) matrix of multiplication by each one
+//   k_encode_bits    GF(2^16) recovery accumulator (src/creator.rs:934-995)
+//                    on bit-sliced symbols
+//   k_slice_md5      IFSC slice MD5, one lane per slice (src/source.rs:168-183)
+//   k_slice_crc32    IFSC slice CRC-32, one wave per slice
+//   k_recovery_md5   recovery packet hash, one lane per packet
+//                    (src/format.rs:40-44 via src/packet/recovery.rs:11-19)
+#include "p2g_internal.hpp"
+
+namespace p2g {
+
+// ======================================================================
+// Coefficient matrices
+// ======================================================================
+
+// Thread per (input i, recovery r).  coef = antilog[(L_i * r) mod 65535]
+// equals gf_pow(base_i, r) for base_i = antilog[L_i] (r = 0 gives
+// antilog[0] = 1).  Column k of the matrix is coef * x^k; row j collects
+// bit j of every column, so out_bit_j = parity(row_j & in_symbol).
+__global__ __launch_bounds__(256) void k_coeff_mats(const uint16_t *__restrict__ antilog,
+                                                    const uint16_t *__restrict__ logbase,
+                                                    uint32_t n, uint32_t R,
+                                                    uint16_t *__restrict__ mats) {
+    const uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (uint64_t)n * R) return;
+    const uint32_t i = (uint32_t)(idx / R), r = (uint32_t)(idx % R);
+    const uint32_t e = (uint32_t)(((uint64_t)logbase[i] * r) % kFieldLimit);
+    uint32_t v = antilog[e];
+    uint32_t col[16];
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+        col[k] = v;
+        v = ((v << 1) & 0xFFFFu) ^ ((v & 0x8000u) ? (kFieldGenerator & 0xFFFFu) : 0u);
+    }
+    uint16_t *out = mats + idx * 16;
+#pragma unroll
+    for (int j = 0; j < 16; j++) {
+        uint32_t row = 0;
+#pragma unroll
+        for (int k = 0; k < 16; k++) row |= ((col[k] >> j) & 1u) << k;
+        out[j] = (uint16_t)row;
+    }
+}
+
+hipError_t launch_coeff_mats(const uint16_t *d_antilog, const uint16_t *d_logbase,
+                             uint32_t n_inputs, uint32_t R, uint16_t *d_mats,
+                             hipStream_t stream) {
+    const uint64_t total = (uint64_t)n_inputs * R;
+    if (total == 0) return hipSuccess;
+    const uint64_t blocks = (total + 255) / 256;
+    hipLaunchKernelGGL(k_coeff_mats, dim3((uint32_t)blocks), dim3(256), 0, stream, d_antilog,
+                       d_logbase, n_inputs, R, d_mats);
+    return hipGetLastError();
+}
+
+// ======================================================================
+// Bounded loads: bytes at or past `valid` read as zero.
+// ======================================================================
+
+__device__ __forceinline__ uint32_t load_word(const uint8_t *p, uint32_t off, uint32_t valid) {
+    if (off + 4 <= valid) return *reinterpret_cast<const uint32_t *>(p + off);
+    uint32_t w = 0;
+    for (uint32_t b = 0; b < 4; b++)
+        if (off + b < valid) w |= (uint32_t)p[off + b] << (8 * b);
+    return w;
+}
+
+// ======================================================================
+// K1: GF(2^16) accumulator on bit-sliced symbols
+// ======================================================================
+//
+// A lane owns 32 consecutive symbols (64 bytes) of the slice column range.
+// Its 16 input words are transposed into 16 bit planes (plane b = bit b of
+// the 32 symbols), so multiplication by a coefficient becomes 16 output
+// planes, each the XOR of the input planes selected by one matrix row.
+// RT recovery rows stay in registers while the inputs stream past.
+
+// Transposes the two 16x16 bit matrices held in the low and high halves of
+// 16 words (row q = word q, column b = bit b).  Its own inverse.
+__device__ __forceinline__ void transpose16(uint32_t w[16]) {
+#pragma unroll
+    for (int st = 0; st < 4; st++) {
+        const int k = 8 >> st;
+        const uint32_t m = (k == 8) ? 0x00FF00FFu : (k == 4) ? 0x0F0F0F0Fu
+                         : (k == 2) ? 0x33333333u : 0x55555555u;
+#pragma unroll
+        for (int q = 0; q < 16; q++) {
+            if ((q & k) == 0) {
+                const uint32_t t = ((w[q] >> k) ^ w[q + k]) & m;
+                w[q + k] ^= t;
+                w[q] ^= t << k;
+            }
+        }
+    }
+}
+
+constexpr int kEncThreads = 256;
+constexpr int kEncRT = 8;
+
+template <bool VEC>
+__global__ __launch_bounds__(kEncThreads) void k_encode_bits(
+    const SliceDesc *__restrict__ inputs, uint32_t n_inputs, const uint16_t *__restrict__ mats,
+    uint32_t R, uint32_t S, uint8_t *__restrict__ rec, int accumulate) {
+    const uint32_t col = (blockIdx.x * kEncThreads + threadIdx.x) * 64u;
+    if (col >= S) return;
+    const uint32_t r0 = blockIdx.y * kEncRT;
+    const uint32_t rt = (R - r0) < (uint32_t)kEncRT ? (R - r0) : (uint32_t)kEncRT;
+
+    uint32_t acc[kEncRT][16];
+#pragma unroll
+    for (int rr = 0; rr < kEncRT; rr++)
+#pragma unroll
+        for (int j = 0; j < 16; j++) acc[rr][j] = 0;
+
+    for (uint32_t i = 0; i < n_inputs; i++) {
+        const SliceDesc d = inputs[i];
+        uint32_t x[16];
+        if (VEC && col + 64 <= d.valid) {
+            const uint4 *p = reinterpret_cast<const uint4 *>(d.ptr + col);
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const uint4 v = p[q];
+                x[4 * q] = v.x; x[4 * q + 1] = v.y; x[4 * q + 2] = v.z; x[4 * q + 3] = v.w;
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < 16; q++) {
+                const uint32_t off = col + 4u * q;
+                x[q] = (off < S) ? load_word(d.ptr, off, d.valid) : 0u;
+            }
+        }
+        transpose16(x);
+        const uint16_t *m = mats + (size_t)i * R * 16 + (size_t)r0 * 16;
+#pragma unroll
+        for (int rr = 0; rr < kEncRT; rr++) {
+            if ((uint32_t)rr < rt) {
+#pragma unroll
+                for (int j = 0; j < 16; j++) {
+                    const uint32_t row = m[rr * 16 + j];
+                    uint32_t a = acc[rr][j];
+#pragma unroll
+                    for (int k = 0; k < 16; k++) a ^= x[k] & (0u - ((row >> k) & 1u));
+                    acc[rr][j] = a;
+                }
+            }
+        }
+    }
+
+#pragma unroll
+    for (int rr = 0; rr < kEncRT; rr++) {
+        if ((uint32_t)rr >= rt) continue;
+        transpose16(acc[rr]);
+        uint8_t *out = rec + (size_t)(r0 + rr) * S + col;
+        if (VEC && col + 64 <= S) {
+            uint4 *o = reinterpret_cast<uint4 *>(out);
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                uint4 v = make_uint4(acc[rr][4 * q], acc[rr][4 * q + 1], acc[rr][4 * q + 2],
+                                     acc[rr][4 * q + 3]);
+                if (accumulate) {
+                    const uint4 old = o[q];
+                    v.x ^= old.x; v.y ^= old.y; v.z ^= old.z; v.w ^= old.w;
+                }
+                o[q] = v;
+            }
+        } else {
+            uint32_t *o = reinterpret_cast<uint32_t *>(out);
+#pragma unroll
+            for (int q = 0; q < 16; q++) {
+                if (col + 4u * q < S) o[q] = accumulate ? (o[q] ^ acc[rr][q]) : acc[rr][q];
+            }
+        }
+    }
+}
+
+hipError_t launch_encode(const SliceDesc *d_inputs, uint32_t n_inputs, const uint16_t *d_mats,
+                         uint32_t R, uint32_t S, uint8_t *d_rec, int accumulate,
+                         hipStream_t stream) {
+    if (R == 0) return hipSuccess;
+    const uint32_t per_block = kEncThreads * 64u;
+    dim3 grid((S + per_block - 1) / per_block, (R + kEncRT - 1) / kEncRT);
+    // 128-bit loads and stores need 16-byte aligned slices and output rows;
+    // slice pointers are checked by the caller, the output base here.
+    if (S % 16 == 0 && reinterpret_cast<uintptr_t>(d_rec) % 16 == 0)
+        hipLaunchKernelGGL(k_encode_bits<true>, grid, dim3(kEncThreads), 0, stream, d_inputs,
+                           n_inputs, d_mats, R, S, d_rec, accumulate);
+    else
+        hipLaunchKernelGGL(k_encode_bits<false>, grid, dim3(kEncThreads), 0, stream, d_inputs,
+                           n_inputs, d_mats, R, S, d_rec, accumulate);
+    return hipGetLastError();
+}
+
+// ======================================================================
+// MD5 (RFC 1321)
+// ======================================================================
+
+__constant__ uint32_t kMd5K[64] = {
+    0xd76aa478u, 0xe8c7b756u, 0x242070dbu, 0xc1bdceeeu, 0xf57c0fafu, 0x4787c62au, 0xa8304613u, 0xfd469501u,
+    0x698098d8u, 0x8b44f7afu, 0xffff5bb1u, 0x895cd7beu, 0x6b901122u, 0xfd987193u, 0xa679438eu, 0x49b40821u,
+    0xf61e2562u, 0xc040b340u, 0x265e5a51u, 0xe9b6c7aau, 0xd62f105du, 0x02441453u, 0xd8a1e681u, 0xe7d3fbc8u,
+    0x21e1cde6u, 0xc33707d6u, 0xf4d50d87u, 0x455a14edu, 0xa9e3e905u, 0xfcefa3f8u, 0x676f02d9u, 0x8d2a4c8au,
+    0xfffa3942u, 0x8771f681u, 0x6d9d6122u, 0xfde5380cu, 0xa4beea44u, 0x4bdecfa9u, 0xf6bb4b60u, 0xbebfbc70u,
+    0x289b7ec6u, 0xeaa127fau, 0xd4ef3085u, 0x04881d05u, 0xd9d4d039u, 0xe6db99e5u, 0x1fa27cf8u, 0xc4ac5665u,
+    0xf4292244u, 0x432aff97u, 0xab9423a7u, 0xfc93a039u, 0x655b59c3u, 0x8f0ccc92u, 0xffeff47du, 0x85845dd1u,
+    0x6fa87e4fu, 0xfe2ce6e0u, 0xa3014314u, 0x4e0811a1u, 0xf7537e82u, 0xbd3af235u, 0x2ad7d2bbu, 0xeb86d391u};
+
+__device__ __forceinline__ uint32_t rotl32(uint32_t x, int s) { return (x << s) | (x >> (32 - s)); }
+
+__device__ __forceinline__ void md5_compress(uint32_t h[4], const uint32_t x[16]) {
+    constexpr int sh[16] = {7, 12, 17, 22, 5, 9, 14, 20, 4, 11, 16, 23, 6, 10, 15, 21};
+    uint32_t a = h[0], b = h[1], c = h[2], d = h[3];
+#pragma unroll
+    for (int i = 0; i < 64; i++) {
+        uint32_t f;
+        int g;
+        if (i < 16) { f = (b & c) | (~b & d); g = i; }
+        else if (i < 32) { f = (b & d) | (c & ~d); g = (5 * i + 1) & 15; }
+        else if (i < 48) { f = b ^ c ^ d; g = (3 * i + 5) & 15; }
+        else { f = c ^ (b | ~d); g = (7 * i) & 15; }
+        const uint32_t t = d;
+        d = c;
+        c = b;
+        b = b + rotl32(a + f + kMd5K[i] + x[g], sh[(i >> 4) * 4 + (i & 3)]);
+        a = t;
+    }
+    h[0] += a; h[1] += b; h[2] += c; h[3] += d;
+}
+
+__device__ __forceinline__ void md5_init(uint32_t h[4]) {
+    h[0] = 0x67452301u; h[1] = 0xefcdab89u; h[2] = 0x98badcfeu; h[3] = 0x10325476u;
+}
+
+__device__ __forceinline__ void store_digest(uint8_t *out, const uint32_t h[4]) {
+    uint32_t *o = reinterpret_cast<uint32_t *>(out);  // 16-byte aligned slot
+#pragma unroll
+    for (int i = 0; i < 4; i++) o[i] = h[i];
+}
+
+// Number of 64-byte MD5 blocks for a message of `len` bytes (len % 4 == 0).
+__device__ __forceinline__ uint32_t md5_blocks(uint64_t len) { return (uint32_t)((len + 8) / 64 + 1); }
+
+// Padding word at byte offset o >= len of a len-byte message (len % 4 == 0)
+// inside block b of nb.
+__device__ __forceinline__ uint32_t md5_pad_word(uint64_t o, uint64_t len, uint32_t w,
+                                                 uint32_t b, uint32_t nb) {
+    if (o == len) return 0x80u;
+    if (b == nb - 1 && w == 14) return (uint32_t)(len * 8);
+    if (b == nb - 1 && w == 15) return (uint32_t)((len * 8) >> 32);
+    return 0u;
+}
+
+// ======================================================================
+// K2a: slice MD5, one lane per slice
+// ======================================================================
+
+__global__ __launch_bounds__(64) void k_slice_md5(const SliceDesc *__restrict__ slices, uint32_t n,
+                                                  uint32_t S, uint8_t *__restrict__ md5_out) {
+    const uint32_t s = blockIdx.x * 64 + threadIdx.x;
+    if (s >= n) return;
+    const SliceDesc d = slices[s];
+    const uint32_t nb = md5_blocks(S);
+    uint32_t h[4];
+    md5_init(h);
+    for (uint32_t b = 0; b < nb; b++) {
+        const uint32_t base = b * 64u;
+        uint32_t x[16];
+        if (base + 64 <= d.valid) {
+            const uint32_t *p = reinterpret_cast<const uint32_t *>(d.ptr + base);
+#pragma unroll
+            for (int w = 0; w < 16; w++) x[w] = p[w];
+        } else {
+#pragma unroll
+            for (int w = 0; w < 16; w++) {
+                const uint32_t o = base + 4u * w;
+                x[w] = (o < S) ? load_word(d.ptr, o, d.valid) : md5_pad_word(o, S, w, b, nb);
+            }
+        }
+        md5_compress(h, x);
+    }
+    store_digest(md5_out + (size_t)s * 16, h);
+}
+
+hipError_t launch_slice_md5(const SliceDesc *d_slices, uint32_t n, uint32_t S, uint8_t *d_md5,
+                            hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_slice_md5, dim3((n + 63) / 64), dim3(64), 0, stream, d_slices, n, S,
+                       d_md5);
+    return hipGetLastError();
+}
+
+// ======================================================================
+// K2b: slice CRC-32, one wave per slice
+// ======================================================================
+//
+// Each lane takes the CRC-32 of a contiguous segment; the 64 segment CRCs are
+// merged in a shuffle tree with crc(A || B) = (crc(A) * x^(8 |B|) mod P) ^
+// crc(B), which holds for the init/xorout 0xFFFFFFFF form.  Polynomials are
+// in reflected form: bit 31 is x^0.
+
+constexpr uint32_t kCrcPoly = 0xEDB88320u;
+
+struct CrcPowers {
+    uint32_t x2n[32];  // x^(2^k) mod P, reflected
+};
+
+__device__ __forceinline__ uint32_t crc_mulmod(uint32_t a, uint32_t b) {
+    uint32_t p = 0;
+#pragma unroll
+    for (int i = 0; i < 32; i++) {
+        if (a & (0x80000000u >> i)) p ^= b;
+        b = (b & 1u) ? ((b >> 1) ^ kCrcPoly) : (b >> 1);
+    }
+    return p;
+}
+
+// x^(8 n) mod P
+__device__ __forceinline__ uint32_t crc_shift_op(const CrcPowers &pw, uint32_t n) {
+    uint32_t p = 0x80000000u;
+    uint32_t k = 3;
+    while (n) {
+        if (n & 1u) p = crc_mulmod(pw.x2n[k & 31], p);
+        n >>= 1;
+        k++;
+    }
+    return p;
+}
+
+__global__ __launch_bounds__(64) void k_slice_crc32(const SliceDesc *__restrict__ slices,
+                                                    uint32_t S, uint32_t *__restrict__ crc_out,
+                                                    CrcPowers pw) {
+    __shared__ uint32_t table[256];
+    for (uint32_t e = threadIdx.x; e < 256; e += 64) {
+        uint32_t c = e;
+        for (int k = 0; k < 8; k++) c = (c & 1u) ? (kCrcPoly ^ (c >> 1)) : (c >> 1);
+        table[e] = c;
+    }
+    __syncthreads();
+    const SliceDesc d = slices[blockIdx.x];
+    const uint32_t seg = (((S + 63u) / 64u) + 3u) & ~3u;
+    const uint32_t lane = threadIdx.x;
+    const uint32_t lo = min(lane * seg, S), hi = min(lo + seg, S);
+    uint32_t c = 0xFFFFFFFFu;
+    for (uint32_t o = lo; o < hi; o += 4) {
+        uint32_t w = load_word(d.ptr, o, d.valid);
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            c = table[(c ^ w) & 0xFFu] ^ (c >> 8);
+            w >>= 8;
+        }
+    }
+    c ^= 0xFFFFFFFFu;
+    for (uint32_t step = 1; step < 64; step <<= 1) {
+        const uint32_t other = __shfl_down(c, step, 64);
+        if ((lane & (2 * step - 1)) == 0) {
+            const uint32_t rlo = min((lane + step) * seg, S), rhi = min((lane + 2 * step) * seg, S);
+            c = crc_mulmod(crc_shift_op(pw, rhi - rlo), c) ^ other;
+        }
+    }
+    if (lane == 0) crc_out[blockIdx.x] = c;
+}
+
+static CrcPowers make_crc_powers() {
+    auto mul = [](uint32_t a, uint32_t b) {
+        uint32_t p = 0;
+        for (int i = 0; i < 32; i++) {
+            if (a & (0x80000000u >> i)) p ^= b;
+            b = (b & 1u) ? ((b >> 1) ^ kCrcPoly) : (b >> 1);
+        }
+        return p;
+    };
+    CrcPowers pw;
+    pw.x2n[0] = 0x40000000u;  // x^1
+    for (int k = 1; k < 32; k++) pw.x2n[k] = mul(pw.x2n[k - 1], pw.x2n[k - 1]);
+    return pw;
+}
+
+hipError_t launch_slice_crc32(const SliceDesc *d_slices, uint32_t n, uint32_t S, uint32_t *d_crc,
+                              hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    static const CrcPowers pw = make_crc_powers();
+    hipLaunchKernelGGL(k_slice_crc32, dim3(n), dim3(64), 0, stream, d_slices, S, d_crc, pw);
+    return hipGetLastError();
+}
+
+// ======================================================================
+// K3: recovery packet MD5, one lane per packet
+// ======================================================================
+
+struct PacketPrefix {
+    uint32_t w[8];  // set_id (4 words) || "PAR 2.0\0RecvSlic" (4 words)
+};
+
+__global__ __launch_bounds__(64) void k_recovery_md5(const uint8_t *__restrict__ rec,
+                                                     uint32_t count, uint32_t S, uint32_t first_exp,
+                                                     PacketPrefix pre, uint8_t *__restrict__ md5_out) {
+    const uint32_t r = blockIdx.x * 64 + threadIdx.x;
+    if (r >= count) return;
+    const uint8_t *data = rec + (size_t)r * S;
+    const uint64_t len = 36ull + S;
+    const uint32_t nb = md5_blocks(len);
+    uint32_t h[4];
+    md5_init(h);
+    for (uint32_t b = 0; b < nb; b++) {
+        const uint64_t base = (uint64_t)b * 64u;
+        uint32_t x[16];
+        if (base >= 36 && base + 64 <= len) {
+            const uint32_t *p = reinterpret_cast<const uint32_t *>(data + (base - 36));
+#pragma unroll
+            for (int w = 0; w < 16; w++) x[w] = p[w];
+        } else {
+#pragma unroll
+            for (int w = 0; w < 16; w++) {
+                const uint64_t o = base + 4u * w;
+                uint32_t v;
+                if (o < 32) v = pre.w[o / 4];
+                else if (o < 36) v = first_exp + r;
+                else if (o < len) v = *reinterpret_cast<const uint32_t *>(data + (o - 36));
+                else v = md5_pad_word(o, len, w, b, nb);
+                x[w] = v;
+            }
+        }
+        md5_compress(h, x);
+    }
+    store_digest(md5_out + (size_t)r * 16, h);
+}
+
+hipError_t launch_recovery_md5(const uint8_t *d_rec, uint32_t count, uint32_t S,
+                               uint32_t first_exp, const uint8_t set_id[16], uint8_t *d_md5,
+                               hipStream_t stream) {
+    if (count == 0) return hipSuccess;
+    static const uint8_t type_tag[16] = {'P', 'A', 'R', ' ', '2', '.', '0', 0,
+                                         'R', 'e', 'c', 'v', 'S', 'l', 'i', 'c'};
+    PacketPrefix pre;
+    for (int w = 0; w < 4; w++) {
+        pre.w[w] = (uint32_t)set_id[4 * w] | ((uint32_t)set_id[4 * w + 1] << 8) |
+                   ((uint32_t)set_id[4 * w + 2] << 16) | ((uint32_t)set_id[4 * w + 3] << 24);
+        pre.w[4 + w] = (uint32_t)type_tag[4 * w] | ((uint32_t)type_tag[4 * w + 1] << 8) |
+                       ((uint32_t)type_tag[4 * w + 2] << 16) | ((uint32_t)type_tag[4 * w + 3] << 24);
+    }
+    hipLaunchKernelGGL(k_recovery_md5, dim3((count + 63) / 64), dim3(64), 0, stream, d_rec, count,
+                       S, first_exp, pre, d_md5);
+    return hipGetLastError();
+}
+
+}  // namespace p2g

@@ -1,0 +1,501 @@
+// par2_create.cpp -- SourceFile::scan and run_create on the GPU engine.
+//
+// Control flow and byte layout follow src/source.rs:59-136 and
+// src/creator.rs:188-321, 1025-1087; the heavy loops go through the C ABI:
+// p2g_slice_checksums (IFSC MD5+CRC-32), p2g_encode_* (recovery blocks and
+// their packet hashes).  hash_full / hash16k and the small critical-packet
+// hashes stay on the host.
+#include "par2_create.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cerrno>
+#include <cstdio>
+#include <cstring>
+#include <fcntl.h>
+#include <sys/stat.h>
+#include <thread>
+#include <unistd.h>
+
+#include "host_md5.hpp"
+#include "p2g_context.hpp"
+
+namespace par2 {
+namespace {
+
+using p2g::HostMd5;
+
+const uint8_t kMagic[8] = {'P', 'A', 'R', '2', 0, 'P', 'K', 'T'};
+const uint8_t kTypeMain[16] = {'P', 'A', 'R', ' ', '2', '.', '0', 0, 'M', 'a', 'i', 'n', 0, 0, 0, 0};
+const uint8_t kTypeFileDesc[16] = {'P', 'A', 'R', ' ', '2', '.', '0', 0,
+                                   'F', 'i', 'l', 'e', 'D', 'e', 's', 'c'};
+const uint8_t kTypeIfsc[16] = {'P', 'A', 'R', ' ', '2', '.', '0', 0, 'I', 'F', 'S', 'C', 0, 0, 0, 0};
+const uint8_t kTypeRecv[16] = {'P', 'A', 'R', ' ', '2', '.', '0', 0,
+                               'R', 'e', 'c', 'v', 'S', 'l', 'i', 'c'};
+const uint8_t kTypeCreator[16] = {'P', 'A', 'R', ' ', '2', '.', '0', 0,
+                                  'C', 'r', 'e', 'a', 't', 'o', 'r', 0};
+const char kCreatorString[] = "Created by par2rust v0.1.0";  // src/packet/creator.rs:3
+
+constexpr uint64_t kChunkBytes = 64ull << 20;  // bytes read per GPU hand-off
+
+int set_err(Error &err, int code, const std::string &msg) {
+    err.code = code;
+    err.message = msg;
+    return code;
+}
+
+void put_le64(std::vector<uint8_t> &v, uint64_t x) {
+    for (int i = 0; i < 8; i++) v.push_back((uint8_t)(x >> (8 * i)));
+}
+void put_le32(std::vector<uint8_t> &v, uint32_t x) {
+    for (int i = 0; i < 4; i++) v.push_back((uint8_t)(x >> (8 * i)));
+}
+
+// build_packet, src/format.rs:33-54.
+void append_packet(std::vector<uint8_t> &out, const uint8_t set_id[16], const uint8_t type[16],
+                   const std::vector<uint8_t> &body) {
+    uint8_t hash[16];
+    HostMd5 m;
+    m.update(set_id, 16);
+    m.update(type, 16);
+    m.update(body.data(), body.size());
+    m.finish(hash);
+    out.insert(out.end(), kMagic, kMagic + 8);
+    put_le64(out, 64 + body.size());
+    out.insert(out.end(), hash, hash + 16);
+    out.insert(out.end(), set_id, set_id + 16);
+    out.insert(out.end(), type, type + 16);
+    out.insert(out.end(), body.begin(), body.end());
+}
+
+// write_atomic, src/creator.rs:1070-1087.
+int write_atomic(const std::string &path, const std::vector<const std::vector<uint8_t> *> &parts,
+                 const std::vector<std::pair<const uint8_t *, size_t>> &raw, Error &err) {
+    const std::string tmp = path + ".par2rust.tmp";
+    int fd = ::open(tmp.c_str(), O_WRONLY | O_CREAT | O_TRUNC, 0644);
+    if (fd < 0) return set_err(err, P2G_ERR_IO, "cannot create " + tmp + ": " + strerror(errno));
+    auto write_all = [&](const uint8_t *p, size_t n) {
+        while (n) {
+            ssize_t w = ::write(fd, p, n);
+            if (w < 0) {
+                if (errno == EINTR) continue;
+                return false;
+            }
+            p += w;
+            n -= (size_t)w;
+        }
+        return true;
+    };
+    bool ok = true;
+    for (auto *v : parts) ok = ok && write_all(v->data(), v->size());
+    for (auto &r : raw) ok = ok && write_all(r.first, r.second);
+    ok = ok && ::fsync(fd) == 0;
+    ::close(fd);
+    if (!ok) return set_err(err, P2G_ERR_IO, "write failed for " + tmp + ": " + strerror(errno));
+    if (::rename(tmp.c_str(), path.c_str()) != 0)
+        return set_err(err, P2G_ERR_IO, "rename failed for " + path + ": " + strerror(errno));
+    return P2G_OK;
+}
+
+// derive_volume_filename, src/creator.rs:1046-1057, with Rust's
+// Path::file_stem / extension split (a leading dot does not start an
+// extension).
+std::string volume_name(const std::string &index, uint32_t first, uint32_t count) {
+    const size_t slash = index.rfind('/');
+    const size_t fstart = slash == std::string::npos ? 0 : slash + 1;
+    const std::string dir = index.substr(0, fstart);
+    const std::string fname = index.substr(fstart);
+    const size_t dot = fname.rfind('.');
+    const bool has_ext = dot != std::string::npos && dot != 0;
+    const std::string stem = has_ext ? fname.substr(0, dot) : fname;
+    const std::string ext = has_ext ? fname.substr(dot + 1) : "par2";
+    return dir + stem + ".vol" + std::to_string(first) + "+" + std::to_string(count) + "." + ext;
+}
+
+// Pinned host buffer for file reads.
+struct Pinned {
+    uint8_t *p = nullptr;
+    size_t cap = 0;
+    ~Pinned() {
+        if (p) hipHostFree(p);
+    }
+    bool reserve(size_t n) {
+        if (n <= cap) return true;
+        if (p) hipHostFree(p);
+        p = nullptr;
+        cap = 0;
+        if (hipHostMalloc(reinterpret_cast<void **>(&p), n, hipHostMallocDefault) != hipSuccess) {
+            p = nullptr;
+            return false;
+        }
+        cap = n;
+        return true;
+    }
+};
+
+bool read_full(int fd, uint8_t *buf, size_t n, size_t &got) {
+    got = 0;
+    while (got < n) {
+        ssize_t r = ::read(fd, buf + got, n - got);
+        if (r < 0) {
+            if (errno == EINTR) continue;
+            return false;
+        }
+        if (r == 0) break;
+        got += (size_t)r;
+    }
+    return true;
+}
+
+uint64_t chunk_bytes_for(uint64_t S) {
+    const uint64_t slices = std::max<uint64_t>(1, kChunkBytes / S);
+    return slices * S;
+}
+
+}  // namespace
+
+int resolve_volume_sizes(const VolumeScheme &s, uint32_t total, std::vector<uint32_t> &out,
+                         Error &err) {
+    out.clear();
+    std::vector<uint32_t> inner;
+    if (s.limited && s.limit_cap == 0)
+        return set_err(err, P2G_ERR_INVALID_VOLUME_SCHEME,
+                       "Limited scheme requires max_blocks_per_volume > 0");
+    switch (s.kind) {
+    case P2G_SCHEME_SINGLE:
+        inner.push_back(total);
+        break;
+    case P2G_SCHEME_EXPONENTIAL: {  // exponential_split, src/creator.rs:367-385
+        uint32_t remaining = total, next = 1;
+        while (remaining > 0) {
+            const uint32_t take = std::min(next, remaining);
+            inner.push_back(take);
+            remaining -= take;
+            if (inner.size() >= 2) next = next > 0x7FFFFFFFu ? 0xFFFFFFFFu : next * 2;
+        }
+        break;
+    }
+    case P2G_SCHEME_UNIFORM: {  // uniform_split, src/creator.rs:390-409
+        const uint32_t count = s.uniform_count;
+        if (count == 0)
+            return set_err(err, P2G_ERR_INVALID_VOLUME_SCHEME, "Uniform scheme requires count > 0");
+        if (count > total)
+            return set_err(err, P2G_ERR_INVALID_VOLUME_SCHEME,
+                           "Uniform scheme count (" + std::to_string(count) +
+                               ") exceeds recovery block total (" + std::to_string(total) + ")");
+        const uint32_t base = total / count, rem = total - base * count;
+        for (uint32_t i = 0; i + 1 < count; i++) inner.push_back(base);
+        inner.push_back(base + rem);
+        break;
+    }
+    case P2G_SCHEME_EXPLICIT: {  // src/creator.rs:346-360
+        if (s.explicit_sizes.empty())
+            return set_err(err, P2G_ERR_INVALID_VOLUME_SCHEME,
+                           "explicit scheme requires at least one volume");
+        uint64_t sum = 0;
+        for (uint32_t v : s.explicit_sizes) {
+            if (v == 0)
+                return set_err(err, P2G_ERR_INVALID_VOLUME_SCHEME,
+                               "explicit scheme volumes must each have >0 recovery blocks");
+            sum += v;
+        }
+        if (sum != total)
+            return set_err(err, P2G_ERR_INVALID_VOLUME_SCHEME,
+                           "explicit volume sizes sum to " + std::to_string(sum) + ", expected " +
+                               std::to_string(total));
+        inner = s.explicit_sizes;
+        break;
+    }
+    default:
+        return set_err(err, P2G_ERR_INVALID_VOLUME_SCHEME, "unknown volume scheme");
+    }
+    if (!s.limited) {
+        out = inner;
+        return P2G_OK;
+    }
+    for (uint32_t n : inner) {  // cap_volume_sizes, src/creator.rs:414-431
+        if (n <= s.limit_cap) {
+            out.push_back(n);
+            continue;
+        }
+        uint32_t rem = n;
+        while (rem > s.limit_cap) {
+            out.push_back(s.limit_cap);
+            rem -= s.limit_cap;
+        }
+        if (rem > 0) out.push_back(rem);
+    }
+    return P2G_OK;
+}
+
+int scan(p2g_ctx *ctx, const std::string &path, const std::vector<uint8_t> &name,
+         uint64_t slice_size, SourceFile &out, Error &err) {
+    // validation order, src/source.rs:65-75
+    if (name.empty() || std::find(name.begin(), name.end(), 0) != name.end())
+        return set_err(err, P2G_ERR_INVALID_FILE_NAME, "filename " + path + " is empty or contains a NUL byte");
+    if (slice_size == 0 || slice_size % 4 != 0)
+        return set_err(err, P2G_ERR_INVALID_SLICE_SIZE,
+                       "invalid slice size " + std::to_string(slice_size));
+    struct stat st;
+    if (::stat(path.c_str(), &st) != 0)
+        return set_err(err, P2G_ERR_IO, "cannot stat " + path + ": " + strerror(errno));
+    const uint64_t length = (uint64_t)st.st_size;
+    if (length == 0) return set_err(err, P2G_ERR_EMPTY_FILE, "input file " + path + " is empty");
+    const uint64_t total_slices = (length + slice_size - 1) / slice_size;
+
+    out = SourceFile();
+    out.name = name;
+    out.path = path;
+    out.length = length;
+    out.slice_checksums.resize(total_slices);
+
+    int fd = ::open(path.c_str(), O_RDONLY);
+    if (fd < 0) return set_err(err, P2G_ERR_IO, "cannot open " + path + ": " + strerror(errno));
+    const uint64_t chunk = chunk_bytes_for(slice_size);
+    Pinned buf;
+    if (!buf.reserve((size_t)std::min<uint64_t>(chunk, length))) {
+        ::close(fd);
+        return set_err(err, P2G_ERR_OUT_OF_MEMORY, "pinned read buffer");
+    }
+    std::vector<uint8_t> md5s(16 * total_slices);
+    std::vector<uint32_t> crcs(total_slices);
+    HostMd5 full, first16k;
+    uint64_t done = 0, slice_at = 0;
+    int rc = P2G_OK;
+    while (done < length) {
+        const size_t want = (size_t)std::min<uint64_t>(chunk, length - done);
+        size_t got = 0;
+        if (!read_full(fd, buf.p, want, got) || got != want) {
+            rc = set_err(err, P2G_ERR_IO, "short read on " + path);
+            break;
+        }
+        // hash_full / hash16k stream on the host while the GPU hashes slices.
+        const uint64_t at = done;
+        std::thread host_hash([&, at, got]() {
+            full.update(buf.p, got);
+            if (at < 16384) first16k.update(buf.p, (size_t)std::min<uint64_t>(got, 16384 - at));
+        });
+        const uint64_t n = (got + slice_size - 1) / slice_size;
+        rc = p2g_slice_checksums(ctx, buf.p, got, slice_size, md5s.data() + 16 * slice_at,
+                                 crcs.data() + slice_at);
+        host_hash.join();
+        if (rc != P2G_OK) {
+            set_err(err, rc, std::string("slice checksums: ") + p2g_last_error(ctx));
+            break;
+        }
+        done += got;
+        slice_at += n;
+    }
+    ::close(fd);
+    if (rc != P2G_OK) return rc;
+    full.finish(out.hash_full);
+    first16k.finish(out.hash16k);
+    for (uint64_t s = 0; s < total_slices; s++) {
+        memcpy(out.slice_checksums[s].md5, md5s.data() + 16 * s, 16);
+        out.slice_checksums[s].crc32 = crcs[s];
+    }
+    // compute_file_id, src/source.rs:303-309
+    HostMd5 id;
+    id.update(out.hash16k, 16);
+    uint8_t l[8];
+    for (int i = 0; i < 8; i++) l[i] = (uint8_t)(length >> (8 * i));
+    id.update(l, 8);
+    id.update(name.data(), name.size());
+    id.finish(out.file_id);
+    return P2G_OK;
+}
+
+int run_create(p2g_ctx *ctx, const CreateOptions &opts, const std::vector<SourceFile> &sources,
+               std::vector<std::string> &written, Error &err) {
+    written.clear();
+    // validation order, src/creator.rs:193-217
+    if (sources.empty()) return set_err(err, P2G_ERR_NO_INPUT_FILES, "no input files supplied");
+    if (sources.size() > P2G_MAX_FILES)
+        return set_err(err, P2G_ERR_TOO_MANY_FILES, "too many input files");
+    if (opts.recovery_block_count > P2G_MAX_RECOVERY_BLOCKS)
+        return set_err(err, P2G_ERR_TOO_MANY_RECOVERY_BLOCKS,
+                       "recovery block count exceeds PAR2 limit of 65535");
+    const uint64_t S = opts.slice_size;
+    if (S == 0 || S % 4 != 0)
+        return set_err(err, P2G_ERR_INVALID_SLICE_SIZE, "invalid slice size " + std::to_string(S));
+    uint64_t total_blocks = 0, total_bytes = 0;
+    for (auto &s : sources) {
+        total_blocks += s.slice_checksums.size();
+        total_bytes += s.length;
+    }
+    if (total_blocks > P2G_MAX_INPUT_BLOCKS) {
+        const uint64_t raw = (total_bytes + P2G_MAX_INPUT_BLOCKS - 1) / P2G_MAX_INPUT_BLOCKS;
+        const uint64_t suggested = (raw + 3) / 4 * 4;
+        return set_err(err, P2G_ERR_TOO_MANY_INPUT_BLOCKS,
+                       "too many input blocks (" + std::to_string(total_blocks) + ") at slice_size " +
+                           std::to_string(S) + "; retry with slice_size >= " +
+                           std::to_string(suggested));
+    }
+
+    // Main packet, src/packet/main_packet.rs:25-42: file ids sorted.
+    std::vector<std::array<uint8_t, 16>> ids(sources.size());
+    for (size_t f = 0; f < sources.size(); f++) memcpy(ids[f].data(), sources[f].file_id, 16);
+    std::sort(ids.begin(), ids.end());
+    std::vector<uint8_t> body;
+    put_le64(body, S);
+    put_le32(body, (uint32_t)sources.size());
+    for (auto &id : ids) body.insert(body.end(), id.begin(), id.end());
+    uint8_t set_id[16];
+    HostMd5::digest(body.data(), body.size(), set_id);
+    std::vector<uint8_t> critical;
+    append_packet(critical, set_id, kTypeMain, body);
+    // FileDesc + IFSC per source in caller order, src/creator.rs:227-230
+    for (auto &s : sources) {
+        body.clear();  // src/packet/file_desc.rs:17-28
+        body.insert(body.end(), s.file_id, s.file_id + 16);
+        body.insert(body.end(), s.hash_full, s.hash_full + 16);
+        body.insert(body.end(), s.hash16k, s.hash16k + 16);
+        put_le64(body, s.length);
+        body.insert(body.end(), s.name.begin(), s.name.end());
+        body.resize(56 + ((s.name.size() + 3) & ~size_t(3)), 0);
+        append_packet(critical, set_id, kTypeFileDesc, body);
+        body.clear();  // src/packet/file_verify.rs:12-23
+        body.insert(body.end(), s.file_id, s.file_id + 16);
+        for (auto &c : s.slice_checksums) {
+            body.insert(body.end(), c.md5, c.md5 + 16);
+            put_le32(body, c.crc32);
+        }
+        append_packet(critical, set_id, kTypeIfsc, body);
+    }
+    body.assign(kCreatorString, kCreatorString + sizeof(kCreatorString) - 1);
+    body.resize((body.size() + 3) & ~size_t(3), 0);  // src/packet/creator.rs:8-14
+    append_packet(critical, set_id, kTypeCreator, body);
+
+    int rc = write_atomic(opts.output, {&critical}, {}, err);
+    if (rc != P2G_OK) return rc;
+    written.push_back(opts.output);
+    const uint32_t R = opts.recovery_block_count;
+    if (R == 0) return P2G_OK;
+
+    std::vector<uint32_t> sizes;
+    rc = resolve_volume_sizes(opts.volume_scheme, R, sizes, err);
+    if (rc != P2G_OK) return rc;  // after the index is written, as src/creator.rs:237 then :244
+
+    // Single-pass encode, src/creator.rs:883-1018: every file read once, its
+    // slices pushed at their running global input index.
+    p2g_job *job = nullptr;
+    rc = p2g_encode_begin(ctx, S, (uint32_t)total_blocks, R, &job);
+    if (rc != P2G_OK) return set_err(err, rc, std::string("encode: ") + p2g_last_error(ctx));
+    const uint64_t chunk = chunk_bytes_for(S);
+    Pinned buf;
+    uint32_t input_idx = 0;
+    for (auto &s : sources) {
+        int fd = ::open(s.path.c_str(), O_RDONLY);
+        if (fd < 0) {
+            p2g_encode_free(job);
+            return set_err(err, P2G_ERR_IO, "cannot open " + s.path + ": " + strerror(errno));
+        }
+        uint64_t done = 0;
+        while (done < s.length) {
+            const size_t want = (size_t)std::min<uint64_t>(chunk, s.length - done);
+            size_t got = 0;
+            if (!buf.reserve(want) || !read_full(fd, buf.p, want, got) || got != want) {
+                ::close(fd);
+                p2g_encode_free(job);
+                return set_err(err, P2G_ERR_IO, "short read on " + s.path);
+            }
+            rc = p2g_encode_push(job, input_idx, buf.p, got);
+            if (rc != P2G_OK) {
+                ::close(fd);
+                p2g_encode_free(job);
+                return set_err(err, rc, std::string("encode: ") + p2g_last_error(ctx));
+            }
+            input_idx += (uint32_t)((got + S - 1) / S);
+            done += got;
+        }
+        ::close(fd);
+    }
+    Pinned rec;
+    if (!rec.reserve((size_t)R * S)) {
+        p2g_encode_free(job);
+        return set_err(err, P2G_ERR_OUT_OF_MEMORY, "pinned recovery buffer");
+    }
+    std::vector<uint8_t> pkt_md5((size_t)R * 16);
+    rc = p2g_encode_finish(job, rec.p, set_id, pkt_md5.data());
+    p2g_encode_free(job);
+    if (rc != P2G_OK) return set_err(err, rc, std::string("encode: ") + p2g_last_error(ctx));
+
+    // Volumes, src/creator.rs:279-317 and build_volume_packet :1025-1042:
+    // critical || RecvSlic packets in exponent order || critical.  Packet
+    // hashes come from the GPU; the headers are assembled here.
+    uint32_t first = 0;
+    for (uint32_t count : sizes) {
+        std::vector<std::vector<uint8_t>> hdrs(count);
+        for (uint32_t k = 0; k < count; k++) {
+            const uint32_t exp = first + k;
+            auto &h = hdrs[k];
+            h.insert(h.end(), kMagic, kMagic + 8);
+            put_le64(h, 64 + 4 + S);
+            h.insert(h.end(), &pkt_md5[16 * (size_t)exp], &pkt_md5[16 * (size_t)exp] + 16);
+            h.insert(h.end(), set_id, set_id + 16);
+            h.insert(h.end(), kTypeRecv, kTypeRecv + 16);
+            put_le32(h, exp);
+        }
+        std::vector<std::pair<const uint8_t *, size_t>> parts;
+        for (uint32_t k = 0; k < count; k++) {
+            parts.push_back({hdrs[k].data(), hdrs[k].size()});
+            parts.push_back({rec.p + (size_t)(first + k) * S, (size_t)S});
+        }
+        parts.push_back({critical.data(), critical.size()});
+        const std::string vname = volume_name(opts.output, first, count);
+        rc = write_atomic(vname, {&critical}, parts, err);
+        if (rc != P2G_OK) return rc;
+        written.push_back(vname);
+        first += count;
+    }
+    return P2G_OK;
+}
+
+}  // namespace par2
+
+// =====================================================================
+// extern "C" p2g_run_create
+// =====================================================================
+
+extern "C" int p2g_run_create(p2g_ctx *ctx, const char *output, uint64_t slice_size,
+                              uint32_t recovery_count, const p2g_volume_scheme *scheme,
+                              uint32_t nfiles, const char *const *paths,
+                              const uint8_t *const *names, const uint32_t *name_lens,
+                              char *names_out, size_t names_cap) {
+    if (!ctx || !output || !scheme || (nfiles && (!paths || !names || !name_lens)))
+        return P2G_ERR_INVALID_ARGUMENT;
+    if (names_out && names_cap) names_out[0] = 0;
+    par2::Error err;
+    std::vector<par2::SourceFile> sources(nfiles);
+    for (uint32_t f = 0; f < nfiles; f++) {
+        std::vector<uint8_t> name(names[f], names[f] + name_lens[f]);
+        int rc = par2::scan(ctx, paths[f], name, slice_size, sources[f], err);
+        if (rc != P2G_OK) {
+            reinterpret_cast<p2g::Ctx *>(ctx)->err = err.message;
+            return rc;
+        }
+    }
+    par2::CreateOptions opts;
+    opts.output = output;
+    opts.slice_size = slice_size;
+    opts.recovery_block_count = recovery_count;
+    opts.volume_scheme.kind = scheme->kind;
+    opts.volume_scheme.uniform_count = scheme->uniform_count;
+    opts.volume_scheme.limited = scheme->limited != 0;
+    opts.volume_scheme.limit_cap = scheme->limit_cap;
+    if (scheme->explicit_sizes)
+        opts.volume_scheme.explicit_sizes.assign(scheme->explicit_sizes,
+                                                 scheme->explicit_sizes + scheme->n_explicit);
+    std::vector<std::string> written;
+    int rc = par2::run_create(ctx, opts, sources, written, err);
+    size_t used = 0;
+    for (auto &w : written) {
+        if (names_out && used + w.size() + 1 <= names_cap) {
+            memcpy(names_out + used, w.c_str(), w.size() + 1);
+            used += w.size() + 1;
+        }
+    }
+    if (rc != P2G_OK) reinterpret_cast<p2g::Ctx *>(ctx)->err = err.message;
+    return rc;
+}
