@@ -103,3 +103,32 @@ def test_errors(tmp_path, ctx):
     expect_error(lambda: ctx.run_create(opts(R=3, sch=p2.VolumeScheme.uniform(4)),
                                         [(str(ok), b"ok.bin")]), "InvalidVolumeScheme")
     assert os.path.exists(out)
+
+
+def test_config0_against_golden_manifest(tmp_path, ctx):
+    import hashlib
+    import json
+
+    from tests.par2_check import check_set
+
+    m = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "config0.json")))
+    data = O.xorshift64_bytes(m["input"]["bytes"], m["input"]["seed"])
+    src = write(tmp_path / "input.bin", data)
+    files = ctx.run_create(p2.CreateOptions(str(tmp_path / "recovery.par2"), m["slice_size"],
+                                            m["recovery"], p2.VolumeScheme.exponential()),
+                           [(str(src), b"input.bin")])
+    got = [{"name": os.path.basename(p), "size": os.path.getsize(p),
+            "sha256": hashlib.sha256(open(p, "rb").read()).hexdigest()} for p in files]
+    assert got == m["files"]
+    check_set({os.path.basename(p): open(p, "rb").read() for p in files},
+              [(b"input.bin", data)], m["slice_size"], m["recovery"])
+
+
+def test_tiny_against_golden_bytes(tmp_path, ctx):
+    import json
+
+    t = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "tiny.json")))
+    src = write(tmp_path / "a.bin", t["input"].encode())
+    files = ctx.run_create(p2.CreateOptions(str(tmp_path / "recovery.par2"), 4, 2,
+                                            p2.VolumeScheme.single()), [(str(src), b"a.bin")])
+    assert [open(p, "rb").read().hex() for p in files] == [f["hex"] for f in t["files"]]
