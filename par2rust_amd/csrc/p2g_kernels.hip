@@ -98,6 +98,12 @@ __device__ __forceinline__ void transpose16(uint32_t w[16]) {
     }
 }
 
+// acc ^ (x & m) in one v_bitop3_b32: truth-table bit (s0 << 2 | s1 << 1 | s2)
+// of f(s0, s1, s2) = s0 ^ (s1 & s2) gives 0x78.
+__device__ __forceinline__ uint32_t bitop3_xor_and(uint32_t acc, uint32_t x, uint32_t m) {
+    return __builtin_amdgcn_bitop3_b32(acc, x, m, 0x78);
+}
+
 constexpr int kEncThreads = 256;
 constexpr int kEncRT = 8;
 
@@ -143,7 +149,8 @@ __global__ __launch_bounds__(kEncThreads) void k_encode_bits(
                     const uint32_t row = m[rr * 16 + j];
                     uint32_t a = acc[rr][j];
 #pragma unroll
-                    for (int k = 0; k < 16; k++) a ^= x[k] & (0u - ((row >> k) & 1u));
+                    for (int k = 0; k < 16; k++)
+                        a = bitop3_xor_and(a, x[k], 0u - ((row >> k) & 1u));
                     acc[rr][j] = a;
                 }
             }
