@@ -59,11 +59,11 @@ class HostMd5 {
         uint32_t x[16];
         memcpy(x, p, 64);  // little-endian host
         uint32_t a = h_[0], b = h_[1], c = h_[2], d = h_[3];
-#define P2G_R(f, a, b, c, d, k, s, t) a = b + rotl(a + (f) + x[k] + (t), s)
-#define P2G_F (d ^ (b & (c ^ d)))
-#define P2G_G (c ^ (d & (b ^ c)))
-#define P2G_H (b ^ c ^ d)
-#define P2G_I (c ^ (b | ~d))
+#define P2G_R(f, a, b, c, d, k, s, t) a = b + rotl(a + f(b, c, d) + x[k] + (t), s)
+#define P2G_F(b, c, d) ((d) ^ ((b) & ((c) ^ (d))))
+#define P2G_G(b, c, d) ((c) ^ ((d) & ((b) ^ (c))))
+#define P2G_H(b, c, d) ((b) ^ (c) ^ (d))
+#define P2G_I(b, c, d) ((c) ^ ((b) | ~(d)))
         // round 1
         P2G_R(P2G_F, a, b, c, d, 0, 7, 0xd76aa478u);  P2G_R(P2G_F, d, a, b, c, 1, 12, 0xe8c7b756u);
         P2G_R(P2G_F, c, d, a, b, 2, 17, 0x242070dbu); P2G_R(P2G_F, b, c, d, a, 3, 22, 0xc1bdceeeu);

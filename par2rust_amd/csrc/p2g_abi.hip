@@ -2,6 +2,7 @@
 // include/par2gpu.h (p2g_run_create lives in par2_create.cpp).
 #include <algorithm>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
@@ -107,6 +108,10 @@ int Ctx::open(int dev) {
     if (e != hipSuccess) return fail(P2G_ERR_DEVICE, "hipEventCreate", e);
     build_gf_tables(h_log, h_antilog);
     base_logs = input_base_logs();
+    if (const char *v = getenv("P2G_ENCODE")) {
+        if (!strcmp(v, "bitslice")) encode_variant = kEncodeBitslice;
+        else if (!strcmp(v, "mfma")) encode_variant = kEncodeMfma;
+    }
     if (d_antilog.reserve(65536 * 2) != P2G_OK) return fail(P2G_ERR_OUT_OF_MEMORY, "antilog table");
     e = hipMemcpy(d_antilog.ptr, h_antilog.data(), 65536 * 2, hipMemcpyHostToDevice);
     if (e != hipSuccess) return fail(P2G_ERR_DEVICE, "upload antilog", e);
@@ -195,21 +200,39 @@ int dev_encode(Ctx *c, const SliceDesc *d_descs, uint32_t n, uint64_t S, uint32_
         }
         return P2G_OK;
     }
-    // Inputs go in chunks so the coefficient matrices (32 B per (r, i))
-    // stay within kMatsBudget.
-    uint64_t ch = kMatsBudget / ((uint64_t)R * 32);
-    ch = std::max<uint64_t>(1, std::min<uint64_t>(ch, n));
-    if (c->d_mats.reserve((size_t)ch * R * 32) != P2G_OK)
-        return c->fail(P2G_ERR_OUT_OF_MEMORY, "coefficient matrices");
-    const uint16_t *dlog = static_cast<const uint16_t *>(c->d_base_logs.ptr);
+    const uint16_t *dlog = static_cast<const uint16_t *>(c->d_base_logs.ptr) + first_input;
+    const uint16_t *dalog = static_cast<const uint16_t *>(c->d_antilog.ptr);
+    if (c->encode_variant == kEncodeBitslice) {
+        // Inputs go in chunks so the coefficient matrices (32 B per (r, i))
+        // stay within kMatsBudget.
+        uint64_t ch = kMatsBudget / ((uint64_t)R * 32);
+        ch = std::max<uint64_t>(1, std::min<uint64_t>(ch, n));
+        if (c->d_mats.reserve((size_t)ch * R * 32) != P2G_OK)
+            return c->fail(P2G_ERR_OUT_OF_MEMORY, "coefficient matrices");
+        for (uint64_t c0 = 0; c0 < n; c0 += ch) {
+            const uint32_t cn = (uint32_t)std::min<uint64_t>(ch, n - c0);
+            e = launch_coeff_mats(dalog, dlog + c0, cn, R, static_cast<uint16_t *>(c->d_mats.ptr), st);
+            if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "coefficient kernel", e);
+            e = launch_encode(d_descs + c0, cn, static_cast<const uint16_t *>(c->d_mats.ptr), R,
+                              (uint32_t)S, d_rec, (c0 > 0 || accumulate) ? 1 : 0, st);
+            if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "encode kernel", e);
+        }
+        return P2G_OK;
+    }
+    // Matrix-core form: A fragments (1 KiB per 32 rows x 4 inputs) for a chunk
+    // of inputs, a multiple of 4, within kMatsBudget.
+    const uint64_t per_quad = (uint64_t)mfma_row_tiles(R) * 1024;
+    uint64_t ch = std::max<uint64_t>(1, kMatsBudget / per_quad) * 4;
+    ch = std::min<uint64_t>(ch, n);
+    if (c->d_mats.reserve(mfma_afrag_bytes((uint32_t)ch, R)) != P2G_OK)
+        return c->fail(P2G_ERR_OUT_OF_MEMORY, "matrix-core coefficient fragments");
     for (uint64_t c0 = 0; c0 < n; c0 += ch) {
         const uint32_t cn = (uint32_t)std::min<uint64_t>(ch, n - c0);
-        e = launch_coeff_mats(static_cast<const uint16_t *>(c->d_antilog.ptr),
-                              dlog + first_input + c0, cn, R,
-                              static_cast<uint16_t *>(c->d_mats.ptr), st);
-        if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "coefficient kernel", e);
-        e = launch_encode(d_descs + c0, cn, static_cast<const uint16_t *>(c->d_mats.ptr), R,
-                          (uint32_t)S, d_rec, (c0 > 0 || accumulate) ? 1 : 0, st);
+        uint4 *afrag = static_cast<uint4 *>(c->d_mats.ptr);
+        e = launch_mfma_afrag(dalog, dlog + c0, cn, R, afrag, st);
+        if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "coefficient fragment kernel", e);
+        e = launch_encode_mfma(d_descs + c0, cn, afrag, R, (uint32_t)S, d_rec,
+                               (c0 > 0 || accumulate) ? 1 : 0, st);
         if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "encode kernel", e);
     }
     return P2G_OK;

@@ -39,8 +39,13 @@ struct DescSlot {
     hipEvent_t free = nullptr;  // `h` may be rewritten once this fires
 };
 
+// Encode kernel variants: the matrix-core GF(2) contraction (default) and the
+// bit-sliced VALU kernel.  P2G_ENCODE=bitslice|mfma picks one at p2g_open.
+enum { kEncodeMfma = 0, kEncodeBitslice = 1 };
+
 struct Ctx {
     int device = 0;
+    int encode_variant = kEncodeMfma;
     hipStream_t stream = nullptr;  // kernels and copies of this context
     hipStream_t side = nullptr;    // work overlapped with `stream`
     std::string err;

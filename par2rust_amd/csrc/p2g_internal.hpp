@@ -40,6 +40,18 @@ hipError_t launch_encode(const SliceDesc *d_inputs, uint32_t n_inputs, const uin
                          uint32_t R, uint32_t S, uint8_t *d_rec, int accumulate,
                          hipStream_t stream);
 
+// Matrix-core encode.  A fragments for `n` inputs (first input's L at
+// d_logbase[0]) and R recovery rows: mfma_row_tiles(R) * ceil(n / 4) * 1 KiB.
+inline uint32_t mfma_row_tiles(uint32_t R) { return (R + 15) / 16 * 8; }
+inline size_t mfma_afrag_bytes(uint32_t n, uint32_t R) {
+    return (size_t)mfma_row_tiles(R) * ((n + 3) / 4) * 1024;
+}
+hipError_t launch_mfma_afrag(const uint16_t *d_antilog, const uint16_t *d_logbase, uint32_t n,
+                             uint32_t R, uint4 *d_afrag, hipStream_t stream);
+hipError_t launch_encode_mfma(const SliceDesc *d_inputs, uint32_t n_inputs, const uint4 *d_afrag,
+                              uint32_t R, uint32_t S, uint8_t *d_rec, int accumulate,
+                              hipStream_t stream);
+
 // MD5 of each zero-padded S-byte slice (16 bytes per slice into d_md5).
 hipError_t launch_slice_md5(const SliceDesc *d_slices, uint32_t n, uint32_t S,
                             uint8_t *d_md5, hipStream_t stream);

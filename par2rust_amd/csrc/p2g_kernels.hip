@@ -202,6 +202,177 @@ hipError_t launch_encode(const SliceDesc *d_inputs, uint32_t n_inputs, const uin
 }
 
 // ======================================================================
+// K1 (matrix-core form): GF(2) contraction on fp4 MFMA
+// ======================================================================
+//
+// Over GF(2) the whole encode is one bit-matrix product:
+//   out[16 r + j][p] = parity( sum_{i,b} A[16 r + j][16 i + b] * in[16 i + b][p] )
+// with A[16 r + j][16 i + b] = bit j of coef(r, i) * x^b and in[16 i + b][p]
+// = bit b of symbol p of slice i.  Bits become fp4 (e2m1) 1.0 / 0.0, the
+// v_mfma_f32_32x32x64_f8f6f4 sums them exactly in fp32 (at most 16 N =
+// 524288 < 2^24 terms), and the parity of each sum is the output bit.
+//
+// One MFMA covers 32 rows (recovery rows 2t, 2t + 1), 64 k (inputs
+// 4g .. 4g + 3) and 32 columns (symbols).  A lane in half h (lane / 32)
+// holds k values of inputs 4g + 2h and 4g + 2h + 1, 16 nibbles each, the
+// same order in A and in B, so each (A lane, nibble) meets the B nibble of
+// the same k.
+
+typedef int v8i_t __attribute__((ext_vector_type(8)));
+typedef float v16f_t __attribute__((ext_vector_type(16)));
+
+// Eight bits to eight fp4 nibbles: bit k of x becomes nibble k = 0x2 (1.0).
+__device__ __forceinline__ uint32_t spread8(uint32_t x) {
+    uint32_t t = x & 0xFFu;
+    t = (t | (t << 12)) & 0x000F000Fu;
+    t = (t | (t << 6)) & 0x03030303u;
+    t = (t | (t << 3)) & 0x11111111u;
+    return t << 1;
+}
+
+// E8M0 block scale of 1.0 (biased exponent 127) for both operands; a scale
+// operand of 0 would mean 2^-127.
+constexpr int kE8M0One = 127;
+
+constexpr int kMfTiles = 8;      // 32-row MFMA tiles per workgroup: 16 recovery rows
+constexpr int kMfThreads = 256;  // 4 waves, each 64 symbols (two 32-column tiles)
+
+// A fragments: frag (t, g) is 64 lanes x 16 bytes at afrag[(t * nq + g) * 64 + lane].
+// Thread per (t, g, lane).  Rows 2t + rr >= R and inputs >= n are zero.
+__global__ __launch_bounds__(256) void k_mfma_afrag(const uint16_t *__restrict__ antilog,
+                                                    const uint16_t *__restrict__ logbase,
+                                                    uint32_t n, uint32_t R, uint32_t nq,
+                                                    uint32_t tiles, uint4 *__restrict__ afrag) {
+    const uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (uint64_t)tiles * nq * 64) return;
+    const uint32_t lane = (uint32_t)(idx & 63);
+    const uint64_t tg = idx >> 6;
+    const uint32_t g = (uint32_t)(tg % nq), t = (uint32_t)(tg / nq);
+    const uint32_t row = lane & 31, h = lane >> 5;
+    const uint32_t r = 2 * t + (row >> 4), j = row & 15;
+    uint32_t d[4] = {0, 0, 0, 0};
+    for (uint32_t e = 0; e < 2; e++) {
+        const uint32_t i = 4 * g + 2 * h + e;
+        if (r >= R || i >= n) continue;
+        uint32_t v = antilog[(uint32_t)(((uint64_t)logbase[i] * r) % kFieldLimit)];
+        uint32_t m = 0;  // row j of the matrix of multiplication by v
+        for (int b = 0; b < 16; b++) {
+            m |= ((v >> j) & 1u) << b;
+            v = ((v << 1) & 0xFFFFu) ^ ((v & 0x8000u) ? (kFieldGenerator & 0xFFFFu) : 0u);
+        }
+        d[2 * e] = spread8(m);
+        d[2 * e + 1] = spread8(m >> 8);
+    }
+    afrag[idx] = make_uint4(d[0], d[1], d[2], d[3]);
+}
+
+hipError_t launch_mfma_afrag(const uint16_t *d_antilog, const uint16_t *d_logbase, uint32_t n,
+                             uint32_t R, uint4 *d_afrag, hipStream_t stream) {
+    const uint32_t nq = (n + 3) / 4;
+    const uint32_t tiles = mfma_row_tiles(R);
+    const uint64_t total = (uint64_t)tiles * nq * 64;
+    if (total == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_mfma_afrag, dim3((uint32_t)((total + 255) / 256)), dim3(256), 0, stream,
+                       d_antilog, d_logbase, n, R, nq, tiles, d_afrag);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(kMfThreads, 1) void k_encode_mfma(
+    const SliceDesc *__restrict__ inputs, uint32_t n, const uint4 *__restrict__ afrag,
+    uint32_t nq, uint32_t R, uint32_t S, uint8_t *__restrict__ rec, int accumulate) {
+    const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint32_t c = lane & 31, h = lane >> 5;
+    // this lane's dword: symbols base + 2c (column tile 0) and base + 2c + 1 (tile 1)
+    const uint32_t off = (blockIdx.x * (kMfThreads) + w * 64) * 2 + 4 * c;
+    const bool col_ok = off < S;
+    const uint32_t tile0 = blockIdx.y * kMfTiles;
+    const uint4 *ap = afrag + (size_t)tile0 * nq * 64 + lane;
+
+    v16f_t acc[kMfTiles][2];
+#pragma unroll
+    for (int a = 0; a < kMfTiles; a++)
+#pragma unroll
+        for (int cb = 0; cb < 2; cb++)
+#pragma unroll
+            for (int v = 0; v < 16; v++) acc[a][cb][v] = 0.0f;
+
+    for (uint32_t g = 0; g < nq; g++) {
+        uint4 A[kMfTiles];
+#pragma unroll
+        for (int a = 0; a < kMfTiles; a++) A[a] = ap[((size_t)a * nq + g) * 64];
+        uint32_t word[2];
+#pragma unroll
+        for (int e = 0; e < 2; e++) {
+            const uint32_t i = 4 * g + 2 * h + e;
+            word[e] = 0;
+            if (i < n && col_ok) {
+                const SliceDesc d = inputs[i];
+                word[e] = load_word(d.ptr, off, d.valid);
+            }
+        }
+        v8i_t B[2];
+#pragma unroll
+        for (int cb = 0; cb < 2; cb++) {
+            const uint32_t s0 = (word[0] >> (16 * cb)) & 0xFFFFu;
+            const uint32_t s1 = (word[1] >> (16 * cb)) & 0xFFFFu;
+            B[cb] = v8i_t{(int)spread8(s0), (int)spread8(s0 >> 8), (int)spread8(s1),
+                          (int)spread8(s1 >> 8), 0, 0, 0, 0};
+        }
+#pragma unroll
+        for (int a = 0; a < kMfTiles; a++) {
+            const v8i_t av = v8i_t{(int)A[a].x, (int)A[a].y, (int)A[a].z, (int)A[a].w, 0, 0, 0, 0};
+#pragma unroll
+            for (int cb = 0; cb < 2; cb++)
+                acc[a][cb] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(
+                    av, B[cb], acc[a][cb], 4, 4, 0, kE8M0One, 0, kE8M0One);
+        }
+    }
+
+    // Epilogue.  Accumulator v of a lane holds row 8 (v / 4) + 4 h + v % 4 of
+    // the tile at column c: rows 0..15 are bits of recovery row 2t, rows
+    // 16..31 bits of 2t + 1.  Each lane half owns 8 of a symbol's 16 bits;
+    // one exchange across halves gives half h the whole symbol of row 2t + h.
+#pragma unroll
+    for (int a = 0; a < kMfTiles; a++) {
+        uint32_t sym[2];
+#pragma unroll
+        for (int cb = 0; cb < 2; cb++) {
+            uint32_t lo = 0, hi = 0;  // bits of rows 2t and 2t + 1 held here
+#pragma unroll
+            for (int v = 0; v < 16; v++) {
+                const uint32_t bit = ((uint32_t)acc[a][cb][v]) & 1u;
+                const int pos = (v & 3) + ((v >> 2) & 1) * 8;
+                if (v < 8) lo |= bit << pos;
+                else hi |= bit << pos;
+            }
+            lo <<= 4 * h;
+            hi <<= 4 * h;
+            const uint32_t give = h ? lo : hi;
+            const uint32_t got = (uint32_t)__shfl_xor((int)give, 32, 64);
+            sym[cb] = (h ? hi : lo) | got;
+        }
+        const uint32_t r = 2 * (tile0 + a) + h;
+        if (r < R && col_ok) {
+            uint32_t *o = reinterpret_cast<uint32_t *>(rec + (size_t)r * S + off);
+            const uint32_t v = sym[0] | (sym[1] << 16);
+            *o = accumulate ? (*o ^ v) : v;
+        }
+    }
+}
+
+hipError_t launch_encode_mfma(const SliceDesc *d_inputs, uint32_t n_inputs, const uint4 *d_afrag,
+                              uint32_t R, uint32_t S, uint8_t *d_rec, int accumulate,
+                              hipStream_t stream) {
+    if (R == 0) return hipSuccess;
+    const uint32_t nq = (n_inputs + 3) / 4;
+    const uint32_t per_block = kMfThreads * 2;  // bytes of symbol columns per workgroup
+    dim3 grid((S + per_block - 1) / per_block, mfma_row_tiles(R) / kMfTiles);
+    hipLaunchKernelGGL(k_encode_mfma, grid, dim3(kMfThreads), 0, stream, d_inputs, n_inputs,
+                       d_afrag, nq, R, S, d_rec, accumulate);
+    return hipGetLastError();
+}
+
+// ======================================================================
 // MD5 (RFC 1321)
 // ======================================================================
 
