@@ -33,6 +33,7 @@ WORKLOADS = {
 }
 
 HBM_PEAK_GBPS = 8000.0
+KERNEL_NAME = {"mfma": "k_encode_mfma (K1)", "bitslice": "k_encode_bits (K1)"}
 
 
 def log(*a):
@@ -55,10 +56,9 @@ def main() -> int:
         return 2
 
     import numpy as np
-    import torch
 
     import par2rust_amd as p2
-    from par2rust_amd import synth
+    from par2rust_amd import hipmem, synth
 
     nbytes, S, R, seed = WORKLOADS[args.workload]
     N = (nbytes + S - 1) // S
@@ -68,65 +68,58 @@ def main() -> int:
     host = synth.xorshift64_bytes(nbytes, seed)
     log(f"bench: generated input in {time.time() - t:.1f}s")
 
-    dev = torch.cuda.current_device()
-    src = torch.frombuffer(bytearray(host), dtype=torch.uint8).to(f"cuda:{dev}")
-    rec = torch.empty(R * S, dtype=torch.uint8, device=src.device)
-    slice_md5 = torch.empty(N * 16, dtype=torch.uint8, device=src.device)
-    slice_crc = torch.empty(N, dtype=torch.int32, device=src.device)
-    pkt_md5 = torch.empty(R * 16, dtype=torch.uint8, device=src.device)
+    ctx = p2.Context(0)
+    src = hipmem.DeviceBuffer.from_bytes(host)
+    rec = hipmem.DeviceBuffer(R * S)
+    slice_md5 = hipmem.DeviceBuffer(N * 16)
+    slice_crc = hipmem.DeviceBuffer(N * 4)
+    pkt_md5 = hipmem.DeviceBuffer(R * 16)
     set_id = bytes(range(16))
 
-    ctx = p2.Context(dev)
-    s_enc = torch.cuda.Stream(device=src.device)
-    s_sum = torch.cuda.Stream(device=src.device)
-    ev_k1 = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-             for _ in range(args.steps)]
-    ev_k2 = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-             for _ in range(args.steps)]
-    ev_k3 = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-             for _ in range(args.steps)]
+    s_enc, s_sum = hipmem.Stream(), hipmem.Stream()
+    ev = {k: [(hipmem.Event(), hipmem.Event()) for _ in range(args.steps)]
+          for k in ("k1", "k2", "k3")}
 
     def step(i=None):
         if i is not None:
-            ev_k2[i][0].record(s_sum)
-        ctx.dev_slice_checksums([src], S, slice_md5, slice_crc, stream=s_sum.cuda_stream)
+            ev["k2"][i][0].record(s_sum)
+        ctx.dev_slice_checksums([src], S, slice_md5, slice_crc, stream=s_sum.addr)
         if i is not None:
-            ev_k2[i][1].record(s_sum)
-            ev_k1[i][0].record(s_enc)
-        ctx.dev_encode([src], S, R, rec, stream=s_enc.cuda_stream)
+            ev["k2"][i][1].record(s_sum)
+            ev["k1"][i][0].record(s_enc)
+        ctx.dev_encode([src], S, R, rec, stream=s_enc.addr)
         if i is not None:
-            ev_k1[i][1].record(s_enc)
-            ev_k3[i][0].record(s_enc)
-        ctx.dev_recovery_packet_md5(rec, R, S, 0, set_id, pkt_md5, stream=s_enc.cuda_stream)
+            ev["k1"][i][1].record(s_enc)
+            ev["k3"][i][0].record(s_enc)
+        ctx.dev_recovery_packet_md5(rec, R, S, 0, set_id, pkt_md5, stream=s_enc.addr)
         if i is not None:
-            ev_k3[i][1].record(s_enc)
+            ev["k3"][i][1].record(s_enc)
 
     for _ in range(args.warmup):
         step()
-    torch.cuda.synchronize()
+    hipmem.synchronize()
     log(f"bench: {args.warmup} warmup steps done")
 
     t0 = time.perf_counter()
     for i in range(args.steps):
         step(i)
-    torch.cuda.synchronize()
+    hipmem.synchronize()
     t1 = time.perf_counter()
     elapsed = t1 - t0
     ms_step = 1000.0 * elapsed / args.steps
     log(f"bench: {args.steps} timed steps, {ms_step:.3f} ms/step")
 
-    k1_ms = sum(a.elapsed_time(b) for a, b in ev_k1) / args.steps
-    k2_ms = sum(a.elapsed_time(b) for a, b in ev_k2) / args.steps
-    k3_ms = sum(a.elapsed_time(b) for a, b in ev_k3) / args.steps
+    k_ms = {k: sum(a.elapsed_ms(b) for a, b in v) / args.steps for k, v in ev.items()}
+    k1_ms, k2_ms, k3_ms = k_ms["k1"], k_ms["k2"], k_ms["k3"]
     value = nbytes * args.steps / elapsed / 1e9
     gf_ops = N * R * (S // 2)
     alg_bytes = N * S + R * S  # source read once, recovery written once
     achieved = alg_bytes / (k1_ms / 1e3) / 1e9
 
-    gpu_rec = rec.cpu().numpy().tobytes()
-    gpu_md5 = slice_md5.cpu().numpy().tobytes()
-    gpu_crc = slice_crc.cpu().numpy().astype(np.uint32).tolist()
-    gpu_pkt = pkt_md5.cpu().numpy().tobytes()
+    gpu_rec = rec.download()
+    gpu_md5 = slice_md5.download()
+    gpu_crc = np.frombuffer(slice_crc.download(), dtype="<u4").tolist()
+    gpu_pkt = pkt_md5.download()
 
     out = {
         "metric": METRIC,
@@ -149,7 +142,7 @@ def main() -> int:
         "gf_mul_xor_per_s": round(gf_ops * args.steps / elapsed, 1),
         "kernel_ms": {"k1_encode": round(k1_ms, 4), "k2_slice_md5_crc": round(k2_ms, 4),
                       "k3_packet_md5": round(k3_ms, 4)},
-        "roofline": {"bound": "hbm", "kernel": "k_encode_bits (K1)",
+        "roofline": {"bound": "hbm", "kernel": KERNEL_NAME[os.environ.get("P2G_ENCODE", "mfma")],
                      "achieved": round(achieved, 2), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBPS, 5), "traffic": None,
                      "alg_bytes_per_launch": alg_bytes},

@@ -168,6 +168,32 @@ def _buf(data) -> Tuple[ctypes.c_void_p, int, object]:
     return ctypes.c_void_p(a.ctypes.data), a.nbytes, a
 
 
+def _dev_ptr(x) -> int:
+    """Device address of a DeviceBuffer (par2rust_amd.hipmem), a tensor or an int."""
+    if isinstance(x, int):
+        return x
+    if hasattr(x, "addr"):
+        return x.addr
+    return x.data_ptr()
+
+
+def _dev_len(x) -> int:
+    if hasattr(x, "nbytes") and not hasattr(x, "data_ptr"):
+        return x.nbytes
+    return x.numel() * x.element_size()
+
+
+def _dev_files(files):
+    """p2g_dev_file array from DeviceBuffers / tensors / (address, length) pairs."""
+    items = []
+    for f in files:
+        if isinstance(f, tuple):
+            items.append(_DevFile(f[0], f[1]))
+        else:
+            items.append(_DevFile(_dev_ptr(f), _dev_len(f)))
+    return (_DevFile * max(1, len(items)))(*items), items
+
+
 class Context:
     """One p2g context on one GPU (p2g_open / p2g_close)."""
 
@@ -249,35 +275,34 @@ class Context:
                                                 sid, out))
         return [out.raw[16 * i:16 * i + 16] for i in range(count)]
 
-    # --- device-buffer entry points (torch tensors on the GPU) -----------
+    # --- device-buffer entry points ----------------------------------------
+    # Buffers: par2rust_amd.hipmem.DeviceBuffer, tensors, or (address, length).
     def dev_encode(self, files, slice_size: int, recovery_count: int, rec,
                    first_input: int = 0, total_inputs: Optional[int] = None,
                    accumulate: bool = False, stream: Optional[int] = None):
-        arr = (_DevFile * max(1, len(files)))(*[_DevFile(t.data_ptr(), t.numel() * t.element_size())
-                                               for t in files])
+        arr, items = _dev_files(files)
         if total_inputs is None:
-            total_inputs = first_input + sum(
-                (t.numel() * t.element_size() + slice_size - 1) // slice_size for t in files)
-        self._check(lib.p2g_dev_encode(self.h, arr, len(files), slice_size, first_input,
+            total_inputs = first_input + sum((it.len + slice_size - 1) // slice_size
+                                             for it in items)
+        self._check(lib.p2g_dev_encode(self.h, arr, len(items), slice_size, first_input,
                                        total_inputs, recovery_count,
-                                       ctypes.c_void_p(rec.data_ptr()), int(accumulate),
+                                       ctypes.c_void_p(_dev_ptr(rec)), int(accumulate),
                                        ctypes.c_void_p(stream) if stream else None))
 
     def dev_slice_checksums(self, files, slice_size: int, md5_out, crc_out,
                             stream: Optional[int] = None):
-        arr = (_DevFile * max(1, len(files)))(*[_DevFile(t.data_ptr(), t.numel() * t.element_size())
-                                               for t in files])
-        self._check(lib.p2g_dev_slice_checksums(self.h, arr, len(files), slice_size,
-                                                ctypes.c_void_p(md5_out.data_ptr()),
-                                                ctypes.c_void_p(crc_out.data_ptr()),
+        arr, items = _dev_files(files)
+        self._check(lib.p2g_dev_slice_checksums(self.h, arr, len(items), slice_size,
+                                                ctypes.c_void_p(_dev_ptr(md5_out)),
+                                                ctypes.c_void_p(_dev_ptr(crc_out)),
                                                 ctypes.c_void_p(stream) if stream else None))
 
     def dev_recovery_packet_md5(self, rec, count: int, slice_size: int, first_exponent: int,
                                 set_id: bytes, md5_out, stream: Optional[int] = None):
         sid = ctypes.create_string_buffer(bytes(set_id), 16)
-        self._check(lib.p2g_dev_recovery_packet_md5(self.h, ctypes.c_void_p(rec.data_ptr()),
+        self._check(lib.p2g_dev_recovery_packet_md5(self.h, ctypes.c_void_p(_dev_ptr(rec)),
                                                     count, slice_size, first_exponent, sid,
-                                                    ctypes.c_void_p(md5_out.data_ptr()),
+                                                    ctypes.c_void_p(_dev_ptr(md5_out)),
                                                     ctypes.c_void_p(stream) if stream else None))
 
     def synchronize(self):

@@ -63,34 +63,62 @@ def test_encode_config0_shape(ctx):
 
 
 def test_dev_encode_with_resident_inputs(ctx):
-    import torch
+    from par2rust_amd.hipmem import DeviceBuffer, synchronize
 
     lens, S, R = [123457, 65536, 9], 4096, 11
     files = [data(n, 40 + i) for i, n in enumerate(lens)]
-    dev = [torch.frombuffer(bytearray(f), dtype=torch.uint8).cuda() for f in files]
-    rec = torch.empty(R * S, dtype=torch.uint8, device="cuda")
+    dev = [DeviceBuffer.from_bytes(f) for f in files]
+    rec = DeviceBuffer(R * S)
     ctx.dev_encode(dev, S, R, rec)
     ctx.synchronize()
-    assert bytes(rec.cpu().numpy()) == O.encode_simd(files, S, R)
+    assert rec.download() == O.encode_simd(files, S, R)
     # accumulate: a second pass XORs the same contribution in again -> zero
     ctx.dev_encode(dev, S, R, rec, accumulate=True)
     ctx.synchronize()
-    assert not rec.any().item()
+    assert rec.download() == bytes(R * S)
 
 
 def test_dev_encode_split_over_calls_equals_one_call(ctx):
-    import torch
+    from par2rust_amd.hipmem import DeviceBuffer
 
     S, R = 4096, 5
     a, b = data(3 * S + 100, 50), data(2 * S, 51)
-    da = torch.frombuffer(bytearray(a), dtype=torch.uint8).cuda()
-    db = torch.frombuffer(bytearray(b), dtype=torch.uint8).cuda()
-    rec = torch.zeros(R * S, dtype=torch.uint8, device="cuda")
+    da, db = DeviceBuffer.from_bytes(a), DeviceBuffer.from_bytes(b)
+    rec = DeviceBuffer(R * S)
+    rec.zero()
     # second file first: pushes may come in any order
     ctx.dev_encode([db], S, R, rec, first_input=4, total_inputs=6, accumulate=True)
     ctx.dev_encode([da], S, R, rec, first_input=0, total_inputs=6, accumulate=True)
     ctx.synchronize()
-    assert bytes(rec.cpu().numpy()) == O.encode([a, b], S, R)
+    assert rec.download() == O.encode([a, b], S, R)
+
+
+def test_dev_checksums_and_packet_md5(ctx):
+    from par2rust_amd.hipmem import DeviceBuffer
+
+    S, R = 65536, 3
+    files = [data(200001, 60), data(S, 61)]
+    dev = [DeviceBuffer.from_bytes(f) for f in files]
+    n = sum((len(f) + S - 1) // S for f in files)
+    md5, crc = DeviceBuffer(16 * n), DeviceBuffer(4 * n)
+    ctx.dev_slice_checksums(dev, S, md5, crc)
+    rec = DeviceBuffer(R * S)
+    ctx.dev_encode(dev, S, R, rec)
+    sid = data(16, 62)
+    pkt = DeviceBuffer(16 * R)
+    ctx.dev_recovery_packet_md5(rec, R, S, 0, sid, pkt)
+    ctx.synchronize()
+    want_md5, want_crc = [], []
+    for f in files:
+        m, c = O.slice_checksums(f, S)
+        want_md5 += m
+        want_crc += c
+    assert md5.download() == b"".join(want_md5)
+    assert list(struct.unpack(f"<{n}I", crc.download())) == want_crc
+    r = rec.download()
+    assert r == O.encode(files, S, R)
+    assert pkt.download() == b"".join(O.recovery_packet(sid, k, r[k * S:(k + 1) * S])[16:32]
+                                      for k in range(R))
 
 
 CHECKSUM_CASES = [(5, 4), (17, 8), (4096 * 16, 16), (1000, 100), (100001, 4096),
