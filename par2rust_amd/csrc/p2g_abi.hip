@@ -101,8 +101,6 @@ int Ctx::open(int dev) {
     if (e != hipSuccess) return fail(P2G_ERR_DEVICE, "hipSetDevice", e);
     e = hipStreamCreateWithFlags(&stream, hipStreamNonBlocking);
     if (e != hipSuccess) return fail(P2G_ERR_DEVICE, "hipStreamCreate", e);
-    e = hipStreamCreateWithFlags(&side, hipStreamNonBlocking);
-    if (e != hipSuccess) return fail(P2G_ERR_DEVICE, "hipStreamCreate", e);
     e = hipEventCreateWithFlags(&enc_descs.free, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&sum_descs.free, hipEventDisableTiming);
     if (e != hipSuccess) return fail(P2G_ERR_DEVICE, "hipEventCreate", e);
@@ -139,6 +137,21 @@ Ctx::~Ctx() {
     }
     if (stream) hipStreamDestroy(stream);
     if (side) hipStreamDestroy(side);
+}
+
+int Ctx::side_stream(hipStream_t *out) {
+    // Created on first use: hardware queues are handed out to streams in
+    // creation order, and a context that never streams copies should not
+    // take one from the caller's streams.
+    if (!side) {
+        hipError_t e = hipStreamCreateWithFlags(&side, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            side = nullptr;
+            return fail(P2G_ERR_DEVICE, "hipStreamCreate", e);
+        }
+    }
+    *out = side;
+    return P2G_OK;
 }
 
 int check_slice_size(Ctx *c, uint64_t S) {
@@ -223,7 +236,7 @@ int dev_encode(Ctx *c, const SliceDesc *d_descs, uint32_t n, uint64_t S, uint32_
     // of inputs, a multiple of 4, within kMatsBudget.
     const uint64_t per_quad = (uint64_t)mfma_row_tiles(R) * 1024;
     uint64_t ch = std::max<uint64_t>(1, kMatsBudget / per_quad) * 4;
-    ch = std::min<uint64_t>(ch, n);
+    ch = std::min<uint64_t>(std::min<uint64_t>(ch, kMfmaMaxInputsPerLaunch), n);
     if (c->d_mats.reserve(mfma_afrag_bytes((uint32_t)ch, R)) != P2G_OK)
         return c->fail(P2G_ERR_OUT_OF_MEMORY, "matrix-core coefficient fragments");
     for (uint64_t c0 = 0; c0 < n; c0 += ch) {
@@ -274,7 +287,7 @@ int p2g_synchronize(p2g_ctx *ctx) {
     if (!c) return P2G_ERR_INVALID_ARGUMENT;
     hipSetDevice(c->device);
     hipError_t e = hipStreamSynchronize(c->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(c->side);
+    if (e == hipSuccess && c->side) e = hipStreamSynchronize(c->side);
     if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "synchronize", e);
     return P2G_OK;
 }
@@ -474,7 +487,10 @@ int p2g_encode_push(p2g_job *job, uint32_t first_input_index, const uint8_t *dat
         if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "event create", e);
     }
     if (sg.buf.reserve(len) != P2G_OK) return c->fail(P2G_ERR_OUT_OF_MEMORY, "staging buffer");
-    e = hipMemcpyAsync(sg.buf.ptr, data, len, hipMemcpyHostToDevice, c->side);
+    hipStream_t side;
+    int rc = c->side_stream(&side);
+    if (rc != P2G_OK) return rc;
+    e = hipMemcpyAsync(sg.buf.ptr, data, len, hipMemcpyHostToDevice, side);
     if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "upload slices", e);
     // Descriptors of this push go to their own range of the job's tables,
     // which no other push touches.
@@ -485,12 +501,12 @@ int p2g_encode_push(p2g_job *job, uint32_t first_input_index, const uint8_t *dat
         hd[s] = {base + s * j->S, (uint32_t)(have < j->S ? have : j->S), 0};
     }
     SliceDesc *dd = static_cast<SliceDesc *>(j->d_descs.ptr) + first_input_index;
-    e = hipMemcpyAsync(dd, hd, n * sizeof(SliceDesc), hipMemcpyHostToDevice, c->side);
-    if (e == hipSuccess) e = hipEventRecord(sg.copied, c->side);
+    e = hipMemcpyAsync(dd, hd, n * sizeof(SliceDesc), hipMemcpyHostToDevice, side);
+    if (e == hipSuccess) e = hipEventRecord(sg.copied, side);
     if (e == hipSuccess) e = hipStreamWaitEvent(c->stream, sg.copied, 0);
     if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "upload slice descriptors", e);
-    int rc = dev_encode(c, dd, (uint32_t)n, j->S, first_input_index, j->N, j->R,
-                        static_cast<uint8_t *>(j->d_rec.ptr), 1, c->stream);
+    rc = dev_encode(c, dd, (uint32_t)n, j->S, first_input_index, j->N, j->R,
+                    static_cast<uint8_t *>(j->d_rec.ptr), 1, c->stream);
     if (rc != P2G_OK) return rc;
     e = hipEventRecord(sg.done, c->stream);
     if (e == hipSuccess) e = hipEventSynchronize(sg.copied);

@@ -47,7 +47,7 @@ struct Ctx {
     int device = 0;
     int encode_variant = kEncodeMfma;
     hipStream_t stream = nullptr;  // kernels and copies of this context
-    hipStream_t side = nullptr;    // work overlapped with `stream`
+    hipStream_t side = nullptr;    // copies overlapped with `stream`; see side_stream()
     std::string err;
 
     std::vector<uint16_t> h_log, h_antilog;
@@ -59,6 +59,7 @@ struct Ctx {
     DescSlot sum_descs;  // slice checksum launches
 
     int open(int dev);
+    int side_stream(hipStream_t *out);
     ~Ctx();
     int fail(int code, const char *what, hipError_t e = hipSuccess);
 };

@@ -43,6 +43,7 @@ hipError_t launch_encode(const SliceDesc *d_inputs, uint32_t n_inputs, const uin
 // Matrix-core encode.  A fragments for `n` inputs (first input's L at
 // d_logbase[0]) and R recovery rows: mfma_row_tiles(R) * ceil(n / 4) * 1 KiB.
 inline uint32_t mfma_row_tiles(uint32_t R) { return (R + 15) / 16 * 8; }
+constexpr uint32_t kMfmaMaxInputsPerLaunch = 2048;  // LDS descriptor table of k_encode_mfma
 inline size_t mfma_afrag_bytes(uint32_t n, uint32_t R) {
     return (size_t)mfma_row_tiles(R) * ((n + 3) / 4) * 1024;
 }

@@ -277,16 +277,42 @@ hipError_t launch_mfma_afrag(const uint16_t *d_antilog, const uint16_t *d_logbas
     return hipGetLastError();
 }
 
+// LDS: input descriptors of the launch (at most kMfMaxInputs) and a double
+// buffer of the workgroup's A fragments for one K step (8 tiles x 1 KiB).
+constexpr int kMfMaxInputs = 2048;
+
 __global__ __launch_bounds__(kMfThreads, 1) void k_encode_mfma(
     const SliceDesc *__restrict__ inputs, uint32_t n, const uint4 *__restrict__ afrag,
     uint32_t nq, uint32_t R, uint32_t S, uint8_t *__restrict__ rec, int accumulate) {
-    const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    __shared__ SliceDesc s_desc[kMfMaxInputs];
+    __shared__ uint4 s_a[2][kMfTiles * 64];
+
+    const uint32_t tid = threadIdx.x;
+    const uint32_t lane = tid & 63, w = tid >> 6;
     const uint32_t c = lane & 31, h = lane >> 5;
     // this lane's dword: symbols base + 2c (column tile 0) and base + 2c + 1 (tile 1)
     const uint32_t off = (blockIdx.x * (kMfThreads) + w * 64) * 2 + 4 * c;
     const bool col_ok = off < S;
     const uint32_t tile0 = blockIdx.y * kMfTiles;
-    const uint4 *ap = afrag + (size_t)tile0 * nq * 64 + lane;
+
+    for (uint32_t i = tid; i < n; i += kMfThreads) s_desc[i] = inputs[i];
+
+    // Thread tid stages A entries tid and tid + 256 of each K step: tile
+    // (e / 64), lane (e % 64) of fragment (tile0 + tile, g).
+    const uint4 *ga0 = afrag + ((size_t)(tile0 + (tid >> 6)) * nq) * 64 + lane;
+    const uint4 *ga1 = afrag + ((size_t)(tile0 + 4 + (tid >> 6)) * nq) * 64 + lane;
+
+    auto load_words = [&](uint32_t g, uint32_t wd[2]) {
+#pragma unroll
+        for (int e = 0; e < 2; e++) {
+            const uint32_t i = 4 * g + 2 * h + e;
+            wd[e] = 0;
+            if (i < n && col_ok) {
+                const SliceDesc d = s_desc[i];
+                wd[e] = load_word(d.ptr, off, d.valid);
+            }
+        }
+    };
 
     v16f_t acc[kMfTiles][2];
 #pragma unroll
@@ -296,36 +322,55 @@ __global__ __launch_bounds__(kMfThreads, 1) void k_encode_mfma(
 #pragma unroll
             for (int v = 0; v < 16; v++) acc[a][cb][v] = 0.0f;
 
+    __syncthreads();  // s_desc ready
+
+    // prologue: A(0) into LDS, A(1) and the words of steps 0 and 1 in registers
+    uint4 pa0 = ga0[0], pa1 = ga1[0];
+    s_a[0][tid] = pa0;
+    s_a[0][tid + 256] = pa1;
+    if (nq > 1) { pa0 = ga0[64]; pa1 = ga1[64]; }
+    uint32_t w0[2], w1[2];
+    load_words(0, w0);
+    if (nq > 1) load_words(1, w1);
+    __syncthreads();
+
     for (uint32_t g = 0; g < nq; g++) {
-        uint4 A[kMfTiles];
-#pragma unroll
-        for (int a = 0; a < kMfTiles; a++) A[a] = ap[((size_t)a * nq + g) * 64];
-        uint32_t word[2];
-#pragma unroll
-        for (int e = 0; e < 2; e++) {
-            const uint32_t i = 4 * g + 2 * h + e;
-            word[e] = 0;
-            if (i < n && col_ok) {
-                const SliceDesc d = inputs[i];
-                word[e] = load_word(d.ptr, off, d.valid);
-            }
+        // loads for step g + 2
+        uint4 qa0 = pa0, qa1 = pa1;
+        uint32_t w2[2] = {0, 0};
+        if (g + 2 < nq) {
+            qa0 = ga0[(size_t)(g + 2) * 64];
+            qa1 = ga1[(size_t)(g + 2) * 64];
+            load_words(g + 2, w2);
         }
+        const uint4 *sa = s_a[g & 1];
         v8i_t B[2];
 #pragma unroll
         for (int cb = 0; cb < 2; cb++) {
-            const uint32_t s0 = (word[0] >> (16 * cb)) & 0xFFFFu;
-            const uint32_t s1 = (word[1] >> (16 * cb)) & 0xFFFFu;
+            const uint32_t s0 = (w0[0] >> (16 * cb)) & 0xFFFFu;
+            const uint32_t s1 = (w0[1] >> (16 * cb)) & 0xFFFFu;
             B[cb] = v8i_t{(int)spread8(s0), (int)spread8(s0 >> 8), (int)spread8(s1),
                           (int)spread8(s1 >> 8), 0, 0, 0, 0};
         }
 #pragma unroll
         for (int a = 0; a < kMfTiles; a++) {
-            const v8i_t av = v8i_t{(int)A[a].x, (int)A[a].y, (int)A[a].z, (int)A[a].w, 0, 0, 0, 0};
+            const uint4 A = sa[a * 64 + lane];
+            const v8i_t av = v8i_t{(int)A.x, (int)A.y, (int)A.z, (int)A.w, 0, 0, 0, 0};
 #pragma unroll
             for (int cb = 0; cb < 2; cb++)
                 acc[a][cb] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(
                     av, B[cb], acc[a][cb], 4, 4, 0, kE8M0One, 0, kE8M0One);
         }
+        if (g + 1 < nq) {
+            uint4 *da = s_a[(g + 1) & 1];
+            da[tid] = pa0;
+            da[tid + 256] = pa1;
+        }
+        __syncthreads();
+        pa0 = qa0;
+        pa1 = qa1;
+        w0[0] = w1[0]; w0[1] = w1[1];
+        w1[0] = w2[0]; w1[1] = w2[1];
     }
 
     // Epilogue.  Accumulator v of a lane holds row 8 (v / 4) + 4 h + v % 4 of
@@ -364,6 +409,7 @@ hipError_t launch_encode_mfma(const SliceDesc *d_inputs, uint32_t n_inputs, cons
                               uint32_t R, uint32_t S, uint8_t *d_rec, int accumulate,
                               hipStream_t stream) {
     if (R == 0) return hipSuccess;
+    if (n_inputs > (uint32_t)kMfMaxInputs) return hipErrorInvalidValue;  // caller chunks
     const uint32_t nq = (n_inputs + 3) / 4;
     const uint32_t per_block = kMfThreads * 2;  // bytes of symbol columns per workgroup
     dim3 grid((S + per_block - 1) / per_block, mfma_row_tiles(R) / kMfTiles);
@@ -431,31 +477,70 @@ __device__ __forceinline__ uint32_t md5_pad_word(uint64_t o, uint64_t len, uint3
     return 0u;
 }
 
+// Bulk blocks: nblk whole 64-byte blocks at p, loaded kMd5Ahead blocks ahead of
+// the block being compressed so that the loads of one lane stream behind
+// its (serial) compression.  W is uint4 when p is 16-byte aligned, uint32_t
+// otherwise.
+constexpr int kMd5Ahead = 4;
+
+template <typename W>
+__device__ __forceinline__ void md5_load_block(uint32_t x[16], const W *p) {
+    constexpr int n = 16 / (sizeof(W) / 4);
+#pragma unroll
+    for (int k = 0; k < n; k++) {
+        const W v = p[k];
+        memcpy(&x[k * (sizeof(W) / 4)], &v, sizeof(W));
+    }
+}
+
+template <typename W>
+__device__ __forceinline__ void md5_bulk(uint32_t h[4], const uint8_t *p, uint32_t nblk) {
+    constexpr int kW = 64 / sizeof(W);  // W elements per block
+    const W *q = reinterpret_cast<const W *>(p);
+    uint32_t buf[kMd5Ahead][16];
+#pragma unroll
+    for (int k = 0; k < kMd5Ahead; k++)
+        if ((uint32_t)k < nblk) md5_load_block(buf[k], q + (size_t)k * kW);
+    uint32_t b = 0;
+    for (; b + kMd5Ahead <= nblk; b += kMd5Ahead) {
+#pragma unroll
+        for (int k = 0; k < kMd5Ahead; k++) {
+            uint32_t x[16];
+#pragma unroll
+            for (int w = 0; w < 16; w++) x[w] = buf[k][w];
+            const uint32_t nxt = b + k + kMd5Ahead;
+            if (nxt < nblk) md5_load_block(buf[k], q + (size_t)nxt * kW);
+            md5_compress(h, x);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < kMd5Ahead; k++)
+        if (b + k < nblk) md5_compress(h, buf[k]);
+}
+
 // ======================================================================
 // K2a: slice MD5, one lane per slice
 // ======================================================================
 
+template <bool A16>
 __global__ __launch_bounds__(64) void k_slice_md5(const SliceDesc *__restrict__ slices, uint32_t n,
                                                   uint32_t S, uint8_t *__restrict__ md5_out) {
     const uint32_t s = blockIdx.x * 64 + threadIdx.x;
     if (s >= n) return;
     const SliceDesc d = slices[s];
     const uint32_t nb = md5_blocks(S);
+    const uint32_t nbulk = d.valid / 64;  // blocks of real data, no padding
     uint32_t h[4];
     md5_init(h);
-    for (uint32_t b = 0; b < nb; b++) {
+    if (A16) md5_bulk<uint4>(h, d.ptr, nbulk);
+    else md5_bulk<uint32_t>(h, d.ptr, nbulk);
+    for (uint32_t b = nbulk; b < nb; b++) {
         const uint32_t base = b * 64u;
         uint32_t x[16];
-        if (base + 64 <= d.valid) {
-            const uint32_t *p = reinterpret_cast<const uint32_t *>(d.ptr + base);
 #pragma unroll
-            for (int w = 0; w < 16; w++) x[w] = p[w];
-        } else {
-#pragma unroll
-            for (int w = 0; w < 16; w++) {
-                const uint32_t o = base + 4u * w;
-                x[w] = (o < S) ? load_word(d.ptr, o, d.valid) : md5_pad_word(o, S, w, b, nb);
-            }
+        for (int w = 0; w < 16; w++) {
+            const uint32_t o = base + 4u * w;
+            x[w] = (o < S) ? load_word(d.ptr, o, d.valid) : md5_pad_word(o, S, w, b, nb);
         }
         md5_compress(h, x);
     }
@@ -465,8 +550,13 @@ __global__ __launch_bounds__(64) void k_slice_md5(const SliceDesc *__restrict__ 
 hipError_t launch_slice_md5(const SliceDesc *d_slices, uint32_t n, uint32_t S, uint8_t *d_md5,
                             hipStream_t stream) {
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_slice_md5, dim3((n + 63) / 64), dim3(64), 0, stream, d_slices, n, S,
-                       d_md5);
+    // Slices are 16-byte aligned when S is a multiple of 16 (file bases are).
+    if (S % 16 == 0)
+        hipLaunchKernelGGL(k_slice_md5<true>, dim3((n + 63) / 64), dim3(64), 0, stream, d_slices,
+                           n, S, d_md5);
+    else
+        hipLaunchKernelGGL(k_slice_md5<false>, dim3((n + 63) / 64), dim3(64), 0, stream, d_slices,
+                           n, S, d_md5);
     return hipGetLastError();
 }
 
@@ -580,29 +670,29 @@ __global__ __launch_bounds__(64) void k_recovery_md5(const uint8_t *__restrict__
     const uint8_t *data = rec + (size_t)r * S;
     const uint64_t len = 36ull + S;
     const uint32_t nb = md5_blocks(len);
+    // Blocks 1 .. nfull-1 lie wholly inside the block data (message offset
+    // 64 b - 36); block 0 holds the 36-byte prefix, the rest the padding.
+    const uint32_t nfull = (uint32_t)(len / 64);
     uint32_t h[4];
     md5_init(h);
-    for (uint32_t b = 0; b < nb; b++) {
+    auto slow_block = [&](uint32_t b) {
         const uint64_t base = (uint64_t)b * 64u;
         uint32_t x[16];
-        if (base >= 36 && base + 64 <= len) {
-            const uint32_t *p = reinterpret_cast<const uint32_t *>(data + (base - 36));
 #pragma unroll
-            for (int w = 0; w < 16; w++) x[w] = p[w];
-        } else {
-#pragma unroll
-            for (int w = 0; w < 16; w++) {
-                const uint64_t o = base + 4u * w;
-                uint32_t v;
-                if (o < 32) v = pre.w[o / 4];
-                else if (o < 36) v = first_exp + r;
-                else if (o < len) v = *reinterpret_cast<const uint32_t *>(data + (o - 36));
-                else v = md5_pad_word(o, len, w, b, nb);
-                x[w] = v;
-            }
+        for (int w = 0; w < 16; w++) {
+            const uint64_t o = base + 4u * w;
+            uint32_t v;
+            if (o < 32) v = pre.w[o / 4];
+            else if (o < 36) v = first_exp + r;
+            else if (o < len) v = *reinterpret_cast<const uint32_t *>(data + (o - 36));
+            else v = md5_pad_word(o, len, w, b, nb);
+            x[w] = v;
         }
         md5_compress(h, x);
-    }
+    };
+    slow_block(0);
+    if (nfull > 1) md5_bulk<uint32_t>(h, data + 28, nfull - 1);
+    for (uint32_t b = nfull > 1 ? nfull : 1; b < nb; b++) slow_block(b);
     store_digest(md5_out + (size_t)r * 16, h);
 }
 
