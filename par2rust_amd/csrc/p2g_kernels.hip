@@ -432,7 +432,10 @@ __constant__ uint32_t kMd5K[64] = {
     0xf4292244u, 0x432aff97u, 0xab9423a7u, 0xfc93a039u, 0x655b59c3u, 0x8f0ccc92u, 0xffeff47du, 0x85845dd1u,
     0x6fa87e4fu, 0xfe2ce6e0u, 0xa3014314u, 0x4e0811a1u, 0xf7537e82u, 0xbd3af235u, 0x2ad7d2bbu, 0xeb86d391u};
 
-__device__ __forceinline__ uint32_t rotl32(uint32_t x, int s) { return (x << s) | (x >> (32 - s)); }
+// Rotate left by s (1..31) in one v_alignbit_b32: (x:x) >> (32 - s).
+__device__ __forceinline__ uint32_t rotl32(uint32_t x, int s) {
+    return __builtin_amdgcn_alignbit(x, x, 32 - s);
+}
 
 __device__ __forceinline__ void md5_compress(uint32_t h[4], const uint32_t x[16]) {
     constexpr int sh[16] = {7, 12, 17, 22, 5, 9, 14, 20, 4, 11, 16, 23, 6, 10, 15, 21};
@@ -481,7 +484,7 @@ __device__ __forceinline__ uint32_t md5_pad_word(uint64_t o, uint64_t len, uint3
 // the block being compressed so that the loads of one lane stream behind
 // its (serial) compression.  W is uint4 when p is 16-byte aligned, uint32_t
 // otherwise.
-constexpr int kMd5Ahead = 4;
+constexpr int kMd5Ahead = 8;
 
 template <typename W>
 __device__ __forceinline__ void md5_load_block(uint32_t x[16], const W *p) {
