@@ -521,93 +521,12 @@ __device__ __forceinline__ void md5_bulk(uint32_t h[4], const uint8_t *p, uint32
         if (b + k < nblk) md5_compress(h, buf[k]);
 }
 
-// ======================================================================
-// Staged multi-buffer MD5: one wave hashes 64 independent streams
-// ======================================================================
-//
-// Lane l owns stream l.  The streams' bytes reach LDS through coalesced
-// direct-to-LDS loads (each instruction fetches 1 KiB of one stream, or 256 B
-// in the dword form), chunk c + 1 being fetched during the 16 block steps of
-// chunk c; each lane then compresses its own blocks from LDS.  This replaces
-// 64 scattered 64-byte reads per instruction by contiguous ones.
-
-typedef __attribute__((address_space(1))) void gvoid_t;
-typedef __attribute__((address_space(3))) void lvoid_t;
-
-constexpr uint32_t kStgChunk = 1024;          // stream bytes per chunk: 16 blocks
-constexpr uint32_t kStgRow = kStgChunk + 16;  // LDS bytes per stream row (16-byte aligned)
-constexpr uint32_t kStgBytes = 2 * 64 * kStgRow;
-
-// Direct-to-LDS loads of chunk c of one stream (wave-uniform ps, vs, row).
-// Bytes at or past vs are not fetched.
-template <bool X4>
-__device__ __forceinline__ void stage_stream(const uint8_t *ps, uint32_t vs, uint32_t c,
-                                             uint8_t *row, uint32_t lane) {
-    if (X4) {
-        const uint32_t o = c * kStgChunk + 16 * lane;
-        if (o + 16 <= vs)
-            __builtin_amdgcn_global_load_lds((gvoid_t *)(ps + o), (lvoid_t *)row, 16, 0, 0);
-    } else {
-#pragma unroll
-        for (uint32_t k = 0; k < 4; k++) {
-            const uint32_t o = c * kStgChunk + 256 * k + 4 * lane;
-            if (o + 4 <= vs)
-                __builtin_amdgcn_global_load_lds((gvoid_t *)(ps + o), (lvoid_t *)(row + 256 * k),
-                                                 4, 0, 0);
-        }
-    }
-}
-
-__device__ __forceinline__ const uint8_t *readlane_ptr(const uint8_t *p, uint32_t l) {
-    const uint64_t v = reinterpret_cast<uint64_t>(p);
-    const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)v, l);
-    const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(v >> 32), l);
-    return reinterpret_cast<const uint8_t *>(((uint64_t)hi << 32) | lo);
-}
-
-// Hashes blocks [0, nblk) of this lane's stream p (64 * nblk <= valid).  Every
-// lane of the wave must call it (lanes without work pass nblk = 0).  X4:
-// 16-byte loads (stream pointers 16-byte aligned), else 4-byte loads.
-template <bool X4>
-__device__ void md5_staged(uint32_t h[4], const uint8_t *p, uint32_t valid, uint32_t nblk,
-                           uint8_t *lds) {
-    const uint32_t lane = threadIdx.x & 63;
-    uint32_t nmax = nblk;
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) nmax = max(nmax, (uint32_t)__shfl_xor((int)nmax, o, 64));
-    nmax = __builtin_amdgcn_readfirstlane(nmax);
-    const uint32_t nch = (nmax + 15) / 16;
-    if (nch == 0) return;
-    for (uint32_t s = 0; s < 64; s++)
-        stage_stream<X4>(readlane_ptr(p, s), __builtin_amdgcn_readlane(valid, s), 0,
-                         lds + s * kStgRow, lane);
-    for (uint32_t c = 0; c < nch; c++) {
-        __builtin_amdgcn_s_waitcnt(0);  // this chunk's loads have landed (vmcnt = lgkmcnt = 0)
-        uint8_t *buf = lds + (c & 1) * 64 * kStgRow;
-        uint8_t *nbuf = lds + ((c + 1) & 1) * 64 * kStgRow;
-        const uint4 *mine = reinterpret_cast<const uint4 *>(buf + lane * kStgRow);
-        for (uint32_t k = 0; k < 16; k++) {
-            if (c + 1 < nch) {
-#pragma unroll
-                for (uint32_t q = 0; q < 4; q++) {
-                    const uint32_t s = 4 * k + q;
-                    stage_stream<X4>(readlane_ptr(p, s), __builtin_amdgcn_readlane(valid, s),
-                                     c + 1, nbuf + s * kStgRow, lane);
-                }
-            }
-            const uint32_t b = 16 * c + k;
-            if (b < nblk) {
-                uint32_t x[16];
-#pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    const uint4 v = mine[4 * k + q];
-                    x[4 * q] = v.x; x[4 * q + 1] = v.y; x[4 * q + 2] = v.z; x[4 * q + 3] = v.w;
-                }
-                md5_compress(h, x);
-            }
-        }
-    }
-}
+// One MD5 wave per CU: each lane streams its own message, so the waves'
+// scattered 64-byte reads would otherwise share (and thrash) a CU's L1 and
+// address unit.  The launches request this much dynamic LDS (unused) per
+// single-wave workgroup, which keeps the dispatcher from placing two of
+// them on one CU.
+constexpr uint32_t kMd5LdsReserve = 96 * 1024;
 
 // ======================================================================
 // K2a: slice MD5, one lane per slice
@@ -616,17 +535,15 @@ __device__ void md5_staged(uint32_t h[4], const uint8_t *p, uint32_t valid, uint
 template <bool A16>
 __global__ __launch_bounds__(64) void k_slice_md5(const SliceDesc *__restrict__ slices, uint32_t n,
                                                   uint32_t S, uint8_t *__restrict__ md5_out) {
-    __shared__ __attribute__((aligned(16))) uint8_t lds[kStgBytes];
     const uint32_t s = blockIdx.x * 64 + threadIdx.x;
-    const bool live = s < n;  // dead lanes still take part in the staging loop
-    SliceDesc d{nullptr, 0, 0};
-    if (live) d = slices[s];
+    if (s >= n) return;
+    const SliceDesc d = slices[s];
     const uint32_t nb = md5_blocks(S);
     const uint32_t nbulk = d.valid / 64;  // whole blocks of real data
     uint32_t h[4];
     md5_init(h);
-    md5_staged<A16>(h, d.ptr, d.valid, nbulk, lds);
-    if (!live) return;
+    if (A16) md5_bulk<uint4>(h, d.ptr, nbulk);
+    else md5_bulk<uint32_t>(h, d.ptr, nbulk);
     for (uint32_t b = nbulk; b < nb; b++) {
         const uint32_t base = b * 64u;
         uint32_t x[16];
@@ -645,11 +562,11 @@ hipError_t launch_slice_md5(const SliceDesc *d_slices, uint32_t n, uint32_t S, u
     if (n == 0) return hipSuccess;
     // Slices are 16-byte aligned when S is a multiple of 16 (file bases are).
     if (S % 16 == 0)
-        hipLaunchKernelGGL(k_slice_md5<true>, dim3((n + 63) / 64), dim3(64), 0, stream, d_slices,
-                           n, S, d_md5);
+        hipLaunchKernelGGL(k_slice_md5<true>, dim3((n + 63) / 64), dim3(64), kMd5LdsReserve, stream,
+                           d_slices, n, S, d_md5);
     else
-        hipLaunchKernelGGL(k_slice_md5<false>, dim3((n + 63) / 64), dim3(64), 0, stream, d_slices,
-                           n, S, d_md5);
+        hipLaunchKernelGGL(k_slice_md5<false>, dim3((n + 63) / 64), dim3(64), kMd5LdsReserve,
+                           stream, d_slices, n, S, d_md5);
     return hipGetLastError();
 }
 
@@ -758,10 +675,9 @@ struct PacketPrefix {
 __global__ __launch_bounds__(64) void k_recovery_md5(const uint8_t *__restrict__ rec,
                                                      uint32_t count, uint32_t S, uint32_t first_exp,
                                                      PacketPrefix pre, uint8_t *__restrict__ md5_out) {
-    __shared__ __attribute__((aligned(16))) uint8_t lds[kStgBytes];
     const uint32_t r = blockIdx.x * 64 + threadIdx.x;
-    const bool live = r < count;
-    const uint8_t *data = rec + (size_t)(live ? r : 0) * S;
+    if (r >= count) return;
+    const uint8_t *data = rec + (size_t)r * S;
     const uint64_t len = 36ull + S;
     const uint32_t nb = md5_blocks(len);
     // Block 0 holds the 36-byte prefix and data bytes 0..27; blocks 1 .. nfull-1
@@ -784,10 +700,8 @@ __global__ __launch_bounds__(64) void k_recovery_md5(const uint8_t *__restrict__
         }
         md5_compress(h, x);
     };
-    if (live) slow_block(0);
-    const uint32_t nstage = nfull > 1 ? nfull - 1 : 0;
-    md5_staged<false>(h, data + 28, live ? S - 28 : 0, live ? nstage : 0, lds);
-    if (!live) return;
+    slow_block(0);
+    if (nfull > 1) md5_bulk<uint32_t>(h, data + 28, nfull - 1);
     for (uint32_t b = nfull > 1 ? nfull : 1; b < nb; b++) slow_block(b);
     store_digest(md5_out + (size_t)r * 16, h);
 }
@@ -805,8 +719,8 @@ hipError_t launch_recovery_md5(const uint8_t *d_rec, uint32_t count, uint32_t S,
         pre.w[4 + w] = (uint32_t)type_tag[4 * w] | ((uint32_t)type_tag[4 * w + 1] << 8) |
                        ((uint32_t)type_tag[4 * w + 2] << 16) | ((uint32_t)type_tag[4 * w + 3] << 24);
     }
-    hipLaunchKernelGGL(k_recovery_md5, dim3((count + 63) / 64), dim3(64), 0, stream, d_rec, count,
-                       S, first_exp, pre, d_md5);
+    hipLaunchKernelGGL(k_recovery_md5, dim3((count + 63) / 64), dim3(64), kMd5LdsReserve, stream,
+                       d_rec, count, S, first_exp, pre, d_md5);
     return hipGetLastError();
 }
 
