@@ -232,6 +232,8 @@ int dev_encode(Ctx *c, const SliceDesc *d_descs, uint32_t n, uint64_t S, uint32_
         }
         return P2G_OK;
     }
+    e = launch_desc_tails(const_cast<SliceDesc *>(d_descs), n, st);
+    if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "descriptor tail kernel", e);
     // Matrix-core form: A fragments (1 KiB per 32 rows x 4 inputs) for a chunk
     // of inputs, a multiple of 4, within kMatsBudget.
     const uint64_t per_quad = (uint64_t)mfma_row_tiles(R) * 1024;

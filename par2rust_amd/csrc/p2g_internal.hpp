@@ -18,7 +18,7 @@ namespace p2g {
 struct SliceDesc {
     const uint8_t *ptr;
     uint32_t valid;
-    uint32_t pad_;
+    uint32_t tail;  // bytes [valid & ~3, valid) as a zero-padded word (set on the device)
 };
 static_assert(sizeof(SliceDesc) == 16, "SliceDesc must stay 16 bytes");
 
@@ -39,6 +39,9 @@ hipError_t launch_coeff_mats(const uint16_t *d_antilog, const uint16_t *d_logbas
 hipError_t launch_encode(const SliceDesc *d_inputs, uint32_t n_inputs, const uint16_t *d_mats,
                          uint32_t R, uint32_t S, uint8_t *d_rec, int accumulate,
                          hipStream_t stream);
+
+// Fills SliceDesc::tail of n descriptors in place.
+hipError_t launch_desc_tails(SliceDesc *d_descs, uint32_t n, hipStream_t stream);
 
 // Matrix-core encode.  A fragments for `n` inputs (first input's L at
 // d_logbase[0]) and R recovery rows: mfma_row_tiles(R) * ceil(n / 4) * 1 KiB.

@@ -69,6 +69,24 @@ __device__ __forceinline__ uint32_t load_word(const uint8_t *p, uint32_t off, ui
     return w;
 }
 
+// Tail word of each descriptor: the bytes of a trailing partial dword, so
+// that word loads never touch memory past `valid`.
+__global__ __launch_bounds__(256) void k_desc_tails(SliceDesc *__restrict__ d, uint32_t n) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t v = d[i].valid;
+    d[i].tail = (v & 3) ? load_word(d[i].ptr, v & ~3u, v) : 0u;
+}
+
+hipError_t launch_desc_tails(SliceDesc *d_descs, uint32_t n, hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_desc_tails, dim3((n + 255) / 256), dim3(256), 0, stream, d_descs, n);
+    return hipGetLastError();
+}
+
+// A word that is always zero: the target of masked-off word loads.
+__device__ const uint32_t kZeroWord = 0;
+
 // ======================================================================
 // K1: GF(2^16) accumulator on bit-sliced symbols
 // ======================================================================
@@ -302,15 +320,19 @@ __global__ __launch_bounds__(kMfThreads, 1) void k_encode_mfma(
     const uint4 *ga0 = afrag + ((size_t)(tile0 + (tid >> 6)) * nq) * 64 + lane;
     const uint4 *ga1 = afrag + ((size_t)(tile0 + 4 + (tid >> 6)) * nq) * 64 + lane;
 
+    // Branch-free word loads: whole words come from the slice, the partial
+    // last word from the descriptor, anything else reads kZeroWord.
     auto load_words = [&](uint32_t g, uint32_t wd[2]) {
 #pragma unroll
         for (int e = 0; e < 2; e++) {
             const uint32_t i = 4 * g + 2 * h + e;
-            wd[e] = 0;
-            if (i < n && col_ok) {
-                const SliceDesc d = s_desc[i];
-                wd[e] = load_word(d.ptr, off, d.valid);
-            }
+            const SliceDesc d = s_desc[min(i, n - 1)];
+            const bool in = i < n && col_ok;
+            const bool full = in && off + 4 <= d.valid;
+            const bool part = in && !full && off < d.valid;
+            const uint32_t *src = full ? reinterpret_cast<const uint32_t *>(d.ptr + off) : &kZeroWord;
+            const uint32_t v = *src;
+            wd[e] = part ? d.tail : v;
         }
     };
 
@@ -324,53 +346,68 @@ __global__ __launch_bounds__(kMfThreads, 1) void k_encode_mfma(
 
     __syncthreads();  // s_desc ready
 
-    // prologue: A(0) into LDS, A(1) and the words of steps 0 and 1 in registers
+    auto make_b = [&](const uint32_t wd[2], v8i_t B[2]) {
+#pragma unroll
+        for (int cb = 0; cb < 2; cb++) {
+            const uint32_t s0 = (wd[0] >> (16 * cb)) & 0xFFFFu;
+            const uint32_t s1 = (wd[1] >> (16 * cb)) & 0xFFFFu;
+            B[cb] = v8i_t{(int)spread8(s0), (int)spread8(s0 >> 8), (int)spread8(s1),
+                          (int)spread8(s1 >> 8), 0, 0, 0, 0};
+        }
+    };
+
+    // Software pipeline.  Entering iteration g: A(g) is in LDS buffer g & 1,
+    // B(g) is built, A(g + 1) waits in registers (pa*), the words of step
+    // g + 1 in w1.  The iteration issues the loads of step g + 2, builds
+    // B(g + 1) while the MFMAs of step g run, then stores A(g + 1) to the
+    // other LDS buffer.
     uint4 pa0 = ga0[0], pa1 = ga1[0];
     s_a[0][tid] = pa0;
     s_a[0][tid + 256] = pa1;
     if (nq > 1) { pa0 = ga0[64]; pa1 = ga1[64]; }
-    uint32_t w0[2], w1[2];
+    uint32_t w0[2], w1[2] = {0, 0};
     load_words(0, w0);
     if (nq > 1) load_words(1, w1);
+    v8i_t Bc[2];
+    make_b(w0, Bc);
     __syncthreads();
 
     for (uint32_t g = 0; g < nq; g++) {
-        // loads for step g + 2
-        uint4 qa0 = pa0, qa1 = pa1;
-        uint32_t w2[2] = {0, 0};
-        if (g + 2 < nq) {
-            qa0 = ga0[(size_t)(g + 2) * 64];
-            qa1 = ga1[(size_t)(g + 2) * 64];
-            load_words(g + 2, w2);
-        }
+        // loads of step g + 2 (the last step's loads repeat harmlessly)
+        const uint32_t g2 = min(g + 2, nq - 1);
+        const uint4 qa0 = ga0[(size_t)g2 * 64], qa1 = ga1[(size_t)g2 * 64];
+        uint32_t w2[2];
+        load_words(g2, w2);
         const uint4 *sa = s_a[g & 1];
-        v8i_t B[2];
+        uint4 A[kMfTiles];
 #pragma unroll
-        for (int cb = 0; cb < 2; cb++) {
-            const uint32_t s0 = (w0[0] >> (16 * cb)) & 0xFFFFu;
-            const uint32_t s1 = (w0[1] >> (16 * cb)) & 0xFFFFu;
-            B[cb] = v8i_t{(int)spread8(s0), (int)spread8(s0 >> 8), (int)spread8(s1),
-                          (int)spread8(s1 >> 8), 0, 0, 0, 0};
-        }
+        for (int a = 0; a < kMfTiles; a++) A[a] = sa[a * 64 + lane];
+        v8i_t Bn[2];
+        make_b(w1, Bn);
 #pragma unroll
         for (int a = 0; a < kMfTiles; a++) {
-            const uint4 A = sa[a * 64 + lane];
-            const v8i_t av = v8i_t{(int)A.x, (int)A.y, (int)A.z, (int)A.w, 0, 0, 0, 0};
+            const v8i_t av = v8i_t{(int)A[a].x, (int)A[a].y, (int)A[a].z, (int)A[a].w, 0, 0, 0, 0};
 #pragma unroll
             for (int cb = 0; cb < 2; cb++)
                 acc[a][cb] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(
-                    av, B[cb], acc[a][cb], 4, 4, 0, kE8M0One, 0, kE8M0One);
+                    av, Bc[cb], acc[a][cb], 4, 4, 0, kE8M0One, 0, kE8M0One);
         }
-        if (g + 1 < nq) {
-            uint4 *da = s_a[(g + 1) & 1];
-            da[tid] = pa0;
-            da[tid + 256] = pa1;
+        // interleave: each MFMA followed by up to 6 VALU (B(g + 1), addresses)
+#pragma unroll
+        for (int k = 0; k < 2 * kMfTiles; k++) {
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // 1 MFMA
+            __builtin_amdgcn_sched_group_barrier(0x002, 6, 0);  // 6 VALU
         }
+        uint4 *da = s_a[(g + 1) & 1];  // unused after the last step
+        da[tid] = pa0;
+        da[tid + 256] = pa1;
         __syncthreads();
         pa0 = qa0;
         pa1 = qa1;
-        w0[0] = w1[0]; w0[1] = w1[1];
-        w1[0] = w2[0]; w1[1] = w2[1];
+        Bc[0] = Bn[0];
+        Bc[1] = Bn[1];
+        w1[0] = w2[0];
+        w1[1] = w2[1];
     }
 
     // Epilogue.  Accumulator v of a lane holds row 8 (v / 4) + 4 h + v % 4 of
