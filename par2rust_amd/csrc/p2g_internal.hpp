@@ -44,11 +44,14 @@ hipError_t launch_encode(const SliceDesc *d_inputs, uint32_t n_inputs, const uin
 hipError_t launch_desc_tails(SliceDesc *d_descs, uint32_t n, hipStream_t stream);
 
 // Matrix-core encode.  A fragments for `n` inputs (first input's L at
-// d_logbase[0]) and R recovery rows: mfma_row_tiles(R) * ceil(n / 4) * 1 KiB.
+// d_logbase[0]) and R recovery rows: mfma_row_tiles(R) * mfma_steps(n) * 1 KiB.
+// A K step covers four inputs; the step count is padded to whole groups of
+// four steps with zero fragments (zero A contributes nothing).
 inline uint32_t mfma_row_tiles(uint32_t R) { return (R + 15) / 16 * 8; }
+inline uint32_t mfma_steps(uint32_t n) { return ((n + 3) / 4 + 3) / 4 * 4; }
 constexpr uint32_t kMfmaMaxInputsPerLaunch = 2048;  // LDS descriptor table of k_encode_mfma
 inline size_t mfma_afrag_bytes(uint32_t n, uint32_t R) {
-    return (size_t)mfma_row_tiles(R) * ((n + 3) / 4) * 1024;
+    return (size_t)mfma_row_tiles(R) * mfma_steps(n) * 1024;
 }
 hipError_t launch_mfma_afrag(const uint16_t *d_antilog, const uint16_t *d_logbase, uint32_t n,
                              uint32_t R, uint4 *d_afrag, hipStream_t stream);

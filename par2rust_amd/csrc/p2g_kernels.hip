@@ -61,11 +61,28 @@ hipError_t launch_coeff_mats(const uint16_t *d_antilog, const uint16_t *d_logbas
 // Bounded loads: bytes at or past `valid` read as zero.
 // ======================================================================
 
+// Slice pointers arrive in descriptors as generic pointers, and a load
+// through a generic pointer is a flat_load, which the wait counters track
+// as both a vector-memory and an LDS operation: every LDS wait after it
+// would also wait for the load.  All source data is global memory, so loads
+// go through the global address space (global_load, vmcnt only).
+template <typename T>
+__device__ __forceinline__ T gload(const T *p) {
+    return *(const __attribute__((address_space(1))) T *)p;
+}
+// uint4's copy constructor takes a generic reference, so it is loaded as a
+// native four-lane vector instead.
+__device__ __forceinline__ uint4 gload(const uint4 *p) {
+    typedef uint32_t v4u_t __attribute__((ext_vector_type(4)));
+    const v4u_t v = *(const __attribute__((address_space(1))) v4u_t *)p;
+    return make_uint4(v.x, v.y, v.z, v.w);
+}
+
 __device__ __forceinline__ uint32_t load_word(const uint8_t *p, uint32_t off, uint32_t valid) {
-    if (off + 4 <= valid) return *reinterpret_cast<const uint32_t *>(p + off);
+    if (off + 4 <= valid) return gload(reinterpret_cast<const uint32_t *>(p + off));
     uint32_t w = 0;
     for (uint32_t b = 0; b < 4; b++)
-        if (off + b < valid) w |= (uint32_t)p[off + b] << (8 * b);
+        if (off + b < valid) w |= (uint32_t)gload(p + off + b) << (8 * b);
     return w;
 }
 
@@ -147,7 +164,7 @@ __global__ __launch_bounds__(kEncThreads) void k_encode_bits(
             const uint4 *p = reinterpret_cast<const uint4 *>(d.ptr + col);
 #pragma unroll
             for (int q = 0; q < 4; q++) {
-                const uint4 v = p[q];
+                const uint4 v = gload(p + q);
                 x[4 * q] = v.x; x[4 * q + 1] = v.y; x[4 * q + 2] = v.z; x[4 * q + 3] = v.w;
             }
         } else {
@@ -286,7 +303,7 @@ __global__ __launch_bounds__(256) void k_mfma_afrag(const uint16_t *__restrict__
 
 hipError_t launch_mfma_afrag(const uint16_t *d_antilog, const uint16_t *d_logbase, uint32_t n,
                              uint32_t R, uint4 *d_afrag, hipStream_t stream) {
-    const uint32_t nq = (n + 3) / 4;
+    const uint32_t nq = mfma_steps(n);
     const uint32_t tiles = mfma_row_tiles(R);
     const uint64_t total = (uint64_t)tiles * nq * 64;
     if (total == 0) return hipSuccess;
@@ -298,6 +315,43 @@ hipError_t launch_mfma_afrag(const uint16_t *d_antilog, const uint16_t *d_logbas
 // LDS: input descriptors of the launch (at most kMfMaxInputs) and a double
 // buffer of the workgroup's A fragments for one K step (8 tiles x 1 KiB).
 constexpr int kMfMaxInputs = 2048;
+constexpr int kMfAhead = 4;  // K steps between issuing a load and consuming it
+
+// Branch-free word loads of one K step: whole words come from the slice,
+// the partial last word from the descriptor's tail field in global memory,
+// anything else reads kZeroWord.  Only the address is selected, so the
+// loaded word is final and nothing waits for it before it is consumed.
+__device__ __forceinline__ void mf_load_words(const SliceDesc *s_desc,
+                                              const SliceDesc *__restrict__ g_desc, uint32_t n,
+                                              uint32_t g, uint32_t h, uint32_t off, bool col_ok,
+                                              uint32_t &w0, uint32_t &w1) {
+    uint32_t wd[2];
+#pragma unroll
+    for (int e = 0; e < 2; e++) {
+        const uint32_t i = 4 * g + 2 * h + e;
+        const uint32_t ic = min(i, n - 1);
+        const SliceDesc d = s_desc[ic];
+        const bool in = i < n && col_ok;
+        const bool full = in && off + 4 <= d.valid;
+        const bool part = in && !full && off < d.valid;
+        const uint32_t *src = full   ? reinterpret_cast<const uint32_t *>(d.ptr + off)
+                              : part ? &g_desc[ic].tail
+                                     : &kZeroWord;
+        wd[e] = gload(src);
+    }
+    w0 = wd[0];
+    w1 = wd[1];
+}
+
+__device__ __forceinline__ void mf_make_b(uint32_t w0, uint32_t w1, v8i_t B[2]) {
+#pragma unroll
+    for (int cb = 0; cb < 2; cb++) {
+        const uint32_t s0 = (w0 >> (16 * cb)) & 0xFFFFu;
+        const uint32_t s1 = (w1 >> (16 * cb)) & 0xFFFFu;
+        B[cb] = v8i_t{(int)spread8(s0), (int)spread8(s0 >> 8), (int)spread8(s1),
+                      (int)spread8(s1 >> 8), 0, 0, 0, 0};
+    }
+}
 
 __global__ __launch_bounds__(kMfThreads, 1) void k_encode_mfma(
     const SliceDesc *__restrict__ inputs, uint32_t n, const uint4 *__restrict__ afrag,
@@ -320,22 +374,6 @@ __global__ __launch_bounds__(kMfThreads, 1) void k_encode_mfma(
     const uint4 *ga0 = afrag + ((size_t)(tile0 + (tid >> 6)) * nq) * 64 + lane;
     const uint4 *ga1 = afrag + ((size_t)(tile0 + 4 + (tid >> 6)) * nq) * 64 + lane;
 
-    // Branch-free word loads: whole words come from the slice, the partial
-    // last word from the descriptor, anything else reads kZeroWord.
-    auto load_words = [&](uint32_t g, uint32_t wd[2]) {
-#pragma unroll
-        for (int e = 0; e < 2; e++) {
-            const uint32_t i = 4 * g + 2 * h + e;
-            const SliceDesc d = s_desc[min(i, n - 1)];
-            const bool in = i < n && col_ok;
-            const bool full = in && off + 4 <= d.valid;
-            const bool part = in && !full && off < d.valid;
-            const uint32_t *src = full ? reinterpret_cast<const uint32_t *>(d.ptr + off) : &kZeroWord;
-            const uint32_t v = *src;
-            wd[e] = part ? d.tail : v;
-        }
-    };
-
     v16f_t acc[kMfTiles][2];
 #pragma unroll
     for (int a = 0; a < kMfTiles; a++)
@@ -346,69 +384,71 @@ __global__ __launch_bounds__(kMfThreads, 1) void k_encode_mfma(
 
     __syncthreads();  // s_desc ready
 
-    auto make_b = [&](const uint32_t wd[2], v8i_t B[2]) {
-#pragma unroll
-        for (int cb = 0; cb < 2; cb++) {
-            const uint32_t s0 = (wd[0] >> (16 * cb)) & 0xFFFFu;
-            const uint32_t s1 = (wd[1] >> (16 * cb)) & 0xFFFFu;
-            B[cb] = v8i_t{(int)spread8(s0), (int)spread8(s0 >> 8), (int)spread8(s1),
-                          (int)spread8(s1 >> 8), 0, 0, 0, 0};
-        }
-    };
-
-    // Software pipeline.  Entering iteration g: A(g) is in LDS buffer g & 1,
-    // B(g) is built, A(g + 1) waits in registers (pa*), the words of step
-    // g + 1 in w1.  The iteration issues the loads of step g + 2, builds
-    // B(g + 1) while the MFMAs of step g run, then stores A(g + 1) to the
-    // other LDS buffer.
-    uint4 pa0 = ga0[0], pa1 = ga1[0];
-    s_a[0][tid] = pa0;
-    s_a[0][tid + 256] = pa1;
-    if (nq > 1) { pa0 = ga0[64]; pa1 = ga1[64]; }
-    uint32_t w0[2], w1[2] = {0, 0};
-    load_words(0, w0);
-    if (nq > 1) load_words(1, w1);
+    // Software pipeline, kMfAhead steps deep.  Ring slot p holds the staged
+    // A entries and source words of a step gg with gg % kMfAhead == p.  Step
+    // gg: issue the loads of step gg + kMfAhead into its own (consumed) slot,
+    // run the MFMAs of step gg from LDS buffer gg & 1 and B(gg) while B(gg + 1)
+    // is built, store A(gg + 1) into the other LDS buffer, barrier.  The slot
+    // index of each step is a literal (MF_STEP), so the ring stays in registers.
+    // slot p: staged A entries a0_p, a1_p and source words w0_p, w1_p
+    uint4 a0_0, a1_0, a0_1, a1_1, a0_2, a1_2, a0_3, a1_3;
+    uint32_t w0_0, w1_0, w0_1, w1_1, w0_2, w1_2, w0_3, w1_3;
+#define MF_FILL(P)                                                                            \
+    do {                                                                                      \
+        const uint32_t gp = min((uint32_t)(P), nq - 1);                                       \
+        a0_##P = ga0[(size_t)gp * 64];                                                        \
+        a1_##P = ga1[(size_t)gp * 64];                                                        \
+        mf_load_words(s_desc, inputs, n, gp, h, off, col_ok, w0_##P, w1_##P);                         \
+    } while (0)
+    MF_FILL(0);
+    MF_FILL(1);
+    MF_FILL(2);
+    MF_FILL(3);
+#undef MF_FILL
+    s_a[0][tid] = a0_0;
+    s_a[0][tid + 256] = a1_0;
     v8i_t Bc[2];
-    make_b(w0, Bc);
+    mf_make_b(w0_0, w1_0, Bc);
     __syncthreads();
 
-    for (uint32_t g = 0; g < nq; g++) {
-        // loads of step g + 2 (the last step's loads repeat harmlessly)
-        const uint32_t g2 = min(g + 2, nq - 1);
-        const uint4 qa0 = ga0[(size_t)g2 * 64], qa1 = ga1[(size_t)g2 * 64];
-        uint32_t w2[2];
-        load_words(g2, w2);
-        const uint4 *sa = s_a[g & 1];
-        uint4 A[kMfTiles];
-#pragma unroll
-        for (int a = 0; a < kMfTiles; a++) A[a] = sa[a * 64 + lane];
-        v8i_t Bn[2];
-        make_b(w1, Bn);
-#pragma unroll
-        for (int a = 0; a < kMfTiles; a++) {
-            const v8i_t av = v8i_t{(int)A[a].x, (int)A[a].y, (int)A[a].z, (int)A[a].w, 0, 0, 0, 0};
-#pragma unroll
-            for (int cb = 0; cb < 2; cb++)
-                acc[a][cb] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(
-                    av, Bc[cb], acc[a][cb], 4, 4, 0, kE8M0One, 0, kE8M0One);
-        }
-        // interleave: each MFMA followed by up to 6 VALU (B(g + 1), addresses)
-#pragma unroll
-        for (int k = 0; k < 2 * kMfTiles; k++) {
-            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // 1 MFMA
-            __builtin_amdgcn_sched_group_barrier(0x002, 6, 0);  // 6 VALU
-        }
-        uint4 *da = s_a[(g + 1) & 1];  // unused after the last step
-        da[tid] = pa0;
-        da[tid + 256] = pa1;
-        __syncthreads();
-        pa0 = qa0;
-        pa1 = qa1;
-        Bc[0] = Bn[0];
-        Bc[1] = Bn[1];
-        w1[0] = w2[0];
-        w1[1] = w2[1];
+#define MF_STEP(GG, P, PN)                                                                    \
+    do {                                                                                      \
+        const uint32_t gf = min((GG) + kMfAhead, nq - 1);                                     \
+        a0_##P = ga0[(size_t)gf * 64];                                                        \
+        a1_##P = ga1[(size_t)gf * 64];                                                        \
+        mf_load_words(s_desc, inputs, n, gf, h, off, col_ok, w0_##P, w1_##P);                         \
+        const uint4 *sa = s_a[(GG) & 1];                                                      \
+        v8i_t Bn[2];                                                                          \
+        mf_make_b(w0_##PN, w1_##PN, Bn);                                                      \
+        _Pragma("unroll") for (int a = 0; a < kMfTiles; a++) {                                \
+            const uint4 A = sa[a * 64 + lane];                                                \
+            const v8i_t av = v8i_t{(int)A.x, (int)A.y, (int)A.z, (int)A.w, 0, 0, 0, 0};       \
+            _Pragma("unroll") for (int cb = 0; cb < 2; cb++) acc[a][cb] =                     \
+                __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(av, Bc[cb], acc[a][cb], 4, 4, \
+                                                                0, kE8M0One, 0, kE8M0One);    \
+        }                                                                                     \
+        _Pragma("unroll") for (int k = 0; k < 2 * kMfTiles; k++) {                            \
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);                                \
+            __builtin_amdgcn_sched_group_barrier(0x002, 6, 0);                                \
+        }                                                                                     \
+        uint4 *da = s_a[((GG) + 1) & 1];                                                      \
+        da[tid] = a0_##PN;                                                                    \
+        da[tid + 256] = a1_##PN;                                                              \
+        __syncthreads();                                                                      \
+        Bc[0] = Bn[0];                                                                        \
+        Bc[1] = Bn[1];                                                                        \
+    } while (0)
+
+    // nq is a multiple of 4 (mfma_steps), so every iteration runs all four
+    // steps and the wait counts are the same on every trip.
+    static_assert(kMfAhead == 4, "the step sequence below is written out for 4 slots");
+    for (uint32_t g = 0; g < nq; g += kMfAhead) {
+        MF_STEP(g, 0, 1);
+        MF_STEP(g + 1, 1, 2);
+        MF_STEP(g + 2, 2, 3);
+        MF_STEP(g + 3, 3, 0);
     }
+#undef MF_STEP
 
     // Epilogue.  Accumulator v of a lane holds row 8 (v / 4) + 4 h + v % 4 of
     // the tile at column c: rows 0..15 are bits of recovery row 2t, rows
@@ -447,7 +487,7 @@ hipError_t launch_encode_mfma(const SliceDesc *d_inputs, uint32_t n_inputs, cons
                               hipStream_t stream) {
     if (R == 0) return hipSuccess;
     if (n_inputs > (uint32_t)kMfMaxInputs) return hipErrorInvalidValue;  // caller chunks
-    const uint32_t nq = (n_inputs + 3) / 4;
+    const uint32_t nq = mfma_steps(n_inputs);
     const uint32_t per_block = kMfThreads * 2;  // bytes of symbol columns per workgroup
     dim3 grid((S + per_block - 1) / per_block, mfma_row_tiles(R) / kMfTiles);
     hipLaunchKernelGGL(k_encode_mfma, grid, dim3(kMfThreads), 0, stream, d_inputs, n_inputs,
@@ -528,7 +568,7 @@ __device__ __forceinline__ void md5_load_block(uint32_t x[16], const W *p) {
     constexpr int n = 16 / (sizeof(W) / 4);
 #pragma unroll
     for (int k = 0; k < n; k++) {
-        const W v = p[k];
+        const W v = gload(p + k);
         memcpy(&x[k * (sizeof(W) / 4)], &v, sizeof(W));
     }
 }
