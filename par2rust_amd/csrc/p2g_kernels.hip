@@ -249,20 +249,38 @@ hipError_t launch_encode(const SliceDesc *d_inputs, uint32_t n_inputs, const uin
 //
 // One MFMA covers 32 rows (recovery rows 2t, 2t + 1), 64 k (inputs
 // 4g .. 4g + 3) and 32 columns (symbols).  A lane in half h (lane / 32)
-// holds k values of inputs 4g + 2h and 4g + 2h + 1, 16 nibbles each, the
-// same order in A and in B, so each (A lane, nibble) meets the B nibble of
-// the same k.
+// holds k values of inputs 4g + 2h and 4g + 2h + 1, 16 nibbles each (two
+// dwords per input), the same order in A and in B, so each (A lane, nibble)
+// meets the B nibble of the same k.
+//
+// Within an input, nibble n of dword d (d = 0, 1; n = 0..7) carries symbol
+// bit 4 (n % 4) + n / 4 + 2 d.  Any order shared by A and B gives the same
+// sums; this one makes the B build of a loaded word (two symbols) four
+// masked shifts and four byte permutes.
 
 typedef int v8i_t __attribute__((ext_vector_type(8)));
 typedef float v16f_t __attribute__((ext_vector_type(16)));
 
-// Eight bits to eight fp4 nibbles: bit k of x becomes nibble k = 0x2 (1.0).
-__device__ __forceinline__ uint32_t spread8(uint32_t x) {
-    uint32_t t = x & 0xFFu;
-    t = (t | (t << 12)) & 0x000F000Fu;
-    t = (t | (t << 6)) & 0x03030303u;
-    t = (t | (t << 3)) & 0x11111111u;
-    return t << 1;
+// B dwords of the two symbols of word w: fp4 1.0 (nibble 0x2) where the bit
+// is set.  m0..m3 hold bits 4q, 4q + 1, 4q + 2, 4q + 3 of w at bit 1 of
+// nibble q; the permutes gather the low (symbol 0) or high (symbol 1) halves.
+__device__ __forceinline__ void mf_b_words(uint32_t w, uint32_t out[4]) {
+    const uint32_t m0 = (w << 1) & 0x22222222u, m1 = w & 0x22222222u;
+    const uint32_t m2 = (w >> 1) & 0x22222222u, m3 = (w >> 2) & 0x22222222u;
+    out[0] = __builtin_amdgcn_perm(m1, m0, 0x05040100u);  // symbol 0, dword 0
+    out[1] = __builtin_amdgcn_perm(m3, m2, 0x05040100u);  // symbol 0, dword 1
+    out[2] = __builtin_amdgcn_perm(m1, m0, 0x07060302u);  // symbol 1, dword 0
+    out[3] = __builtin_amdgcn_perm(m3, m2, 0x07060302u);  // symbol 1, dword 1
+}
+
+// A dword d of matrix row m (bit b = coefficient of symbol bit b), in the
+// same nibble order.
+__device__ __forceinline__ uint32_t mf_a_word(uint32_t m, int d) {
+    uint32_t out = 0;
+#pragma unroll
+    for (int n = 0; n < 8; n++)
+        out |= ((m >> (4 * (n & 3) + (n >> 2) + 2 * d)) & 1u) << (4 * n + 1);
+    return out;
 }
 
 // E8M0 block scale of 1.0 (biased exponent 127) for both operands; a scale
@@ -295,8 +313,8 @@ __global__ __launch_bounds__(256) void k_mfma_afrag(const uint16_t *__restrict__
             m |= ((v >> j) & 1u) << b;
             v = ((v << 1) & 0xFFFFu) ^ ((v & 0x8000u) ? (kFieldGenerator & 0xFFFFu) : 0u);
         }
-        d[2 * e] = spread8(m);
-        d[2 * e + 1] = spread8(m >> 8);
+        d[2 * e] = mf_a_word(m, 0);
+        d[2 * e + 1] = mf_a_word(m, 1);
     }
     afrag[idx] = make_uint4(d[0], d[1], d[2], d[3]);
 }
@@ -321,42 +339,56 @@ constexpr int kMfAhead = 4;  // K steps between issuing a load and consuming it
 // the partial last word from the descriptor's tail field in global memory,
 // anything else reads kZeroWord.  Only the address is selected, so the
 // loaded word is final and nothing waits for it before it is consumed.
-__device__ __forceinline__ void mf_load_words(const SliceDesc *s_desc,
-                                              const SliceDesc *__restrict__ g_desc, uint32_t n,
-                                              uint32_t g, uint32_t h, uint32_t off, bool col_ok,
-                                              uint32_t &w0, uint32_t &w1) {
+// s_desc has 4 nq entries, those past the inputs with valid = 0, and
+// valid <= S, so inputs past n and columns past S read zero without tests
+// of their own (g_desc is dereferenced only for a partial word, i < n).
+// The descriptor is read as one 16-byte LDS load {ptr lo, ptr hi, valid,
+// tail} and all three addresses are formed unconditionally, so the select
+// stays two conditional moves per address instead of a branch.
+__device__ __forceinline__ void mf_load_words(const uint4 *s_desc, const SliceDesc *__restrict__ g_desc,
+                                              uint32_t g, uint32_t h, uint32_t off, uint32_t &w0,
+                                              uint32_t &w1) {
+    static_assert(sizeof(SliceDesc) == 16 && offsetof(SliceDesc, valid) == 8 &&
+                      offsetof(SliceDesc, tail) == 12,
+                  "descriptor layout read as uint4");
+    const uint64_t zero = reinterpret_cast<uint64_t>(&kZeroWord);
     uint32_t wd[2];
 #pragma unroll
     for (int e = 0; e < 2; e++) {
         const uint32_t i = 4 * g + 2 * h + e;
-        const uint32_t ic = min(i, n - 1);
-        const SliceDesc d = s_desc[ic];
-        const bool in = i < n && col_ok;
-        const bool full = in && off + 4 <= d.valid;
-        const bool part = in && !full && off < d.valid;
-        const uint32_t *src = full   ? reinterpret_cast<const uint32_t *>(d.ptr + off)
-                              : part ? &g_desc[ic].tail
-                                     : &kZeroWord;
-        wd[e] = gload(src);
+        uint4 d = s_desc[i];
+        // Opaque to the optimizer: keeps the 16-byte read whole and the
+        // address select a select (it would otherwise sink the pointer read
+        // into a branch on `valid`, serializing two LDS round trips).
+        asm("" : "+v"(d.x), "+v"(d.y), "+v"(d.z));
+        const uint64_t data = ((uint64_t)d.y << 32 | d.x) + off;
+        const uint64_t tail = reinterpret_cast<uint64_t>(&g_desc[i].tail);
+        // full word: data; partial word: tail; past the end: zero.  As masks
+        // (full implies partial) so that no branch is formed.
+        uint32_t mf = 0u - (uint32_t)(off + 4 <= d.z), mp = 0u - (uint32_t)(off < d.z);
+        asm("" : "+v"(mf), "+v"(mp));
+        const uint64_t f64 = (uint64_t)mf << 32 | mf, p64 = (uint64_t)mp << 32 | mp;
+        const uint64_t a = (data & f64) | (tail & p64 & ~f64) | (zero & ~p64);
+        wd[e] = gload(reinterpret_cast<const uint32_t *>(a));
     }
     w0 = wd[0];
     w1 = wd[1];
 }
 
+// B operands of the two column tiles (cb = symbol 0 / 1 of each word) from
+// the words of inputs 4g + 2h (w0) and 4g + 2h + 1 (w1).
 __device__ __forceinline__ void mf_make_b(uint32_t w0, uint32_t w1, v8i_t B[2]) {
-#pragma unroll
-    for (int cb = 0; cb < 2; cb++) {
-        const uint32_t s0 = (w0 >> (16 * cb)) & 0xFFFFu;
-        const uint32_t s1 = (w1 >> (16 * cb)) & 0xFFFFu;
-        B[cb] = v8i_t{(int)spread8(s0), (int)spread8(s0 >> 8), (int)spread8(s1),
-                      (int)spread8(s1 >> 8), 0, 0, 0, 0};
-    }
+    uint32_t p[4], q[4];
+    mf_b_words(w0, p);
+    mf_b_words(w1, q);
+    B[0] = v8i_t{(int)p[0], (int)p[1], (int)q[0], (int)q[1], 0, 0, 0, 0};
+    B[1] = v8i_t{(int)p[2], (int)p[3], (int)q[2], (int)q[3], 0, 0, 0, 0};
 }
 
 __global__ __launch_bounds__(kMfThreads, 1) void k_encode_mfma(
     const SliceDesc *__restrict__ inputs, uint32_t n, const uint4 *__restrict__ afrag,
     uint32_t nq, uint32_t R, uint32_t S, uint8_t *__restrict__ rec, int accumulate) {
-    __shared__ SliceDesc s_desc[kMfMaxInputs];
+    __shared__ uint4 s_desc[kMfMaxInputs];  // SliceDesc as {ptr lo, ptr hi, valid, tail}
     __shared__ uint4 s_a[2][kMfTiles * 64];
 
     const uint32_t tid = threadIdx.x;
@@ -367,7 +399,8 @@ __global__ __launch_bounds__(kMfThreads, 1) void k_encode_mfma(
     const bool col_ok = off < S;
     const uint32_t tile0 = blockIdx.y * kMfTiles;
 
-    for (uint32_t i = tid; i < n; i += kMfThreads) s_desc[i] = inputs[i];
+    for (uint32_t i = tid; i < 4 * nq; i += kMfThreads)
+        s_desc[i] = i < n ? gload(reinterpret_cast<const uint4 *>(inputs) + i) : make_uint4(0, 0, 0, 0);
 
     // Thread tid stages A entries tid and tid + 256 of each K step: tile
     // (e / 64), lane (e % 64) of fragment (tile0 + tile, g).
@@ -398,7 +431,7 @@ __global__ __launch_bounds__(kMfThreads, 1) void k_encode_mfma(
         const uint32_t gp = min((uint32_t)(P), nq - 1);                                       \
         a0_##P = ga0[(size_t)gp * 64];                                                        \
         a1_##P = ga1[(size_t)gp * 64];                                                        \
-        mf_load_words(s_desc, inputs, n, gp, h, off, col_ok, w0_##P, w1_##P);                         \
+        mf_load_words(s_desc, inputs, gp, h, off, w0_##P, w1_##P);                         \
     } while (0)
     MF_FILL(0);
     MF_FILL(1);
@@ -416,12 +449,14 @@ __global__ __launch_bounds__(kMfThreads, 1) void k_encode_mfma(
         const uint32_t gf = min((GG) + kMfAhead, nq - 1);                                     \
         a0_##P = ga0[(size_t)gf * 64];                                                        \
         a1_##P = ga1[(size_t)gf * 64];                                                        \
-        mf_load_words(s_desc, inputs, n, gf, h, off, col_ok, w0_##P, w1_##P);                         \
+        mf_load_words(s_desc, inputs, gf, h, off, w0_##P, w1_##P);                         \
         const uint4 *sa = s_a[(GG) & 1];                                                      \
         v8i_t Bn[2];                                                                          \
         mf_make_b(w0_##PN, w1_##PN, Bn);                                                      \
+        uint4 Af[kMfTiles];                                                                   \
+        _Pragma("unroll") for (int a = 0; a < kMfTiles; a++) Af[a] = sa[a * 64 + lane];       \
         _Pragma("unroll") for (int a = 0; a < kMfTiles; a++) {                                \
-            const uint4 A = sa[a * 64 + lane];                                                \
+            const uint4 A = Af[a];                                                            \
             const v8i_t av = v8i_t{(int)A.x, (int)A.y, (int)A.z, (int)A.w, 0, 0, 0, 0};       \
             _Pragma("unroll") for (int cb = 0; cb < 2; cb++) acc[a][cb] =                     \
                 __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(av, Bc[cb], acc[a][cb], 4, 4, \
@@ -486,8 +521,8 @@ hipError_t launch_encode_mfma(const SliceDesc *d_inputs, uint32_t n_inputs, cons
                               uint32_t R, uint32_t S, uint8_t *d_rec, int accumulate,
                               hipStream_t stream) {
     if (R == 0) return hipSuccess;
-    if (n_inputs > (uint32_t)kMfMaxInputs) return hipErrorInvalidValue;  // caller chunks
     const uint32_t nq = mfma_steps(n_inputs);
+    if (4 * nq > (uint32_t)kMfMaxInputs) return hipErrorInvalidValue;  // caller chunks
     const uint32_t per_block = kMfThreads * 2;  // bytes of symbol columns per workgroup
     dim3 grid((S + per_block - 1) / per_block, mfma_row_tiles(R) / kMfTiles);
     hipLaunchKernelGGL(k_encode_mfma, grid, dim3(kMfThreads), 0, stream, d_inputs, n_inputs,
