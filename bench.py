@@ -33,7 +33,23 @@ WORKLOADS = {
 }
 
 HBM_PEAK_GBPS = 8000.0
+FP4_DENSE_PEAK_TFLOPS = 10100.0  # MI355X dense MXFP4 matrix peak
+VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12  # 32-bit lane-ops/s: 256 CUs x 64 lanes x 2.4 GHz
 KERNEL_NAME = {"mfma": "k_encode_mfma (K1)", "bitslice": "k_encode_bits (K1)"}
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_encode_mfma_headline_traffic.md")
+
+
+def measured_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` from the committed counter summary
+    (FETCH_SIZE and WRITE_SIZE, each scaled by a plain copy's reading), or
+    None when the summary has no figure for it."""
+    try:
+        for line in open(TRAFFIC_FILE):
+            if line.startswith("traffic_bytes_per_launch") and kernel in line:
+                return int(line.split("=")[1].split()[0])
+    except OSError:
+        pass
+    return None
 
 
 def log(*a):
@@ -47,6 +63,9 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="headline", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run one 512 MiB device-to-device copy (the reference reading "
+                         "for the FETCH_SIZE / WRITE_SIZE counters) and exit")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -59,6 +78,15 @@ def main() -> int:
 
     import par2rust_amd as p2
     from par2rust_amd import hipmem, synth
+
+    if args.copy_calibration:
+        n = 512 << 20
+        a, b = hipmem.DeviceBuffer(n), hipmem.DeviceBuffer(n)
+        a.zero()
+        hipmem.copy_device(b, a, n)
+        hipmem.synchronize()
+        print(json.dumps({"copy_calibration_bytes": n}), flush=True)
+        return 0
 
     nbytes, S, R, seed = WORKLOADS[args.workload]
     N = (nbytes + S - 1) // S
@@ -114,7 +142,26 @@ def main() -> int:
     value = nbytes * args.steps / elapsed / 1e9
     gf_ops = N * R * (S // 2)
     alg_bytes = N * S + R * S  # source read once, recovery written once
-    achieved = alg_bytes / (k1_ms / 1e3) / 1e9
+    # The encode as a GF(2) matrix product (rows 16 R, k 16 N, columns S / 2):
+    # 2 * 16R * 16N * S/2 = 256 N R S operations, run as dense fp4 MFMA.
+    mfma_ops = 256 * N * R * S
+    kname = KERNEL_NAME[os.environ.get("P2G_ENCODE", "mfma")]
+    if kname.startswith("k_encode_mfma"):
+        roof = {"bound": "mfma", "kernel": kname,
+                "achieved": round(mfma_ops / (k1_ms / 1e3) / 1e12, 2),
+                "peak": FP4_DENSE_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(mfma_ops / (k1_ms / 1e3) / 1e12 / FP4_DENSE_PEAK_TFLOPS, 4),
+                "traffic": measured_traffic("k_encode_mfma"),
+                "alg_ops_per_launch": mfma_ops, "alg_bytes_per_launch": alg_bytes,
+                "hbm_frac": round(alg_bytes / (k1_ms / 1e3) / 1e9 / HBM_PEAK_GBPS, 5)}
+    else:
+        # bit-sliced form: 16 x 16 masked XORs (one v_bitop3 each) per 32 symbols
+        valu_ops = N * R * (S // 2) // 32 * 256
+        roof = {"bound": "valu", "kernel": kname,
+                "achieved": round(valu_ops / (k1_ms / 1e3) / 1e12, 2),
+                "peak": round(VALU_PEAK_TOPS, 1), "unit": "Tlane-op/s",
+                "frac": round(valu_ops / (k1_ms / 1e3) / 1e12 / VALU_PEAK_TOPS, 4),
+                "traffic": None, "alg_bytes_per_launch": alg_bytes}
 
     gpu_rec = rec.download()
     gpu_md5 = slice_md5.download()
@@ -142,10 +189,7 @@ def main() -> int:
         "gf_mul_xor_per_s": round(gf_ops * args.steps / elapsed, 1),
         "kernel_ms": {"k1_encode": round(k1_ms, 4), "k2_slice_md5_crc": round(k2_ms, 4),
                       "k3_packet_md5": round(k3_ms, 4)},
-        "roofline": {"bound": "hbm", "kernel": KERNEL_NAME[os.environ.get("P2G_ENCODE", "mfma")],
-                     "achieved": round(achieved, 2), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBPS, 5), "traffic": None,
-                     "alg_bytes_per_launch": alg_bytes},
+        "roofline": roof,
     }
 
     if not args.no_cpu_baseline:

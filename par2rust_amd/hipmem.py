@@ -28,7 +28,7 @@ _hip.hipGetErrorString.argtypes = [ctypes.c_int]
 _hip.hipGetDeviceCount.argtypes = [ctypes.POINTER(ctypes.c_int)]
 _hip.hipSetDevice.argtypes = [ctypes.c_int]
 
-H2D, D2H = 1, 2
+H2D, D2H, D2D = 1, 2, 3
 
 
 def _check(rc: int, what: str):
@@ -48,6 +48,13 @@ def set_device(d: int) -> None:
 
 def synchronize() -> None:
     _check(_hip.hipDeviceSynchronize(), "hipDeviceSynchronize")
+
+
+def copy_device(dst: "DeviceBuffer", src: "DeviceBuffer", nbytes: int) -> None:
+    """Device-to-device copy of nbytes (the runtime's copy kernel)."""
+    if nbytes > min(dst.nbytes, src.nbytes):
+        raise ValueError("copy past the end of a buffer")
+    _check(_hip.hipMemcpy(dst.ptr, src.ptr, nbytes, D2D), "hipMemcpy D2D")
 
 
 class DeviceBuffer:
