@@ -4,8 +4,11 @@
 
 A step is one pass of the device hot path over the workload resident in
 HBM: the GF(2^16) recovery encode (K1), the per-slice MD5 + CRC-32 of the IFSC
-packet (K2, on a second stream, overlapping K1) and the recovery-packet MD5s
-(K3, after K1).  `value` is source bytes per second over the K timed steps.
+packet (K2) and the recovery-packet MD5s (K3).  By default a step is one
+p2g_dev_create_pass call (all three in one persistent launch, the packet MD5
+following the encode band by band); --mode separate runs K1, K2 and K3 as
+their own launches (K2 on a second stream).  `value` is source bytes per
+second over the K timed steps.
 
 The headline workload is one 538,218,411-byte file of xorshift64 bytes
 (seed 1) at slice size 524288 (1027 slices, the last one zero padded) with
@@ -36,19 +39,22 @@ HBM_PEAK_GBPS = 8000.0
 FP4_DENSE_PEAK_TFLOPS = 10100.0  # MI355X dense MXFP4 matrix peak
 VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12  # 32-bit lane-ops/s: 256 CUs x 64 lanes x 2.4 GHz
 KERNEL_NAME = {"mfma": "k_encode_mfma (K1)", "bitslice": "k_encode_bits (K1)"}
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_encode_mfma_headline_traffic.md")
+PASS_KERNEL = "k_create_pass"
+TRAFFIC_FILES = [os.path.join(ROOT, "profiles", f"{k}_headline_traffic.md")
+                 for k in (PASS_KERNEL, "k_encode_mfma")]
 
 
 def measured_traffic(kernel: str):
     """HBM bytes per launch of `kernel` from the committed counter summary
     (FETCH_SIZE and WRITE_SIZE, each scaled by a plain copy's reading), or
     None when the summary has no figure for it."""
-    try:
-        for line in open(TRAFFIC_FILE):
-            if line.startswith("traffic_bytes_per_launch") and kernel in line:
-                return int(line.split("=")[1].split()[0])
-    except OSError:
-        pass
+    for path in TRAFFIC_FILES:
+        try:
+            for line in open(path):
+                if line.startswith("traffic_bytes_per_launch") and f" {kernel} " in line:
+                    return int(line.split("=")[1].split()[0])
+        except OSError:
+            pass
     return None
 
 
@@ -62,6 +68,8 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="headline", choices=sorted(WORKLOADS))
+    ap.add_argument("--mode", default="pass", choices=["pass", "separate"],
+                    help="pass: one fused create pass per step; separate: K1, K2, K3 launches")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run one 512 MiB device-to-device copy (the reference reading "
@@ -105,10 +113,18 @@ def main() -> int:
     set_id = bytes(range(16))
 
     s_enc, s_sum = hipmem.Stream(), hipmem.Stream()
-    ev = {k: [(hipmem.Event(), hipmem.Event()) for _ in range(args.steps)]
-          for k in ("k1", "k2", "k3")}
+    kinds = ("pass",) if args.mode == "pass" else ("k1", "k2", "k3")
+    ev = {k: [(hipmem.Event(), hipmem.Event()) for _ in range(args.steps)] for k in kinds}
 
-    def step(i=None):
+    def step_pass(i=None):
+        if i is not None:
+            ev["pass"][i][0].record(s_enc)
+        ctx.dev_create_pass([src], S, R, set_id, rec, slice_md5, slice_crc, pkt_md5,
+                            stream=s_enc.addr)
+        if i is not None:
+            ev["pass"][i][1].record(s_enc)
+
+    def step_separate(i=None):
         if i is not None:
             ev["k2"][i][0].record(s_sum)
         ctx.dev_slice_checksums([src], S, slice_md5, slice_crc, stream=s_sum.addr)
@@ -123,22 +139,24 @@ def main() -> int:
         if i is not None:
             ev["k3"][i][1].record(s_enc)
 
+    step = step_pass if args.mode == "pass" else step_separate
     for _ in range(args.warmup):
         step()
-    hipmem.synchronize()
+    ctx.synchronize()
     log(f"bench: {args.warmup} warmup steps done")
 
     t0 = time.perf_counter()
     for i in range(args.steps):
         step(i)
-    hipmem.synchronize()
+    ctx.synchronize()  # also reports a pass that could not complete
     t1 = time.perf_counter()
     elapsed = t1 - t0
     ms_step = 1000.0 * elapsed / args.steps
     log(f"bench: {args.steps} timed steps, {ms_step:.3f} ms/step")
 
     k_ms = {k: sum(a.elapsed_ms(b) for a, b in v) / args.steps for k, v in ev.items()}
-    k1_ms, k2_ms, k3_ms = k_ms["k1"], k_ms["k2"], k_ms["k3"]
+    # the launch the roofline is about: the fused pass, or K1 alone
+    k1_ms = k_ms["pass"] if args.mode == "pass" else k_ms["k1"]
     value = nbytes * args.steps / elapsed / 1e9
     gf_ops = N * R * (S // 2)
     alg_bytes = N * S + R * S  # source read once, recovery written once
@@ -146,12 +164,17 @@ def main() -> int:
     # 2 * 16R * 16N * S/2 = 256 N R S operations, run as dense fp4 MFMA.
     mfma_ops = 256 * N * R * S
     kname = KERNEL_NAME[os.environ.get("P2G_ENCODE", "mfma")]
-    if kname.startswith("k_encode_mfma"):
+    traffic_kernel = "k_encode_mfma"
+    if args.mode == "pass" and kname.startswith("k_encode_mfma"):
+        kname = (f"{PASS_KERNEL} (encode on MFMA + slice MD5/CRC-32 + packet MD5; timed with "
+                 "its descriptor, fragment and reset launches)")
+        traffic_kernel = PASS_KERNEL
+    if "mfma" in kname or PASS_KERNEL in kname:
         roof = {"bound": "mfma", "kernel": kname,
                 "achieved": round(mfma_ops / (k1_ms / 1e3) / 1e12, 2),
                 "peak": FP4_DENSE_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(mfma_ops / (k1_ms / 1e3) / 1e12 / FP4_DENSE_PEAK_TFLOPS, 4),
-                "traffic": measured_traffic("k_encode_mfma"),
+                "traffic": measured_traffic(traffic_kernel),
                 "alg_ops_per_launch": mfma_ops, "alg_bytes_per_launch": alg_bytes,
                 "hbm_frac": round(alg_bytes / (k1_ms / 1e3) / 1e9 / HBM_PEAK_GBPS, 5)}
     else:
@@ -183,12 +206,14 @@ def main() -> int:
         "data": f"synthetic: xorshift64 seed {seed}, {nbytes} bytes, resident in HBM",
         "config": {"workload": f"{args.workload}: 1 file x {nbytes} B, -s {S} "
                                f"({N} slices), -c {R}; K1 encode + K2 slice md5/crc32 + "
-                               "K3 packet md5",
+                               f"K3 packet md5 ({args.mode})",
                    "file_bytes": nbytes, "slice_size": S, "slices": N, "recovery": R,
                    "parallelism": "1 GPU"},
         "gf_mul_xor_per_s": round(gf_ops * args.steps / elapsed, 1),
-        "kernel_ms": {"k1_encode": round(k1_ms, 4), "k2_slice_md5_crc": round(k2_ms, 4),
-                      "k3_packet_md5": round(k3_ms, 4)},
+        "kernel_ms": ({"create_pass": round(k_ms["pass"], 4),
+                       "pass_roles_start_end_ms": ctx.pass_timing()} if args.mode == "pass" else
+                      {"k1_encode": round(k_ms["k1"], 4), "k2_slice_md5_crc": round(k_ms["k2"], 4),
+                       "k3_packet_md5": round(k_ms["k3"], 4)}),
         "roofline": roof,
     }
 

@@ -131,6 +131,32 @@ int p2g_dev_recovery_packet_md5(p2g_ctx *ctx, const uint8_t *rec, uint32_t count
                                 const uint8_t set_id[16], uint8_t *pkt_md5_out,
                                 void *stream);
 
+/* One device pass of the create hot path over a whole set resident in HBM:
+ * the recovery encode (as p2g_dev_encode with first_input_index 0,
+ * total_inputs = the slice count, accumulate 0), the per-slice MD5 and
+ * CRC-32 (as p2g_dev_slice_checksums) and the packet hashes of the R
+ * recovery packets (as p2g_dev_recovery_packet_md5 with first_exponent 0),
+ * with the packet hashes following the encode band by band inside one
+ * launch.  Replaces, for one set, the scan checksums (src/source.rs:165-220),
+ * encode_all_recovery_blocks (src/creator.rs:883-1018) and the build_packet
+ * hashes of build_volume_packet (src/creator.rs:1031-1041) together.
+ * Outputs: rec (R * slice_size), slice_md5_out (16 B per slice),
+ * slice_crc_out (one per slice), pkt_md5_out (16 B per packet), all device
+ * memory.  p2g_synchronize reports a pass that could not complete. */
+int p2g_dev_create_pass(p2g_ctx *ctx, const p2g_dev_file *files, uint32_t nfiles,
+                        uint64_t slice_size, uint32_t recovery_count, const uint8_t set_id[16],
+                        uint8_t *rec, uint8_t *slice_md5_out, uint32_t *slice_crc_out,
+                        uint8_t *pkt_md5_out, void *stream);
+
+/* Diagnostics, no reference counterpart: after the last p2g_dev_create_pass
+ * of this context has finished (it waits for that), the first start and the
+ * last end of its packet-MD5, slice-MD5, encode and CRC work, in ms from the
+ * earliest start: ms_out[2k] start and ms_out[2k + 1] end of role k, 0 for a
+ * role with no work (or when the pass ran as separate kernels). */
+int p2g_pass_timing(p2g_ctx *ctx, double *ms_out);
+
+/* Waits for all work queued by this context, on its own streams and on the
+ * streams callers passed in. */
 int p2g_synchronize(p2g_ctx *ctx);
 
 /* ---------------------------------------------------------------------

@@ -39,8 +39,8 @@ EXPORTED_SYMBOLS = [
     "p2g_open", "p2g_close", "p2g_last_error", "p2g_version",
     "p2g_slice_checksums", "p2g_encode_begin", "p2g_encode_push", "p2g_encode_finish",
     "p2g_encode_free", "p2g_recovery_packet_md5", "p2g_dev_encode",
-    "p2g_dev_slice_checksums", "p2g_dev_recovery_packet_md5", "p2g_synchronize",
-    "p2g_run_create",
+    "p2g_dev_slice_checksums", "p2g_dev_recovery_packet_md5", "p2g_dev_create_pass",
+    "p2g_pass_timing", "p2g_synchronize", "p2g_run_create",
 ]
 
 
@@ -93,6 +93,9 @@ def _load() -> ctypes.CDLL:
                                                    P]),
         "p2g_dev_recovery_packet_md5": (ctypes.c_int, [P, P, ctypes.c_uint32, ctypes.c_uint64,
                                                        ctypes.c_uint32, P, P, P]),
+        "p2g_dev_create_pass": (ctypes.c_int, [P, P, ctypes.c_uint32, ctypes.c_uint64,
+                                               ctypes.c_uint32, P, P, P, P, P, P]),
+        "p2g_pass_timing": (ctypes.c_int, [P, ctypes.POINTER(ctypes.c_double)]),
         "p2g_synchronize": (ctypes.c_int, [P]),
         "p2g_run_create": (ctypes.c_int, [P, ctypes.c_char_p, ctypes.c_uint64, ctypes.c_uint32,
                                           ctypes.POINTER(_Scheme), ctypes.c_uint32,
@@ -304,6 +307,27 @@ class Context:
                                                     count, slice_size, first_exponent, sid,
                                                     ctypes.c_void_p(_dev_ptr(md5_out)),
                                                     ctypes.c_void_p(stream) if stream else None))
+
+    def dev_create_pass(self, files, slice_size: int, recovery_count: int, set_id: bytes, rec,
+                        slice_md5_out, slice_crc_out, pkt_md5_out,
+                        stream: Optional[int] = None):
+        """Encode + slice MD5/CRC-32 + packet MD5 of a device-resident set in
+        one pass (p2g_dev_create_pass); results land in the device buffers."""
+        arr, items = _dev_files(files)
+        sid = ctypes.create_string_buffer(bytes(set_id), 16)
+        self._check(lib.p2g_dev_create_pass(self.h, arr, len(items), slice_size, recovery_count,
+                                            sid, ctypes.c_void_p(_dev_ptr(rec)),
+                                            ctypes.c_void_p(_dev_ptr(slice_md5_out)),
+                                            ctypes.c_void_p(_dev_ptr(slice_crc_out)),
+                                            ctypes.c_void_p(_dev_ptr(pkt_md5_out)),
+                                            ctypes.c_void_p(stream) if stream else None))
+
+    def pass_timing(self) -> dict:
+        """(start, end) in ms of each role of the last fused pass."""
+        out = (ctypes.c_double * 8)()
+        self._check(lib.p2g_pass_timing(self.h, out))
+        return {role: (round(out[2 * k], 4), round(out[2 * k + 1], 4))
+                for k, role in enumerate(("packet_md5", "slice_md5", "encode", "crc"))}
 
     def synchronize(self):
         self._check(lib.p2g_synchronize(self.h))

@@ -103,9 +103,16 @@ int Ctx::open(int dev) {
     if (e != hipSuccess) return fail(P2G_ERR_DEVICE, "hipStreamCreate", e);
     e = hipEventCreateWithFlags(&enc_descs.free, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&sum_descs.free, hipEventDisableTiming);
+    for (hipEvent_t &ev : scratch_done)
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
     if (e != hipSuccess) return fail(P2G_ERR_DEVICE, "hipEventCreate", e);
     build_gf_tables(h_log, h_antilog);
     base_logs = input_base_logs();
+    int cus = 0;
+    e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e != hipSuccess || cus <= 0) return fail(P2G_ERR_DEVICE, "compute unit count", e);
+    num_cus = (uint32_t)cus;
+    if (const char *v = getenv("P2G_BAND_COLS")) band_cols = (uint32_t)strtoul(v, nullptr, 10);
     if (const char *v = getenv("P2G_ENCODE")) {
         if (!strcmp(v, "bitslice")) encode_variant = kEncodeBitslice;
         else if (!strcmp(v, "mfma")) encode_variant = kEncodeMfma;
@@ -130,9 +137,13 @@ Ctx::~Ctx() {
     d_md5.release();
     d_crc.release();
     d_stage.release();
+    d_sync.release();
+    for (hipEvent_t ev : scratch_done)
+        if (ev) hipEventDestroy(ev);
     for (DescSlot *sl : {&enc_descs, &sum_descs}) {
         sl->h.release();
         sl->d.release();
+        sl->tb.release();
         if (sl->free) hipEventDestroy(sl->free);
     }
     if (stream) hipStreamDestroy(stream);
@@ -152,6 +163,16 @@ int Ctx::side_stream(hipStream_t *out) {
     }
     *out = side;
     return P2G_OK;
+}
+
+int Ctx::scratch_acquire(Scratch which, hipStream_t st) {
+    hipError_t e = hipStreamWaitEvent(st, scratch_done[which], 0);
+    return e == hipSuccess ? P2G_OK : fail(P2G_ERR_DEVICE, "stream wait", e);
+}
+
+int Ctx::scratch_release(Scratch which, hipStream_t st) {
+    hipError_t e = hipEventRecord(scratch_done[which], st);
+    return e == hipSuccess ? P2G_OK : fail(P2G_ERR_DEVICE, "event record", e);
 }
 
 int check_slice_size(Ctx *c, uint64_t S) {
@@ -232,7 +253,7 @@ int dev_encode(Ctx *c, const SliceDesc *d_descs, uint32_t n, uint64_t S, uint32_
         }
         return P2G_OK;
     }
-    e = launch_desc_tails(const_cast<SliceDesc *>(d_descs), n, st);
+    e = launch_desc_tails(const_cast<SliceDesc *>(d_descs), n, nullptr, st);
     if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "descriptor tail kernel", e);
     // Matrix-core form: A fragments (1 KiB per 32 rows x 4 inputs) for a chunk
     // of inputs, a multiple of 4, within kMatsBudget.
@@ -290,7 +311,19 @@ int p2g_synchronize(p2g_ctx *ctx) {
     hipSetDevice(c->device);
     hipError_t e = hipStreamSynchronize(c->stream);
     if (e == hipSuccess && c->side) e = hipStreamSynchronize(c->side);
+    for (hipEvent_t ev : c->scratch_done)  // work queued on callers' streams
+        if (e == hipSuccess) e = hipEventSynchronize(ev);
     if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "synchronize", e);
+    if (c->pass_pending) {
+        c->pass_pending = false;
+        uint32_t flags = 0;
+        e = hipMemcpy(&flags, static_cast<uint32_t *>(c->d_sync.ptr) + kPassFlags, 4,
+                      hipMemcpyDeviceToHost);
+        if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "read create-pass flags", e);
+        if (flags & kPassErrWait)
+            return c->fail(P2G_ERR_DEVICE,
+                           "create pass: a packet hash stopped waiting for the encode (no progress for 2 s)");
+    }
     return P2G_OK;
 }
 
@@ -312,11 +345,14 @@ int p2g_dev_encode(p2g_ctx *ctx, const p2g_dev_file *files, uint32_t nfiles, uin
     if (descs.size() > kMaxInputs)
         return c->fail(P2G_ERR_TOO_MANY_INPUT_BLOCKS, "more than 32768 input slices");
     hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c->stream;
-    rc = upload_descs(c, c->enc_descs, descs, st);
+    rc = c->scratch_acquire(Ctx::kScratchEnc, st);
+    if (rc == P2G_OK) rc = upload_descs(c, c->enc_descs, descs, st);
+    if (rc == P2G_OK)
+        rc = dev_encode(c, static_cast<const SliceDesc *>(c->enc_descs.d.ptr), (uint32_t)descs.size(),
+                        slice_size, first_input_index, total_inputs, recovery_count, rec,
+                        accumulate, st);
     if (rc != P2G_OK) return rc;
-    return dev_encode(c, static_cast<const SliceDesc *>(c->enc_descs.d.ptr), (uint32_t)descs.size(),
-                      slice_size, first_input_index, total_inputs, recovery_count, rec,
-                      accumulate, st);
+    return c->scratch_release(Ctx::kScratchEnc, st);
 }
 
 int p2g_dev_slice_checksums(p2g_ctx *ctx, const p2g_dev_file *files, uint32_t nfiles,
@@ -336,15 +372,20 @@ int p2g_dev_slice_checksums(p2g_ctx *ctx, const p2g_dev_file *files, uint32_t nf
     if (rc != P2G_OK) return rc;
     if (descs.empty()) return P2G_OK;
     hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c->stream;
-    rc = upload_descs(c, c->sum_descs, descs, st);
+    rc = c->scratch_acquire(Ctx::kScratchSum, st);
+    if (rc == P2G_OK) rc = upload_descs(c, c->sum_descs, descs, st);
     if (rc != P2G_OK) return rc;
-    const SliceDesc *dd = static_cast<const SliceDesc *>(c->sum_descs.d.ptr);
+    SliceDesc *dd = static_cast<SliceDesc *>(c->sum_descs.d.ptr);
     const uint32_t n = (uint32_t)descs.size();
-    hipError_t e = launch_slice_md5(dd, n, (uint32_t)slice_size, md5_out, st);
+    if (c->sum_descs.tb.reserve((size_t)n * 64) != P2G_OK)
+        return c->fail(P2G_ERR_OUT_OF_MEMORY, "slice tail blocks");
+    uint4 *tb = static_cast<uint4 *>(c->sum_descs.tb.ptr);
+    hipError_t e = launch_desc_tails(dd, n, tb, st);
+    if (e == hipSuccess) e = launch_slice_md5(dd, tb, n, (uint32_t)slice_size, md5_out, st);
     if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "slice md5 kernel", e);
     e = launch_slice_crc32(dd, n, (uint32_t)slice_size, crc_out, st);
     if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "slice crc kernel", e);
-    return P2G_OK;
+    return c->scratch_release(Ctx::kScratchSum, st);
 }
 
 int p2g_dev_recovery_packet_md5(p2g_ctx *ctx, const uint8_t *rec, uint32_t count,
@@ -366,6 +407,96 @@ int p2g_dev_recovery_packet_md5(p2g_ctx *ctx, const uint8_t *rec, uint32_t count
                                        pkt_md5_out, st);
     if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "recovery md5 kernel", e);
     return P2G_OK;
+}
+
+int p2g_pass_timing(p2g_ctx *ctx, double *ms_out) {
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (!c || !ms_out) return P2G_ERR_INVALID_ARGUMENT;
+    if (!c->d_sync.ptr) return c->fail(P2G_ERR_INVALID_ARGUMENT, "no fused create pass has run");
+    hipSetDevice(c->device);
+    hipError_t e = hipEventSynchronize(c->scratch_done[Ctx::kScratchEnc]);
+    unsigned long long t[2 * kPassRoles];
+    if (e == hipSuccess)
+        e = hipMemcpy(t, static_cast<uint32_t *>(c->d_sync.ptr) + kPassTimes, sizeof t,
+                      hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "read create-pass stamps", e);
+    unsigned long long t0 = ~0ull;
+    for (uint32_t r = 0; r < kPassRoles; r++)
+        if (t[2 * r + 1]) t0 = std::min(t0, t[2 * r]);
+    for (uint32_t k = 0; k < 2 * kPassRoles; k++)  // 100 MHz counter: 1e5 ticks per ms
+        ms_out[k] = t[k | 1] && t[k] >= t0 ? (double)(t[k] - t0) / 1e5 : 0.0;
+    return P2G_OK;
+}
+
+int p2g_dev_create_pass(p2g_ctx *ctx, const p2g_dev_file *files, uint32_t nfiles,
+                        uint64_t slice_size, uint32_t recovery_count, const uint8_t set_id[16],
+                        uint8_t *rec, uint8_t *slice_md5_out, uint32_t *slice_crc_out,
+                        uint8_t *pkt_md5_out, void *stream) {
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (!c) return P2G_ERR_INVALID_ARGUMENT;
+    const uint32_t R = recovery_count;
+    if ((nfiles && !files) || !set_id || !slice_md5_out || !slice_crc_out ||
+        (R && (!rec || !pkt_md5_out)))
+        return c->fail(P2G_ERR_INVALID_ARGUMENT, "null argument");
+    for (const void *p : {(const void *)rec, (const void *)slice_md5_out,
+                          (const void *)slice_crc_out, (const void *)pkt_md5_out})
+        if (reinterpret_cast<uintptr_t>(p) % 4 != 0)
+            return c->fail(P2G_ERR_INVALID_ARGUMENT, "outputs must be 4-byte aligned");
+    if (R > P2G_MAX_RECOVERY_BLOCKS)
+        return c->fail(P2G_ERR_TOO_MANY_RECOVERY_BLOCKS, "recovery count above 65535");
+    int rc = check_slice_size(c, slice_size);
+    if (rc != P2G_OK) return rc;
+    hipSetDevice(c->device);
+    std::vector<SliceDesc> descs;
+    rc = append_descs(c, files, nfiles, slice_size, descs);
+    if (rc != P2G_OK) return rc;
+    if (descs.size() > kMaxInputs)
+        return c->fail(P2G_ERR_TOO_MANY_INPUT_BLOCKS, "more than 32768 input slices");
+    const uint32_t n = (uint32_t)descs.size();
+    const uint32_t S = (uint32_t)slice_size;
+    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c->stream;
+    rc = c->scratch_acquire(Ctx::kScratchEnc, st);
+    if (rc == P2G_OK) rc = upload_descs(c, c->enc_descs, descs, st);
+    if (rc != P2G_OK) return rc;
+    SliceDesc *dd = static_cast<SliceDesc *>(c->enc_descs.d.ptr);
+    if (c->enc_descs.tb.reserve((size_t)std::max<uint32_t>(n, 1) * 64) != P2G_OK)
+        return c->fail(P2G_ERR_OUT_OF_MEMORY, "slice tail blocks");
+    uint4 *tb = static_cast<uint4 *>(c->enc_descs.tb.ptr);
+    hipError_t e;
+
+    PassPlan plan;
+    const bool fused = c->encode_variant == kEncodeMfma &&
+                       plan_create_pass(n, R, S, c->num_cus, c->band_cols, &plan) &&
+                       mfma_afrag_bytes(n, R) <= kPassAfragBudget;
+    if (!fused) {
+        // The same results from the separate kernels, in stream order.
+        rc = dev_encode(c, dd, n, slice_size, 0, n, R, rec, 0, st);
+        if (rc != P2G_OK) return rc;
+        if (n) {
+            e = launch_desc_tails(dd, n, tb, st);
+            if (e == hipSuccess) e = launch_slice_md5(dd, tb, n, S, slice_md5_out, st);
+            if (e == hipSuccess) e = launch_slice_crc32(dd, n, S, slice_crc_out, st);
+            if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "slice checksum kernels", e);
+        }
+        e = launch_recovery_md5(rec, R, S, 0, set_id, pkt_md5_out, st);
+        if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "recovery md5 kernel", e);
+        return c->scratch_release(Ctx::kScratchEnc, st);
+    }
+    if (c->d_mats.reserve(mfma_afrag_bytes(n, R)) != P2G_OK)
+        return c->fail(P2G_ERR_OUT_OF_MEMORY, "matrix-core coefficient fragments");
+    if (c->d_sync.reserve((size_t)pass_sync_words(plan) * 4) != P2G_OK)
+        return c->fail(P2G_ERR_OUT_OF_MEMORY, "create-pass sync words");
+    uint4 *afrag = static_cast<uint4 *>(c->d_mats.ptr);
+    e = launch_desc_tails(dd, n, tb, st);
+    if (e == hipSuccess)
+        e = launch_mfma_afrag(static_cast<const uint16_t *>(c->d_antilog.ptr),
+                              static_cast<const uint16_t *>(c->d_base_logs.ptr), n, R, afrag, st);
+    if (e == hipSuccess)
+        e = launch_create_pass(plan, dd, tb, n, afrag, R, S, rec, slice_md5_out, slice_crc_out, set_id,
+                               pkt_md5_out, static_cast<uint32_t *>(c->d_sync.ptr), st);
+    if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "create pass kernels", e);
+    c->pass_pending = true;
+    return c->scratch_release(Ctx::kScratchEnc, st);
 }
 
 // ---- host-buffer entry points ----------------------------------------
@@ -507,8 +638,11 @@ int p2g_encode_push(p2g_job *job, uint32_t first_input_index, const uint8_t *dat
     if (e == hipSuccess) e = hipEventRecord(sg.copied, side);
     if (e == hipSuccess) e = hipStreamWaitEvent(c->stream, sg.copied, 0);
     if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "upload slice descriptors", e);
-    rc = dev_encode(c, dd, (uint32_t)n, j->S, first_input_index, j->N, j->R,
-                    static_cast<uint8_t *>(j->d_rec.ptr), 1, c->stream);
+    rc = c->scratch_acquire(Ctx::kScratchEnc, c->stream);
+    if (rc == P2G_OK)
+        rc = dev_encode(c, dd, (uint32_t)n, j->S, first_input_index, j->N, j->R,
+                        static_cast<uint8_t *>(j->d_rec.ptr), 1, c->stream);
+    if (rc == P2G_OK) rc = c->scratch_release(Ctx::kScratchEnc, c->stream);
     if (rc != P2G_OK) return rc;
     e = hipEventRecord(sg.done, c->stream);
     if (e == hipSuccess) e = hipEventSynchronize(sg.copied);

@@ -14,6 +14,7 @@ namespace p2g {
 constexpr uint32_t kMaxInputs = 32768;            // MAX_INPUT_BLOCKS, src/creator.rs:106
 constexpr uint64_t kMaxSliceBytes = 1ull << 31;   // kernels index slices with u32
 constexpr uint64_t kMatsBudget = 256ull << 20;    // coefficient matrices per launch chunk
+constexpr uint64_t kPassAfragBudget = 4ull << 30; // A fragments of one fused create pass
 
 // Grow-only device allocation.
 struct DevBuf {
@@ -36,6 +37,7 @@ struct PinBuf {
 struct DescSlot {
     PinBuf h;
     DevBuf d;
+    DevBuf tb;  // 64-byte tail block per descriptor (slice MD5)
     hipEvent_t free = nullptr;  // `h` may be rewritten once this fires
 };
 
@@ -46,6 +48,8 @@ enum { kEncodeMfma = 0, kEncodeBitslice = 1 };
 struct Ctx {
     int device = 0;
     int encode_variant = kEncodeMfma;
+    uint32_t num_cus = 0;          // compute units of the device
+    uint32_t band_cols = 0;        // fused-pass band width in 512-byte column groups (0: default)
     hipStream_t stream = nullptr;  // kernels and copies of this context
     hipStream_t side = nullptr;    // copies overlapped with `stream`; see side_stream()
     std::string err;
@@ -55,11 +59,21 @@ struct Ctx {
 
     DevBuf d_antilog, d_base_logs;
     DevBuf d_mats, d_md5, d_crc, d_stage;
+    DevBuf d_sync;                 // fused-pass queue, error flags and band counts
+    bool pass_pending = false;     // a fused pass may have set error flags not yet read
+    // Shared device buffers and the event recorded after the last kernel that
+    // reads them; an entry point that rewrites them first makes its stream
+    // wait for that event, so calls on different streams never overlap on
+    // them.  Encode side: enc_descs, d_mats, d_sync.  Checksum side: sum_descs.
+    enum Scratch { kScratchEnc = 0, kScratchSum = 1 };
+    hipEvent_t scratch_done[2] = {nullptr, nullptr};
     DescSlot enc_descs;  // encode launches
     DescSlot sum_descs;  // slice checksum launches
 
     int open(int dev);
     int side_stream(hipStream_t *out);
+    int scratch_acquire(Scratch which, hipStream_t st);  // st waits for earlier users
+    int scratch_release(Scratch which, hipStream_t st);  // marks the end of this use on st
     ~Ctx();
     int fail(int code, const char *what, hipError_t e = hipSuccess);
 };

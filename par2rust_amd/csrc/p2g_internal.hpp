@@ -41,7 +41,10 @@ hipError_t launch_encode(const SliceDesc *d_inputs, uint32_t n_inputs, const uin
                          hipStream_t stream);
 
 // Fills SliceDesc::tail of n descriptors in place.
-hipError_t launch_desc_tails(SliceDesc *d_descs, uint32_t n, hipStream_t stream);
+// With d_tail_blocks (64 bytes per descriptor, may be null), also writes each
+// slice's zero-padded 64-byte block holding byte `valid` (for slice MD5).
+hipError_t launch_desc_tails(SliceDesc *d_descs, uint32_t n, uint4 *d_tail_blocks,
+                             hipStream_t stream);
 
 // Matrix-core encode.  A fragments for `n` inputs (first input's L at
 // d_logbase[0]) and R recovery rows: mfma_row_tiles(R) * mfma_steps(n) * 1 KiB.
@@ -59,9 +62,37 @@ hipError_t launch_encode_mfma(const SliceDesc *d_inputs, uint32_t n_inputs, cons
                               uint32_t R, uint32_t S, uint8_t *d_rec, int accumulate,
                               hipStream_t stream);
 
+// Fused create pass (encode + slice MD5 + packet MD5, one launch).  The
+// sync words: [kPassFlags] error flags; from kPassTimes, 2 kPassRoles u64
+// real-time counter stamps (100 MHz): the first start and the last end of
+// each role's work (packet MD5, slice MD5, encode, CRC); [kPassBands + b]
+// finished units of column band b.
+constexpr uint32_t kPassFlags = 0, kPassTimes = 2, kPassRoles = 4;
+constexpr uint32_t kPassBands = kPassTimes + 4 * kPassRoles;
+enum { kPassPacket = 0, kPassSlice = 1, kPassEncode = 2, kPassCrc = 3 };
+constexpr uint32_t kPassErrWait = 1u;  // a packet lane stopped waiting for a band
+struct PassPlan {
+    uint32_t grid = 0, h2_blocks = 0, h3_blocks = 0;
+    uint32_t row_blocks = 0, col_groups = 0, band_cols = 0, n_bands = 0, n_units = 0;
+};
+// False when the fused pass does not apply (no inputs or recovery blocks,
+// more inputs than one encode launch takes, too few CUs).  band_cols = 0
+// picks the default band width.
+bool plan_create_pass(uint32_t n, uint32_t R, uint32_t S, uint32_t num_cus, uint32_t band_cols,
+                      PassPlan *p);
+uint32_t pass_sync_words(const PassPlan &p);
+// Zeroes the sync words, then runs the pass.  d_inputs must have their tail
+// words (launch_desc_tails) and d_afrag the fragments of all n inputs.
+hipError_t launch_create_pass(const PassPlan &p, const SliceDesc *d_inputs,
+                              const uint4 *d_tail_blocks, uint32_t n, const uint4 *d_afrag,
+                              uint32_t R, uint32_t S, uint8_t *d_rec,
+                              uint8_t *d_slice_md5, uint32_t *d_slice_crc, const uint8_t set_id[16],
+                              uint8_t *d_pkt_md5, uint32_t *d_sync, hipStream_t stream);
+
 // MD5 of each zero-padded S-byte slice (16 bytes per slice into d_md5).
-hipError_t launch_slice_md5(const SliceDesc *d_slices, uint32_t n, uint32_t S,
-                            uint8_t *d_md5, hipStream_t stream);
+// d_tail_blocks: the 64-byte tail blocks of launch_desc_tails.
+hipError_t launch_slice_md5(const SliceDesc *d_slices, const uint4 *d_tail_blocks, uint32_t n,
+                            uint32_t S, uint8_t *d_md5, hipStream_t stream);
 
 // CRC-32 of each zero-padded S-byte slice.
 hipError_t launch_slice_crc32(const SliceDesc *d_slices, uint32_t n, uint32_t S,

@@ -9,6 +9,10 @@
 //   k_slice_crc32    IFSC slice CRC-32, one wave per slice
 //   k_recovery_md5   recovery packet hash, one lane per packet
 //                    (src/format.rs:40-44 via src/packet/recovery.rs:11-19)
+#include <algorithm>
+#include <cstddef>
+#include <type_traits>
+
 #include "p2g_internal.hpp"
 
 namespace p2g {
@@ -87,17 +91,31 @@ __device__ __forceinline__ uint32_t load_word(const uint8_t *p, uint32_t off, ui
 }
 
 // Tail word of each descriptor: the bytes of a trailing partial dword, so
-// that word loads never touch memory past `valid`.
-__global__ __launch_bounds__(256) void k_desc_tails(SliceDesc *__restrict__ d, uint32_t n) {
+// that word loads never touch memory past `valid`.  With tail_blocks, also
+// the 64-byte block of the slice that holds byte `valid` (block valid / 64),
+// zero padded: MD5 lanes read it in place of the slice, so every lane of a
+// wave runs the same number of whole-block steps.
+__global__ __launch_bounds__(256) void k_desc_tails(SliceDesc *__restrict__ d, uint32_t n,
+                                                    uint4 *__restrict__ tail_blocks) {
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
     const uint32_t v = d[i].valid;
     d[i].tail = (v & 3) ? load_word(d[i].ptr, v & ~3u, v) : 0u;
+    if (tail_blocks) {
+        const uint32_t base = v / 64 * 64;
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+            tail_blocks[4 * (size_t)i + q] = make_uint4(
+                load_word(d[i].ptr, base + 16 * q, v), load_word(d[i].ptr, base + 16 * q + 4, v),
+                load_word(d[i].ptr, base + 16 * q + 8, v), load_word(d[i].ptr, base + 16 * q + 12, v));
+    }
 }
 
-hipError_t launch_desc_tails(SliceDesc *d_descs, uint32_t n, hipStream_t stream) {
+hipError_t launch_desc_tails(SliceDesc *d_descs, uint32_t n, uint4 *d_tail_blocks,
+                             hipStream_t stream) {
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_desc_tails, dim3((n + 255) / 256), dim3(256), 0, stream, d_descs, n);
+    hipLaunchKernelGGL(k_desc_tails, dim3((n + 255) / 256), dim3(256), 0, stream, d_descs, n,
+                       d_tail_blocks);
     return hipGetLastError();
 }
 
@@ -385,22 +403,36 @@ __device__ __forceinline__ void mf_make_b(uint32_t w0, uint32_t w1, v8i_t B[2]) 
     B[1] = v8i_t{(int)p[2], (int)p[3], (int)q[2], (int)q[3], 0, 0, 0, 0};
 }
 
-__global__ __launch_bounds__(kMfThreads, 1) void k_encode_mfma(
-    const SliceDesc *__restrict__ inputs, uint32_t n, const uint4 *__restrict__ afrag,
-    uint32_t nq, uint32_t R, uint32_t S, uint8_t *__restrict__ rec, int accumulate) {
-    __shared__ uint4 s_desc[kMfMaxInputs];  // SliceDesc as {ptr lo, ptr hi, valid, tail}
-    __shared__ uint4 s_a[2][kMfTiles * 64];
+// LDS of the encode: the padded descriptor table and the A double buffer.
+struct MfLds {
+    uint4 desc[kMfMaxInputs];     // SliceDesc as {ptr lo, ptr hi, valid, tail}
+    uint4 a[2][kMfTiles * 64];
+};
 
+// Fills lds.desc with the n descriptors and 4 nq - n empty ones (valid = 0).
+// The caller synchronizes before the first unit.
+__device__ __forceinline__ void mf_load_desc(MfLds &lds, const SliceDesc *__restrict__ inputs,
+                                             uint32_t n, uint32_t nq) {
+    for (uint32_t i = threadIdx.x; i < 4 * nq; i += kMfThreads)
+        lds.desc[i] = i < n ? gload(reinterpret_cast<const uint4 *>(inputs) + i) : make_uint4(0, 0, 0, 0);
+}
+
+// One encode unit: row tiles tile0 .. tile0 + 7 (recovery rows 2 tile0 ..
+// 2 tile0 + 15) over the 256 symbols at byte colbase of every slice, all nq K
+// steps, stored to rec (XORed in when accumulate).  All 256 threads call it;
+// it leaves lds.a free for the next unit.
+__device__ __forceinline__ void mf_unit(MfLds &lds, const SliceDesc *__restrict__ inputs,
+                                        const uint4 *__restrict__ afrag, uint32_t nq, uint32_t R,
+                                        uint32_t S, uint8_t *__restrict__ rec, int accumulate,
+                                        uint32_t tile0, uint32_t colbase) {
+    const uint4 *s_desc = lds.desc;
+    uint4 (*s_a)[kMfTiles * 64] = lds.a;
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = tid & 63, w = tid >> 6;
     const uint32_t c = lane & 31, h = lane >> 5;
     // this lane's dword: symbols base + 2c (column tile 0) and base + 2c + 1 (tile 1)
-    const uint32_t off = (blockIdx.x * (kMfThreads) + w * 64) * 2 + 4 * c;
+    const uint32_t off = colbase + w * 128 + 4 * c;
     const bool col_ok = off < S;
-    const uint32_t tile0 = blockIdx.y * kMfTiles;
-
-    for (uint32_t i = tid; i < 4 * nq; i += kMfThreads)
-        s_desc[i] = i < n ? gload(reinterpret_cast<const uint4 *>(inputs) + i) : make_uint4(0, 0, 0, 0);
 
     // Thread tid stages A entries tid and tid + 256 of each K step: tile
     // (e / 64), lane (e % 64) of fragment (tile0 + tile, g).
@@ -414,8 +446,6 @@ __global__ __launch_bounds__(kMfThreads, 1) void k_encode_mfma(
         for (int cb = 0; cb < 2; cb++)
 #pragma unroll
             for (int v = 0; v < 16; v++) acc[a][cb][v] = 0.0f;
-
-    __syncthreads();  // s_desc ready
 
     // Software pipeline, kMfAhead steps deep.  Ring slot p holds the staged
     // A entries and source words of a step gg with gg % kMfAhead == p.  Step
@@ -517,6 +547,16 @@ __global__ __launch_bounds__(kMfThreads, 1) void k_encode_mfma(
     }
 }
 
+__global__ __launch_bounds__(kMfThreads, 1) void k_encode_mfma(
+    const SliceDesc *__restrict__ inputs, uint32_t n, const uint4 *__restrict__ afrag,
+    uint32_t nq, uint32_t R, uint32_t S, uint8_t *__restrict__ rec, int accumulate) {
+    __shared__ MfLds lds;
+    mf_load_desc(lds, inputs, n, nq);
+    __syncthreads();
+    mf_unit(lds, inputs, afrag, nq, R, S, rec, accumulate, blockIdx.y * kMfTiles,
+            blockIdx.x * (kMfThreads * 2));
+}
+
 hipError_t launch_encode_mfma(const SliceDesc *d_inputs, uint32_t n_inputs, const uint4 *d_afrag,
                               uint32_t R, uint32_t S, uint8_t *d_rec, int accumulate,
                               hipStream_t stream) {
@@ -608,14 +648,13 @@ __device__ __forceinline__ void md5_load_block(uint32_t x[16], const W *p) {
     }
 }
 
-template <typename W>
-__device__ __forceinline__ void md5_bulk(uint32_t h[4], const uint8_t *p, uint32_t nblk) {
-    constexpr int kW = 64 / sizeof(W);  // W elements per block
-    const W *q = reinterpret_cast<const W *>(p);
+// nblk blocks, block b loaded from addr(b) (a const W *).
+template <typename W, typename Addr>
+__device__ __forceinline__ void md5_blocks_at(uint32_t h[4], Addr addr, uint32_t nblk) {
     uint32_t buf[kMd5Ahead][16];
 #pragma unroll
     for (int k = 0; k < kMd5Ahead; k++)
-        if ((uint32_t)k < nblk) md5_load_block(buf[k], q + (size_t)k * kW);
+        if ((uint32_t)k < nblk) md5_load_block(buf[k], addr((uint32_t)k));
     uint32_t b = 0;
     for (; b + kMd5Ahead <= nblk; b += kMd5Ahead) {
 #pragma unroll
@@ -624,7 +663,7 @@ __device__ __forceinline__ void md5_bulk(uint32_t h[4], const uint8_t *p, uint32
 #pragma unroll
             for (int w = 0; w < 16; w++) x[w] = buf[k][w];
             const uint32_t nxt = b + k + kMd5Ahead;
-            if (nxt < nblk) md5_load_block(buf[k], q + (size_t)nxt * kW);
+            if (nxt < nblk) md5_load_block(buf[k], addr(nxt));
             md5_compress(h, x);
         }
     }
@@ -632,6 +671,16 @@ __device__ __forceinline__ void md5_bulk(uint32_t h[4], const uint8_t *p, uint32
     for (int k = 0; k < kMd5Ahead; k++)
         if (b + k < nblk) md5_compress(h, buf[k]);
 }
+
+// nblk consecutive blocks at p.
+template <typename W>
+__device__ __forceinline__ void md5_bulk(uint32_t h[4], const uint8_t *p, uint32_t nblk) {
+    md5_blocks_at<W>(h, [p](uint32_t b) { return reinterpret_cast<const W *>(p + 64 * (size_t)b); },
+                     nblk);
+}
+
+// 64 zero bytes: the message block of a slice wholly past its data.
+__device__ const uint4 kZeroBlock[4] = {};
 
 // One MD5 wave per CU: each lane streams its own message, so the waves'
 // scattered 64-byte reads would otherwise share (and thrash) a CU's L1 and
@@ -644,19 +693,28 @@ constexpr uint32_t kMd5LdsReserve = 96 * 1024;
 // K2a: slice MD5, one lane per slice
 // ======================================================================
 
+// MD5 of one zero-padded S-byte slice into out (one lane).  A16: the slice
+// start is 16-byte aligned.  tb: the slice's tail block (k_desc_tails).  The
+// S / 64 whole message blocks run as one loop of the same length on every
+// lane, whatever `valid`: block b comes from the slice while it is whole
+// data, from tb for the block holding byte `valid`, and from kZeroBlock after
+// that; only the last (padding) blocks take the word-by-word path.
 template <bool A16>
-__global__ __launch_bounds__(64) void k_slice_md5(const SliceDesc *__restrict__ slices, uint32_t n,
-                                                  uint32_t S, uint8_t *__restrict__ md5_out) {
-    const uint32_t s = blockIdx.x * 64 + threadIdx.x;
-    if (s >= n) return;
-    const SliceDesc d = slices[s];
+__device__ __forceinline__ void slice_md5_lane(const SliceDesc d, const uint4 *tb, uint32_t S,
+                                               uint8_t *out) {
+    using W = typename std::conditional<A16, uint4, uint32_t>::type;
     const uint32_t nb = md5_blocks(S);
-    const uint32_t nbulk = d.valid / 64;  // whole blocks of real data
+    const uint32_t nmsg = S / 64;         // whole message blocks
+    const uint32_t nfull = d.valid / 64;  // of them, whole blocks of real data
+    const uint8_t *ptr = d.ptr;
     uint32_t h[4];
     md5_init(h);
-    if (A16) md5_bulk<uint4>(h, d.ptr, nbulk);
-    else md5_bulk<uint32_t>(h, d.ptr, nbulk);
-    for (uint32_t b = nbulk; b < nb; b++) {
+    md5_blocks_at<W>(h, [=](uint32_t b) {
+        const void *a = b < nfull ? (const void *)(ptr + 64 * (size_t)b)
+                      : b == nfull ? (const void *)tb : (const void *)kZeroBlock;
+        return reinterpret_cast<const W *>(a);
+    }, nmsg);
+    for (uint32_t b = nmsg; b < nb; b++) {
         const uint32_t base = b * 64u;
         uint32_t x[16];
 #pragma unroll
@@ -666,19 +724,28 @@ __global__ __launch_bounds__(64) void k_slice_md5(const SliceDesc *__restrict__ 
         }
         md5_compress(h, x);
     }
-    store_digest(md5_out + (size_t)s * 16, h);
+    store_digest(out, h);
 }
 
-hipError_t launch_slice_md5(const SliceDesc *d_slices, uint32_t n, uint32_t S, uint8_t *d_md5,
-                            hipStream_t stream) {
+template <bool A16>
+__global__ __launch_bounds__(64) void k_slice_md5(const SliceDesc *__restrict__ slices,
+                                                  const uint4 *__restrict__ tail_blocks, uint32_t n,
+                                                  uint32_t S, uint8_t *__restrict__ md5_out) {
+    const uint32_t s = blockIdx.x * 64 + threadIdx.x;
+    if (s >= n) return;
+    slice_md5_lane<A16>(slices[s], tail_blocks + 4 * (size_t)s, S, md5_out + (size_t)s * 16);
+}
+
+hipError_t launch_slice_md5(const SliceDesc *d_slices, const uint4 *d_tail_blocks, uint32_t n,
+                            uint32_t S, uint8_t *d_md5, hipStream_t stream) {
     if (n == 0) return hipSuccess;
     // Slices are 16-byte aligned when S is a multiple of 16 (file bases are).
     if (S % 16 == 0)
         hipLaunchKernelGGL(k_slice_md5<true>, dim3((n + 63) / 64), dim3(64), kMd5LdsReserve, stream,
-                           d_slices, n, S, d_md5);
+                           d_slices, d_tail_blocks, n, S, d_md5);
     else
         hipLaunchKernelGGL(k_slice_md5<false>, dim3((n + 63) / 64), dim3(64), kMd5LdsReserve,
-                           stream, d_slices, n, S, d_md5);
+                           stream, d_slices, d_tail_blocks, n, S, d_md5);
     return hipGetLastError();
 }
 
@@ -694,7 +761,7 @@ hipError_t launch_slice_md5(const SliceDesc *d_slices, uint32_t n, uint32_t S, u
 constexpr uint32_t kCrcPoly = 0xEDB88320u;
 
 struct CrcPowers {
-    uint32_t x2n[32];  // x^(2^k) mod P, reflected
+    uint32_t x2n[40];  // x^(2^k) mod P, reflected; 8 n < 2^35 for n < 2^32 bytes
 };
 
 __device__ __forceinline__ uint32_t crc_mulmod(uint32_t a, uint32_t b) {
@@ -712,45 +779,93 @@ __device__ __forceinline__ uint32_t crc_shift_op(const CrcPowers &pw, uint32_t n
     uint32_t p = 0x80000000u;
     uint32_t k = 3;
     while (n) {
-        if (n & 1u) p = crc_mulmod(pw.x2n[k & 31], p);
+        if (n & 1u) p = crc_mulmod(pw.x2n[k], p);
         n >>= 1;
         k++;
     }
     return p;
 }
 
-__global__ __launch_bounds__(64) void k_slice_crc32(const SliceDesc *__restrict__ slices,
-                                                    uint32_t S, uint32_t *__restrict__ crc_out,
-                                                    CrcPowers pw) {
-    __shared__ uint32_t table[256];
-    for (uint32_t e = threadIdx.x; e < 256; e += 64) {
-        uint32_t c = e;
-        for (int k = 0; k < 8; k++) c = (c & 1u) ? (kCrcPoly ^ (c >> 1)) : (c >> 1);
-        table[e] = c;
-    }
+// Four bytes per step (slicing by 4), reflected: t[0] is the byte table and
+// t[k][i] = t[k-1][i] >> 8 ^ t[0][t[k-1][i] & 0xFF], the effect of byte i
+// followed by k zero bytes.  Filled by a whole 256-thread workgroup.
+struct CrcLds {
+    uint32_t t[4][256];
+    uint32_t part[4];  // per-wave partial CRCs
+};
+
+__device__ __forceinline__ void crc_tables_init(CrcLds &L) {
+    const uint32_t i = threadIdx.x;
+    uint32_t c = i;
+    for (int k = 0; k < 8; k++) c = (c & 1u) ? (kCrcPoly ^ (c >> 1)) : (c >> 1);
+    L.t[0][i] = c;
     __syncthreads();
-    const SliceDesc d = slices[blockIdx.x];
-    const uint32_t seg = (((S + 63u) / 64u) + 3u) & ~3u;
-    const uint32_t lane = threadIdx.x;
-    const uint32_t lo = min(lane * seg, S), hi = min(lo + seg, S);
+    for (int k = 1; k < 4; k++) {
+        const uint32_t q = L.t[k - 1][i];
+        L.t[k][i] = (q >> 8) ^ L.t[0][q & 0xFFu];
+        __syncthreads();
+    }
+}
+
+__device__ __forceinline__ uint32_t crc_step4(const CrcLds &L, uint32_t c, uint32_t w) {
+    c ^= w;
+    return L.t[3][c & 0xFFu] ^ L.t[2][(c >> 8) & 0xFFu] ^ L.t[1][(c >> 16) & 0xFFu] ^
+           L.t[0][c >> 24];
+}
+
+// CRC-32 of one zero-padded S-byte slice by a whole 256-thread workgroup:
+// thread t takes bytes [t seg, (t + 1) seg) (seg a multiple of 16 when the
+// slice is 16-byte aligned), then the 256 segment CRCs are merged with
+// crc(A || B) = crc(A) * x^(8 |B|) mod P ^ crc(B), which holds for the
+// init / xorout 0xFFFFFFFF form: a shuffle tree inside each wave, then the
+// four wave results in order.  Thread 0 stores the result.
+__device__ __forceinline__ void crc_slice_block(CrcLds &L, const SliceDesc d, uint32_t S,
+                                                bool a16, const CrcPowers &pw, uint32_t *out) {
+    const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const uint32_t q = a16 ? 16u : 4u;
+    const uint32_t seg = ((S + 255u) / 256u + q - 1) / q * q;
+    const uint32_t lo = min(tid * seg, S), hi = min(lo + seg, S);
     uint32_t c = 0xFFFFFFFFu;
-    for (uint32_t o = lo; o < hi; o += 4) {
-        uint32_t w = load_word(d.ptr, o, d.valid);
-#pragma unroll
-        for (int b = 0; b < 4; b++) {
-            c = table[(c ^ w) & 0xFFu] ^ (c >> 8);
-            w >>= 8;
+    uint32_t o = lo;
+    if (a16) {
+        // whole 16-byte chunks inside the real bytes, then words
+        const uint32_t whole = d.valid > lo ? lo + (min(hi, d.valid) - lo) / 16 * 16 : lo;
+        for (; o < whole; o += 16) {
+            const uint4 v = gload(reinterpret_cast<const uint4 *>(d.ptr + o));
+            c = crc_step4(L, c, v.x);
+            c = crc_step4(L, c, v.y);
+            c = crc_step4(L, c, v.z);
+            c = crc_step4(L, c, v.w);
         }
     }
+    for (; o < hi; o += 4) c = crc_step4(L, c, load_word(d.ptr, o, d.valid));
     c ^= 0xFFFFFFFFu;
     for (uint32_t step = 1; step < 64; step <<= 1) {
         const uint32_t other = __shfl_down(c, step, 64);
         if ((lane & (2 * step - 1)) == 0) {
-            const uint32_t rlo = min((lane + step) * seg, S), rhi = min((lane + 2 * step) * seg, S);
+            const uint32_t rlo = min((tid + step) * seg, S), rhi = min((tid + 2 * step) * seg, S);
             c = crc_mulmod(crc_shift_op(pw, rhi - rlo), c) ^ other;
         }
     }
-    if (lane == 0) crc_out[blockIdx.x] = c;
+    if (lane == 0) L.part[w] = c;
+    __syncthreads();
+    if (tid == 0) {
+        uint32_t r = L.part[0];
+        for (uint32_t k = 1; k < 4; k++) {
+            const uint32_t blo = min(k * 64 * seg, S), bhi = min((k + 1) * 64 * seg, S);
+            r = crc_mulmod(crc_shift_op(pw, bhi - blo), r) ^ L.part[k];
+        }
+        *out = r;
+    }
+    __syncthreads();  // L.part free for the next slice
+}
+
+__global__ __launch_bounds__(256) void k_slice_crc32(const SliceDesc *__restrict__ slices,
+                                                     uint32_t S, uint32_t *__restrict__ crc_out,
+                                                     CrcPowers pw) {
+    __shared__ CrcLds L;
+    crc_tables_init(L);
+    crc_slice_block(L, slices[blockIdx.x], S, S % 16 == 0, pw, crc_out + blockIdx.x);
 }
 
 static CrcPowers make_crc_powers() {
@@ -764,7 +879,7 @@ static CrcPowers make_crc_powers() {
     };
     CrcPowers pw;
     pw.x2n[0] = 0x40000000u;  // x^1
-    for (int k = 1; k < 32; k++) pw.x2n[k] = mul(pw.x2n[k - 1], pw.x2n[k - 1]);
+    for (int k = 1; k < 40; k++) pw.x2n[k] = mul(pw.x2n[k - 1], pw.x2n[k - 1]);
     return pw;
 }
 
@@ -772,7 +887,7 @@ hipError_t launch_slice_crc32(const SliceDesc *d_slices, uint32_t n, uint32_t S,
                               hipStream_t stream) {
     if (n == 0) return hipSuccess;
     static const CrcPowers pw = make_crc_powers();
-    hipLaunchKernelGGL(k_slice_crc32, dim3(n), dim3(64), 0, stream, d_slices, S, d_crc, pw);
+    hipLaunchKernelGGL(k_slice_crc32, dim3(n), dim3(256), 0, stream, d_slices, S, d_crc, pw);
     return hipGetLastError();
 }
 
@@ -784,6 +899,29 @@ struct PacketPrefix {
     uint32_t w[8];  // set_id (4 words) || "PAR 2.0\0RecvSlic" (4 words)
 };
 
+// The packet hash covers set_id || type || le32(exp) || S data bytes:
+// len = 36 + S.  Block 0 holds the 36-byte prefix and data bytes 0..27;
+// block k >= 1 holds data bytes 64 k - 36 .. 64 k + 27; blocks k >= nfull =
+// len / 64 carry the padding.  md5_packet_block compresses any block k from
+// the prefix, the data and the padding.
+__device__ __forceinline__ void md5_packet_block(uint32_t h[4], const PacketPrefix &pre,
+                                                 uint32_t exp, const uint8_t *data, uint64_t len,
+                                                 uint32_t k, uint32_t nb) {
+    const uint64_t base = (uint64_t)k * 64u;
+    uint32_t x[16];
+#pragma unroll
+    for (int w = 0; w < 16; w++) {
+        const uint64_t o = base + 4u * w;
+        uint32_t v;
+        if (o < 32) v = pre.w[o / 4];
+        else if (o < 36) v = exp;
+        else if (o < len) v = gload(reinterpret_cast<const uint32_t *>(data + (o - 36)));
+        else v = md5_pad_word(o, len, w, k, nb);
+        x[w] = v;
+    }
+    md5_compress(h, x);
+}
+
 __global__ __launch_bounds__(64) void k_recovery_md5(const uint8_t *__restrict__ rec,
                                                      uint32_t count, uint32_t S, uint32_t first_exp,
                                                      PacketPrefix pre, uint8_t *__restrict__ md5_out) {
@@ -792,36 +930,17 @@ __global__ __launch_bounds__(64) void k_recovery_md5(const uint8_t *__restrict__
     const uint8_t *data = rec + (size_t)r * S;
     const uint64_t len = 36ull + S;
     const uint32_t nb = md5_blocks(len);
-    // Block 0 holds the 36-byte prefix and data bytes 0..27; blocks 1 .. nfull-1
-    // are data bytes from offset 28 on; the rest carries the padding.
     const uint32_t nfull = (uint32_t)(len / 64);
     uint32_t h[4];
     md5_init(h);
-    auto slow_block = [&](uint32_t b) {
-        const uint64_t base = (uint64_t)b * 64u;
-        uint32_t x[16];
-#pragma unroll
-        for (int w = 0; w < 16; w++) {
-            const uint64_t o = base + 4u * w;
-            uint32_t v;
-            if (o < 32) v = pre.w[o / 4];
-            else if (o < 36) v = first_exp + r;
-            else if (o < len) v = *reinterpret_cast<const uint32_t *>(data + (o - 36));
-            else v = md5_pad_word(o, len, w, b, nb);
-            x[w] = v;
-        }
-        md5_compress(h, x);
-    };
-    slow_block(0);
+    md5_packet_block(h, pre, first_exp + r, data, len, 0, nb);
     if (nfull > 1) md5_bulk<uint32_t>(h, data + 28, nfull - 1);
-    for (uint32_t b = nfull > 1 ? nfull : 1; b < nb; b++) slow_block(b);
+    for (uint32_t b = nfull > 1 ? nfull : 1; b < nb; b++)
+        md5_packet_block(h, pre, first_exp + r, data, len, b, nb);
     store_digest(md5_out + (size_t)r * 16, h);
 }
 
-hipError_t launch_recovery_md5(const uint8_t *d_rec, uint32_t count, uint32_t S,
-                               uint32_t first_exp, const uint8_t set_id[16], uint8_t *d_md5,
-                               hipStream_t stream) {
-    if (count == 0) return hipSuccess;
+static PacketPrefix make_packet_prefix(const uint8_t set_id[16]) {
     static const uint8_t type_tag[16] = {'P', 'A', 'R', ' ', '2', '.', '0', 0,
                                          'R', 'e', 'c', 'v', 'S', 'l', 'i', 'c'};
     PacketPrefix pre;
@@ -831,8 +950,244 @@ hipError_t launch_recovery_md5(const uint8_t *d_rec, uint32_t count, uint32_t S,
         pre.w[4 + w] = (uint32_t)type_tag[4 * w] | ((uint32_t)type_tag[4 * w + 1] << 8) |
                        ((uint32_t)type_tag[4 * w + 2] << 16) | ((uint32_t)type_tag[4 * w + 3] << 24);
     }
+    return pre;
+}
+
+hipError_t launch_recovery_md5(const uint8_t *d_rec, uint32_t count, uint32_t S,
+                               uint32_t first_exp, const uint8_t set_id[16], uint8_t *d_md5,
+                               hipStream_t stream) {
+    if (count == 0) return hipSuccess;
+    const PacketPrefix pre = make_packet_prefix(set_id);
     hipLaunchKernelGGL(k_recovery_md5, dim3((count + 63) / 64), dim3(64), kMd5LdsReserve, stream,
                        d_rec, count, S, first_exp, pre, d_md5);
+    return hipGetLastError();
+}
+
+// ======================================================================
+// Fused create pass: encode, slice MD5 and packet MD5 in one launch
+// ======================================================================
+//
+// One workgroup per CU.  Workgroups [0, h3) hash recovery packets (one lane
+// per packet), [h3, h3 + h2) hash slices (one lane per slice), the rest take
+// encode units in a fixed stride.  A unit is one row block (8 tiles = 16
+// recovery rows) of one 512-byte column group, all K steps.  Units go band by
+// band (band_cols column groups per band) and row block by row block inside
+// a band, so the workgroups running at one time share the A fragments of one
+// or two row blocks and re-read only the source columns of one band.  Each
+// finished unit is counted in its band's counter (every thread's stores are
+// released at agent scope first); a packet lane hashes the blocks of band b
+// once that count is full (acquire), so the packet hashes run under the
+// encode instead of after it.  The MD5 workgroups keep one CU each, one wave
+// per SIMD, for the serial hashes of their lanes.
+
+struct PassArgs {
+    const SliceDesc *inputs;
+    const uint4 *tail_blocks;  // 4 per slice (k_desc_tails)
+    const uint4 *afrag;
+    uint8_t *rec;
+    uint8_t *slice_md5;
+    uint32_t *slice_crc;
+    uint8_t *pkt_md5;
+    uint32_t *sync;  // [kPassFlags] errors, [kPassBands + b] band counts
+    PacketPrefix pre;
+    uint32_t n, nq, R, S;
+    uint32_t row_blocks, col_groups, band_cols, n_bands, n_units;
+    uint32_t h3_blocks, h2_blocks;
+    uint32_t stall_ticks;  // give up after this long without progress (100 MHz ticks)
+    CrcPowers pw;
+};
+
+// Waits until *cnt >= target, then acquires at agent scope.  The count is
+// read wave-uniformly.  Returns false (and sets kPassErrWait) when it has not
+// moved for stall_ticks of the 100 MHz real-time counter, so a wait always
+// ends.
+__device__ bool pass_wait(const uint32_t *cnt, uint32_t target, uint32_t stall_ticks,
+                          uint32_t *flags) {
+    uint32_t seen = __builtin_amdgcn_readfirstlane(
+        __hip_atomic_load(cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    uint64_t since = __builtin_amdgcn_s_memrealtime();
+    while (seen < target) {
+        __builtin_amdgcn_s_sleep(16);
+        const uint32_t now = __builtin_amdgcn_readfirstlane(
+            __hip_atomic_load(cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        const uint64_t t = __builtin_amdgcn_s_memrealtime();
+        if (now != seen) {
+            seen = now;
+            since = t;
+        } else if (t - since > stall_ticks) {
+            atomicOr(flags, kPassErrWait);
+            return false;
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    return true;
+}
+
+// Packet r (exponent r): the blocks of each band as it completes, then the
+// padding.  Blocks [1, bulk_end) are whole data blocks (md5_bulk).
+__device__ void pass_packet_lane(const PassArgs &a, uint32_t r) {
+    const uint8_t *data = a.rec + (size_t)r * a.S;
+    const uint64_t len = 36ull + a.S;
+    const uint32_t nb = md5_blocks(len), nfull = (uint32_t)(len / 64);
+    const uint32_t bulk_end = nfull > 1 ? nfull : 1;
+    uint32_t h[4];
+    md5_init(h);
+    uint32_t k = 0;  // next block to compress
+    for (uint32_t b = 0; b < a.n_bands; b++) {
+        const uint32_t w = min(a.band_cols, a.col_groups - b * a.band_cols);
+        if (!pass_wait(a.sync + kPassBands + b, w * a.row_blocks, a.stall_ticks, a.sync + kPassFlags))
+            return;
+        // data bytes [0, e) of every block are final; block k needs up to 64 k + 28
+        const uint64_t e = min((uint64_t)(b * a.band_cols + w) * (kMfThreads * 2), (uint64_t)a.S);
+        const uint32_t avail = b + 1 == a.n_bands ? nb : e >= 28 ? (uint32_t)((e - 28) / 64) + 1 : 0;
+        if (k == 0 && avail > 0) {
+            md5_packet_block(h, a.pre, r, data, len, 0, nb);
+            k = 1;
+        }
+        const uint32_t stop = min(avail, bulk_end);
+        if (stop > k) {
+            md5_bulk<uint32_t>(h, data + 64 * (size_t)k - 36, stop - k);
+            k = stop;
+        }
+    }
+    for (; k < nb; k++) md5_packet_block(h, a.pre, r, data, len, k, nb);
+    store_digest(a.pkt_md5 + (size_t)r * 16, h);
+}
+
+// Real-time stamps of the pass (one thread per workgroup calls these): the
+// first start and the last end of each role's work.
+__device__ __forceinline__ unsigned long long *pass_times(const PassArgs &a) {
+    return reinterpret_cast<unsigned long long *>(a.sync + kPassTimes);
+}
+__device__ __forceinline__ void pass_stamp_start(const PassArgs &a, int role) {
+    atomicMin(pass_times(a) + 2 * role, (unsigned long long)__builtin_amdgcn_s_memrealtime());
+}
+__device__ __forceinline__ void pass_stamp_end(const PassArgs &a, int role) {
+    atomicMax(pass_times(a) + 2 * role + 1, (unsigned long long)__builtin_amdgcn_s_memrealtime());
+}
+
+__global__ __launch_bounds__(kMfThreads, 1) void k_create_pass(PassArgs a) {
+    __shared__ MfLds lds;
+    __shared__ CrcLds crc;
+    const uint32_t blk = blockIdx.x, tid = threadIdx.x;
+    // MD5 workgroups run one wave (64 lanes): each lane streams its own
+    // message, and one wave's reads in flight are what a CU's L1 can track.
+    if (blk < a.h3_blocks) {
+        if (tid >= 64) return;
+        if (tid == 0) pass_stamp_start(a, kPassPacket);
+        for (uint32_t r = blk * 64 + tid; r < a.R; r += a.h3_blocks * 64) pass_packet_lane(a, r);
+        if (tid == 0) pass_stamp_end(a, kPassPacket);
+        return;
+    }
+    if (blk < a.h3_blocks + a.h2_blocks) {
+        if (tid >= 64) return;
+        if (tid == 0) pass_stamp_start(a, kPassSlice);
+        const uint32_t hb = blk - a.h3_blocks;
+        for (uint32_t s = hb * 64 + tid; s < a.n; s += a.h2_blocks * 64) {
+            // slices start 16-byte aligned when S is a multiple of 16 (file bases are)
+            const uint4 *tb = a.tail_blocks + 4 * (size_t)s;
+            if (a.S % 16 == 0) slice_md5_lane<true>(a.inputs[s], tb, a.S, a.slice_md5 + (size_t)s * 16);
+            else slice_md5_lane<false>(a.inputs[s], tb, a.S, a.slice_md5 + (size_t)s * 16);
+        }
+        if (tid == 0) pass_stamp_end(a, kPassSlice);
+        return;
+    }
+    if (tid == 0) pass_stamp_start(a, kPassEncode);
+    mf_load_desc(lds, a.inputs, a.n, a.nq);
+    crc_tables_init(crc);  // ends with a barrier: the descriptor table is ready too
+    // Encode workgroup e takes units e, e + E, e + 2E, ... (E encode
+    // workgroups): all units of a kind cost the same, the order stays band by
+    // band, and the unit index is a scalar every thread computes, so the loop
+    // has no divergent exit around its barriers (a unit index fetched by one
+    // thread and broadcast through LDS let the compiler loop the other lanes
+    // back to the barrier without that thread).
+    const uint32_t first = a.h3_blocks + a.h2_blocks;
+    const uint32_t e_blocks = gridDim.x - first;
+    const uint32_t per_band = a.row_blocks * a.band_cols;
+    for (uint32_t u = blk - first; u < a.n_units + a.n; u += e_blocks) {
+        if (u >= a.n_units) {  // CRC of slice u - n_units, after the encode units
+            const uint32_t sl = u - a.n_units;
+            if (tid == 0) pass_stamp_start(a, kPassCrc);
+            crc_slice_block(crc, a.inputs[sl], a.S, a.S % 16 == 0, a.pw, a.slice_crc + sl);
+            if (tid == 0) pass_stamp_end(a, kPassCrc);
+            continue;
+        }
+        const uint32_t band = u / per_band, v = u - band * per_band;
+        const uint32_t w = min(a.band_cols, a.col_groups - band * a.band_cols);
+        const uint32_t y = v / w, x = band * a.band_cols + v % w;
+        mf_unit(lds, a.inputs, a.afrag, a.nq, a.R, a.S, a.rec, 0, y * kMfTiles,
+                x * (kMfThreads * 2));
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __syncthreads();  // every thread's stores released before the count
+        if (tid == 0) {
+            __hip_atomic_fetch_add(a.sync + kPassBands + band, 1u, __ATOMIC_RELEASE,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            pass_stamp_end(a, kPassEncode);
+        }
+    }
+}
+
+__global__ void k_pass_reset(uint32_t *sync, uint32_t words) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < words; i += gridDim.x * blockDim.x)
+        // start stamps (even u64 slots) are minima over workgroups
+        sync[i] = (i >= kPassTimes && i < kPassBands && ((i - kPassTimes) / 2) % 2 == 0)
+                      ? 0xFFFFFFFFu
+                      : 0u;
+}
+
+bool plan_create_pass(uint32_t n, uint32_t R, uint32_t S, uint32_t num_cus, uint32_t band_cols,
+                      PassPlan *p) {
+    if (n == 0 || R == 0 || 4 * mfma_steps(n) > (uint32_t)kMfMaxInputs) return false;
+    p->grid = num_cus;
+    // one 64-lane wave per MD5 workgroup; lanes loop when the caps bind
+    p->h3_blocks = std::min<uint32_t>((R + 63) / 64, 8);
+    p->h2_blocks = std::min<uint32_t>((n + 63) / 64, 32);
+    if (4 * (p->h2_blocks + p->h3_blocks) >= num_cus) return false;
+    p->row_blocks = mfma_row_tiles(R) / kMfTiles;
+    p->col_groups = (S + kMfThreads * 2 - 1) / (kMfThreads * 2);
+    p->band_cols = std::max<uint32_t>(1, std::min(band_cols ? band_cols : 128u, p->col_groups));
+    p->n_bands = (p->col_groups + p->band_cols - 1) / p->band_cols;
+    p->n_units = p->row_blocks * p->col_groups;
+    return true;
+}
+
+uint32_t pass_sync_words(const PassPlan &p) { return kPassBands + p.n_bands; }
+
+hipError_t launch_create_pass(const PassPlan &p, const SliceDesc *d_inputs,
+                              const uint4 *d_tail_blocks, uint32_t n,
+                              const uint4 *d_afrag, uint32_t R, uint32_t S, uint8_t *d_rec,
+                              uint8_t *d_slice_md5, uint32_t *d_slice_crc, const uint8_t set_id[16],
+                              uint8_t *d_pkt_md5, uint32_t *d_sync, hipStream_t stream) {
+    const uint32_t words = pass_sync_words(p);
+    hipLaunchKernelGGL(k_pass_reset, dim3(std::min<uint32_t>((words + 255) / 256, 64)), dim3(256),
+                       0, stream, d_sync, words);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    PassArgs a;
+    a.inputs = d_inputs;
+    a.tail_blocks = d_tail_blocks;
+    a.afrag = d_afrag;
+    a.rec = d_rec;
+    a.slice_md5 = d_slice_md5;
+    a.slice_crc = d_slice_crc;
+    a.pkt_md5 = d_pkt_md5;
+    a.sync = d_sync;
+    a.pre = make_packet_prefix(set_id);
+    a.n = n;
+    a.nq = mfma_steps(n);
+    a.R = R;
+    a.S = S;
+    a.row_blocks = p.row_blocks;
+    a.col_groups = p.col_groups;
+    a.band_cols = p.band_cols;
+    a.n_bands = p.n_bands;
+    a.n_units = p.n_units;
+    a.h3_blocks = p.h3_blocks;
+    a.h2_blocks = p.h2_blocks;
+    a.stall_ticks = 200000000u;  // 2 s of the 100 MHz counter without a finished unit
+    static const CrcPowers pw = make_crc_powers();
+    a.pw = pw;
+    hipLaunchKernelGGL(k_create_pass, dim3(p.grid), dim3(kMfThreads), 0, stream, a);
     return hipGetLastError();
 }
 
