@@ -138,6 +138,7 @@ Ctx::~Ctx() {
     d_crc.release();
     d_stage.release();
     d_sync.release();
+    d_partials.release();
     for (hipEvent_t ev : scratch_done)
         if (ev) hipEventDestroy(ev);
     for (DescSlot *sl : {&enc_descs, &sum_descs}) {
@@ -456,9 +457,32 @@ int p2g_dev_create_pass(p2g_ctx *ctx, const p2g_dev_file *files, uint32_t nfiles
     const uint32_t S = (uint32_t)slice_size;
     hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c->stream;
     rc = c->scratch_acquire(Ctx::kScratchEnc, st);
-    if (rc == P2G_OK) rc = upload_descs(c, c->enc_descs, descs, st);
+    if (rc != P2G_OK) return rc;
+    // Partial slices (the last one of each file) are staged into zero-padded
+    // copies when they fit kStageBudget, so that every descriptor is a whole
+    // slice and the encode takes the full-slice loads.  The copies go after
+    // the n descriptors in the same table: {source, valid, staging slot}.
+    std::vector<SliceDesc> parts;
+    for (uint32_t i = 0; i < n; i++)
+        if (descs[i].valid < S) parts.push_back({descs[i].ptr, descs[i].valid, (uint32_t)parts.size()});
+    const bool stage = !parts.empty() && (uint64_t)parts.size() * S <= kStageBudget;
+    if (stage) {
+        if (c->d_partials.reserve(parts.size() * (size_t)S) != P2G_OK)
+            return c->fail(P2G_ERR_OUT_OF_MEMORY, "partial slice staging");
+        uint8_t *staging = static_cast<uint8_t *>(c->d_partials.ptr);
+        for (uint32_t i = 0, k = 0; i < n; i++)
+            if (descs[i].valid < S) descs[i] = {staging + (size_t)(k++) * S, S, 0};
+        descs.insert(descs.end(), parts.begin(), parts.end());
+    }
+    const bool all_full = parts.empty() || stage;
+    rc = upload_descs(c, c->enc_descs, descs, st);
     if (rc != P2G_OK) return rc;
     SliceDesc *dd = static_cast<SliceDesc *>(c->enc_descs.d.ptr);
+    if (stage) {
+        hipError_t es = launch_stage_partials(dd + n, (uint32_t)parts.size(), S,
+                                              static_cast<uint8_t *>(c->d_partials.ptr), st);
+        if (es != hipSuccess) return c->fail(P2G_ERR_DEVICE, "partial slice staging kernel", es);
+    }
     if (c->enc_descs.tb.reserve((size_t)std::max<uint32_t>(n, 1) * 64) != P2G_OK)
         return c->fail(P2G_ERR_OUT_OF_MEMORY, "slice tail blocks");
     uint4 *tb = static_cast<uint4 *>(c->enc_descs.tb.ptr);
@@ -493,7 +517,7 @@ int p2g_dev_create_pass(p2g_ctx *ctx, const p2g_dev_file *files, uint32_t nfiles
                               static_cast<const uint16_t *>(c->d_base_logs.ptr), n, R, afrag, st);
     if (e == hipSuccess)
         e = launch_create_pass(plan, dd, tb, n, afrag, R, S, rec, slice_md5_out, slice_crc_out, set_id,
-                               pkt_md5_out, static_cast<uint32_t *>(c->d_sync.ptr), st);
+                               pkt_md5_out, static_cast<uint32_t *>(c->d_sync.ptr), all_full, st);
     if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "create pass kernels", e);
     c->pass_pending = true;
     return c->scratch_release(Ctx::kScratchEnc, st);

@@ -15,6 +15,7 @@ constexpr uint32_t kMaxInputs = 32768;            // MAX_INPUT_BLOCKS, src/creat
 constexpr uint64_t kMaxSliceBytes = 1ull << 31;   // kernels index slices with u32
 constexpr uint64_t kMatsBudget = 256ull << 20;    // coefficient matrices per launch chunk
 constexpr uint64_t kPassAfragBudget = 4ull << 30; // A fragments of one fused create pass
+constexpr uint64_t kStageBudget = 256ull << 20;   // zero-padded copies of partial slices
 
 // Grow-only device allocation.
 struct DevBuf {
@@ -59,7 +60,8 @@ struct Ctx {
 
     DevBuf d_antilog, d_base_logs;
     DevBuf d_mats, d_md5, d_crc, d_stage;
-    DevBuf d_sync;                 // fused-pass queue, error flags and band counts
+    DevBuf d_sync;                 // fused-pass error flags, stamps and band counts
+    DevBuf d_partials;             // zero-padded copies of partial slices (fused pass)
     bool pass_pending = false;     // a fused pass may have set error flags not yet read
     // Shared device buffers and the event recorded after the last kernel that
     // reads them; an entry point that rewrites them first makes its stream

@@ -87,7 +87,12 @@ hipError_t launch_create_pass(const PassPlan &p, const SliceDesc *d_inputs,
                               const uint4 *d_tail_blocks, uint32_t n, const uint4 *d_afrag,
                               uint32_t R, uint32_t S, uint8_t *d_rec,
                               uint8_t *d_slice_md5, uint32_t *d_slice_crc, const uint8_t set_id[16],
-                              uint8_t *d_pkt_md5, uint32_t *d_sync, hipStream_t stream);
+                              uint8_t *d_pkt_md5, uint32_t *d_sync, bool all_full,
+                              hipStream_t stream);
+// Copies the nparts partial slices described by d_parts (ptr, valid, and in
+// `tail` the staging slot) into zero-padded S-byte slots of d_staging.
+hipError_t launch_stage_partials(const SliceDesc *d_parts, uint32_t nparts, uint32_t S,
+                                 uint8_t *d_staging, hipStream_t stream);
 
 // MD5 of each zero-padded S-byte slice (16 bytes per slice into d_md5).
 // d_tail_blocks: the 64-byte tail blocks of launch_desc_tails.

@@ -119,6 +119,31 @@ hipError_t launch_desc_tails(SliceDesc *d_descs, uint32_t n, uint4 *d_tail_block
     return hipGetLastError();
 }
 
+// Partial slices copied into zero-padded S-byte staging slots: thread per
+// staging word, parts[k] = {source, valid bytes, slot}.
+__global__ __launch_bounds__(256) void k_stage_partials(const SliceDesc *__restrict__ parts,
+                                                        uint32_t nparts, uint32_t S,
+                                                        uint8_t *__restrict__ staging) {
+    const uint32_t words = S / 4;
+    const uint64_t total = (uint64_t)nparts * words;
+    for (uint64_t idx = (uint64_t)blockIdx.x * 256 + threadIdx.x; idx < total;
+         idx += (uint64_t)gridDim.x * 256) {
+        const uint32_t k = (uint32_t)(idx / words), w = (uint32_t)(idx % words);
+        const SliceDesc d = parts[k];
+        reinterpret_cast<uint32_t *>(staging + (size_t)d.tail * S)[w] = load_word(d.ptr, 4 * w, d.valid);
+    }
+}
+
+hipError_t launch_stage_partials(const SliceDesc *d_parts, uint32_t nparts, uint32_t S,
+                                 uint8_t *d_staging, hipStream_t stream) {
+    if (nparts == 0) return hipSuccess;
+    const uint64_t total = (uint64_t)nparts * (S / 4);
+    const uint32_t blocks = (uint32_t)std::min<uint64_t>((total + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_stage_partials, dim3(blocks), dim3(256), 0, stream, d_parts, nparts, S,
+                       d_staging);
+    return hipGetLastError();
+}
+
 // A word that is always zero: the target of masked-off word loads.
 __device__ const uint32_t kZeroWord = 0;
 
@@ -363,12 +388,25 @@ constexpr int kMfAhead = 4;  // K steps between issuing a load and consuming it
 // The descriptor is read as one 16-byte LDS load {ptr lo, ptr hi, valid,
 // tail} and all three addresses are formed unconditionally, so the select
 // stays two conditional moves per address instead of a branch.
+//
+// FULL: every descriptor is a whole slice (valid = S; partial slices were
+// staged into zero-padded copies), entries past the inputs repeat input 0
+// (their A fragments are zero) and `off` is clamped into the slice, so the
+// word is just ptr + off.
+template <bool FULL>
 __device__ __forceinline__ void mf_load_words(const uint4 *s_desc, const SliceDesc *__restrict__ g_desc,
                                               uint32_t g, uint32_t h, uint32_t off, uint32_t &w0,
                                               uint32_t &w1) {
     static_assert(sizeof(SliceDesc) == 16 && offsetof(SliceDesc, valid) == 8 &&
                       offsetof(SliceDesc, tail) == 12,
                   "descriptor layout read as uint4");
+    if (FULL) {
+        const uint2 p0 = reinterpret_cast<const uint2 *>(s_desc + 4 * g + 2 * h)[0];
+        const uint2 p1 = reinterpret_cast<const uint2 *>(s_desc + 4 * g + 2 * h + 1)[0];
+        w0 = gload(reinterpret_cast<const uint32_t *>(((uint64_t)p0.y << 32 | p0.x) + off));
+        w1 = gload(reinterpret_cast<const uint32_t *>(((uint64_t)p1.y << 32 | p1.x) + off));
+        return;
+    }
     const uint64_t zero = reinterpret_cast<uint64_t>(&kZeroWord);
     uint32_t wd[2];
 #pragma unroll
@@ -409,18 +447,23 @@ struct MfLds {
     uint4 a[2][kMfTiles * 64];
 };
 
-// Fills lds.desc with the n descriptors and 4 nq - n empty ones (valid = 0).
-// The caller synchronizes before the first unit.
+// Fills lds.desc with the n descriptors and 4 nq - n more: empty ones
+// (valid = 0), or, FULL, copies of input 0 (read, never counted).  The caller
+// synchronizes before the first unit.
+template <bool FULL>
 __device__ __forceinline__ void mf_load_desc(MfLds &lds, const SliceDesc *__restrict__ inputs,
                                              uint32_t n, uint32_t nq) {
     for (uint32_t i = threadIdx.x; i < 4 * nq; i += kMfThreads)
-        lds.desc[i] = i < n ? gload(reinterpret_cast<const uint4 *>(inputs) + i) : make_uint4(0, 0, 0, 0);
+        lds.desc[i] = i < n  ? gload(reinterpret_cast<const uint4 *>(inputs) + i)
+                      : FULL ? gload(reinterpret_cast<const uint4 *>(inputs))
+                             : make_uint4(0, 0, 0, 0);
 }
 
 // One encode unit: row tiles tile0 .. tile0 + 7 (recovery rows 2 tile0 ..
 // 2 tile0 + 15) over the 256 symbols at byte colbase of every slice, all nq K
 // steps, stored to rec (XORed in when accumulate).  All 256 threads call it;
 // it leaves lds.a free for the next unit.
+template <bool FULL>
 __device__ __forceinline__ void mf_unit(MfLds &lds, const SliceDesc *__restrict__ inputs,
                                         const uint4 *__restrict__ afrag, uint32_t nq, uint32_t R,
                                         uint32_t S, uint8_t *__restrict__ rec, int accumulate,
@@ -433,6 +476,9 @@ __device__ __forceinline__ void mf_unit(MfLds &lds, const SliceDesc *__restrict_
     // this lane's dword: symbols base + 2c (column tile 0) and base + 2c + 1 (tile 1)
     const uint32_t off = colbase + w * 128 + 4 * c;
     const bool col_ok = off < S;
+    // FULL loads stay inside the slice; columns past S are computed from any
+    // bytes and never stored.
+    const uint32_t off_ld = FULL ? min(off, S - 4) : off;
 
     // Thread tid stages A entries tid and tid + 256 of each K step: tile
     // (e / 64), lane (e % 64) of fragment (tile0 + tile, g).
@@ -461,7 +507,7 @@ __device__ __forceinline__ void mf_unit(MfLds &lds, const SliceDesc *__restrict_
         const uint32_t gp = min((uint32_t)(P), nq - 1);                                       \
         a0_##P = ga0[(size_t)gp * 64];                                                        \
         a1_##P = ga1[(size_t)gp * 64];                                                        \
-        mf_load_words(s_desc, inputs, gp, h, off, w0_##P, w1_##P);                         \
+        mf_load_words<FULL>(s_desc, inputs, gp, h, off_ld, w0_##P, w1_##P);                         \
     } while (0)
     MF_FILL(0);
     MF_FILL(1);
@@ -479,7 +525,7 @@ __device__ __forceinline__ void mf_unit(MfLds &lds, const SliceDesc *__restrict_
         const uint32_t gf = min((GG) + kMfAhead, nq - 1);                                     \
         a0_##P = ga0[(size_t)gf * 64];                                                        \
         a1_##P = ga1[(size_t)gf * 64];                                                        \
-        mf_load_words(s_desc, inputs, gf, h, off, w0_##P, w1_##P);                         \
+        mf_load_words<FULL>(s_desc, inputs, gf, h, off_ld, w0_##P, w1_##P);                         \
         const uint4 *sa = s_a[(GG) & 1];                                                      \
         v8i_t Bn[2];                                                                          \
         mf_make_b(w0_##PN, w1_##PN, Bn);                                                      \
@@ -551,10 +597,10 @@ __global__ __launch_bounds__(kMfThreads, 1) void k_encode_mfma(
     const SliceDesc *__restrict__ inputs, uint32_t n, const uint4 *__restrict__ afrag,
     uint32_t nq, uint32_t R, uint32_t S, uint8_t *__restrict__ rec, int accumulate) {
     __shared__ MfLds lds;
-    mf_load_desc(lds, inputs, n, nq);
+    mf_load_desc<false>(lds, inputs, n, nq);
     __syncthreads();
-    mf_unit(lds, inputs, afrag, nq, R, S, rec, accumulate, blockIdx.y * kMfTiles,
-            blockIdx.x * (kMfThreads * 2));
+    mf_unit<false>(lds, inputs, afrag, nq, R, S, rec, accumulate, blockIdx.y * kMfTiles,
+                   blockIdx.x * (kMfThreads * 2));
 }
 
 hipError_t launch_encode_mfma(const SliceDesc *d_inputs, uint32_t n_inputs, const uint4 *d_afrag,
@@ -1001,7 +1047,7 @@ struct PassArgs {
 // read wave-uniformly.  Returns false (and sets kPassErrWait) when it has not
 // moved for stall_ticks of the 100 MHz real-time counter, so a wait always
 // ends.
-__device__ bool pass_wait(const uint32_t *cnt, uint32_t target, uint32_t stall_ticks,
+__device__ __forceinline__ bool pass_wait(const uint32_t *cnt, uint32_t target, uint32_t stall_ticks,
                           uint32_t *flags) {
     uint32_t seen = __builtin_amdgcn_readfirstlane(
         __hip_atomic_load(cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
@@ -1025,7 +1071,7 @@ __device__ bool pass_wait(const uint32_t *cnt, uint32_t target, uint32_t stall_t
 
 // Packet r (exponent r): the blocks of each band as it completes, then the
 // padding.  Blocks [1, bulk_end) are whole data blocks (md5_bulk).
-__device__ void pass_packet_lane(const PassArgs &a, uint32_t r) {
+__device__ __forceinline__ void pass_packet_lane(const PassArgs &a, uint32_t r) {
     const uint8_t *data = a.rec + (size_t)r * a.S;
     const uint64_t len = 36ull + a.S;
     const uint32_t nb = md5_blocks(len), nfull = (uint32_t)(len / 64);
@@ -1066,6 +1112,7 @@ __device__ __forceinline__ void pass_stamp_end(const PassArgs &a, int role) {
     atomicMax(pass_times(a) + 2 * role + 1, (unsigned long long)__builtin_amdgcn_s_memrealtime());
 }
 
+template <bool FULL>
 __global__ __launch_bounds__(kMfThreads, 1) void k_create_pass(PassArgs a) {
     __shared__ MfLds lds;
     __shared__ CrcLds crc;
@@ -1093,7 +1140,7 @@ __global__ __launch_bounds__(kMfThreads, 1) void k_create_pass(PassArgs a) {
         return;
     }
     if (tid == 0) pass_stamp_start(a, kPassEncode);
-    mf_load_desc(lds, a.inputs, a.n, a.nq);
+    mf_load_desc<FULL>(lds, a.inputs, a.n, a.nq);
     crc_tables_init(crc);  // ends with a barrier: the descriptor table is ready too
     // Encode workgroup e takes units e, e + E, e + 2E, ... (E encode
     // workgroups): all units of a kind cost the same, the order stays band by
@@ -1104,6 +1151,12 @@ __global__ __launch_bounds__(kMfThreads, 1) void k_create_pass(PassArgs a) {
     const uint32_t first = a.h3_blocks + a.h2_blocks;
     const uint32_t e_blocks = gridDim.x - first;
     const uint32_t per_band = a.row_blocks * a.band_cols;
+    // Finished units are counted per band and published once per band: when
+    // the next unit of this workgroup lies in another band, or after the last
+    // one.  Publishing = every thread's stores released to the workgroup (they
+    // reach L2), a barrier, then one agent-scope release (L2 write-back) and
+    // one add by thread 0.
+    uint32_t done = 0;  // units of band `band` finished and not yet counted
     for (uint32_t u = blk - first; u < a.n_units + a.n; u += e_blocks) {
         if (u >= a.n_units) {  // CRC of slice u - n_units, after the encode units
             const uint32_t sl = u - a.n_units;
@@ -1115,14 +1168,20 @@ __global__ __launch_bounds__(kMfThreads, 1) void k_create_pass(PassArgs a) {
         const uint32_t band = u / per_band, v = u - band * per_band;
         const uint32_t w = min(a.band_cols, a.col_groups - band * a.band_cols);
         const uint32_t y = v / w, x = band * a.band_cols + v % w;
-        mf_unit(lds, a.inputs, a.afrag, a.nq, a.R, a.S, a.rec, 0, y * kMfTiles,
-                x * (kMfThreads * 2));
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        __syncthreads();  // every thread's stores released before the count
-        if (tid == 0) {
-            __hip_atomic_fetch_add(a.sync + kPassBands + band, 1u, __ATOMIC_RELEASE,
-                                   __HIP_MEMORY_SCOPE_AGENT);
-            pass_stamp_end(a, kPassEncode);
+        mf_unit<FULL>(lds, a.inputs, a.afrag, a.nq, a.R, a.S, a.rec, 0, y * kMfTiles,
+                      x * (kMfThreads * 2));
+        done++;
+        const uint32_t nxt = u + e_blocks;
+        if (nxt >= a.n_units || nxt / per_band != band) {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+            __syncthreads();
+            if (tid == 0) {
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+                __hip_atomic_fetch_add(a.sync + kPassBands + band, done, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+                pass_stamp_end(a, kPassEncode);
+            }
+            done = 0;
         }
     }
 }
@@ -1145,7 +1204,7 @@ bool plan_create_pass(uint32_t n, uint32_t R, uint32_t S, uint32_t num_cus, uint
     if (4 * (p->h2_blocks + p->h3_blocks) >= num_cus) return false;
     p->row_blocks = mfma_row_tiles(R) / kMfTiles;
     p->col_groups = (S + kMfThreads * 2 - 1) / (kMfThreads * 2);
-    p->band_cols = std::max<uint32_t>(1, std::min(band_cols ? band_cols : 128u, p->col_groups));
+    p->band_cols = std::max<uint32_t>(1, std::min(band_cols ? band_cols : 64u, p->col_groups));
     p->n_bands = (p->col_groups + p->band_cols - 1) / p->band_cols;
     p->n_units = p->row_blocks * p->col_groups;
     return true;
@@ -1157,7 +1216,8 @@ hipError_t launch_create_pass(const PassPlan &p, const SliceDesc *d_inputs,
                               const uint4 *d_tail_blocks, uint32_t n,
                               const uint4 *d_afrag, uint32_t R, uint32_t S, uint8_t *d_rec,
                               uint8_t *d_slice_md5, uint32_t *d_slice_crc, const uint8_t set_id[16],
-                              uint8_t *d_pkt_md5, uint32_t *d_sync, hipStream_t stream) {
+                              uint8_t *d_pkt_md5, uint32_t *d_sync, bool all_full,
+                              hipStream_t stream) {
     const uint32_t words = pass_sync_words(p);
     hipLaunchKernelGGL(k_pass_reset, dim3(std::min<uint32_t>((words + 255) / 256, 64)), dim3(256),
                        0, stream, d_sync, words);
@@ -1187,7 +1247,10 @@ hipError_t launch_create_pass(const PassPlan &p, const SliceDesc *d_inputs,
     a.stall_ticks = 200000000u;  // 2 s of the 100 MHz counter without a finished unit
     static const CrcPowers pw = make_crc_powers();
     a.pw = pw;
-    hipLaunchKernelGGL(k_create_pass, dim3(p.grid), dim3(kMfThreads), 0, stream, a);
+    if (all_full)
+        hipLaunchKernelGGL(k_create_pass<true>, dim3(p.grid), dim3(kMfThreads), 0, stream, a);
+    else
+        hipLaunchKernelGGL(k_create_pass<false>, dim3(p.grid), dim3(kMfThreads), 0, stream, a);
     return hipGetLastError();
 }
 
