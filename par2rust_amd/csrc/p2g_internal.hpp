@@ -66,11 +66,23 @@ hipError_t launch_encode_mfma(const SliceDesc *d_inputs, uint32_t n_inputs, cons
 // sync words: [kPassFlags] error flags; from kPassTimes, 2 kPassRoles u64
 // real-time counter stamps (100 MHz): the first start and the last end of
 // each role's work (packet MD5, slice MD5, encode, CRC); [kPassBands + b]
-// finished units of column band b.
+// finished units of column band b; [kPassBands + n_bands + b] the first
+// column group of band b (n_bands + 1 entries).
 constexpr uint32_t kPassFlags = 0, kPassTimes = 2, kPassRoles = 4;
 constexpr uint32_t kPassBands = kPassTimes + 4 * kPassRoles;
 enum { kPassPacket = 0, kPassSlice = 1, kPassEncode = 2, kPassCrc = 3 };
 constexpr uint32_t kPassErrWait = 1u;  // a packet lane stopped waiting for a band
+// Width (in 512-byte column groups) of the next band after `done` groups,
+// with `remaining` left: growing from one group (1, 1, 2, 4, ... up to W) so
+// that the packet hashes can start early, then W, then halving toward the
+// end so that the last band, which they can only start once it is complete,
+// is one group again.
+__host__ __device__ inline uint32_t next_band_width(uint32_t done, uint32_t remaining, uint32_t W) {
+    const uint32_t ramp = done == 0 ? 1u : (done < W ? done : W);
+    const uint32_t tail = remaining > 2 * W ? W : remaining > 1 ? (remaining + 1) / 2 : remaining;
+    return ramp < tail ? ramp : tail;
+}
+
 struct PassPlan {
     uint32_t grid = 0, h2_blocks = 0, h3_blocks = 0;
     uint32_t row_blocks = 0, col_groups = 0, band_cols = 0, n_bands = 0, n_units = 0;

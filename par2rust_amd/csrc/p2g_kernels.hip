@@ -640,16 +640,19 @@ __device__ __forceinline__ void md5_compress(uint32_t h[4], const uint32_t x[16]
     uint32_t a = h[0], b = h[1], c = h[2], d = h[3];
 #pragma unroll
     for (int i = 0; i < 64; i++) {
+        // The four round functions as one v_bitop3 each (truth-table bit
+        // (b << 2 | c << 1 | d)): F = b ? c : d (0xCA), G = d ? b : c (0xE4),
+        // H = b ^ c ^ d (0x96), I = c ^ (b | ~d) (0x39).
         uint32_t f;
         int g;
-        if (i < 16) { f = (b & c) | (~b & d); g = i; }
-        else if (i < 32) { f = (b & d) | (c & ~d); g = (5 * i + 1) & 15; }
-        else if (i < 48) { f = b ^ c ^ d; g = (3 * i + 5) & 15; }
-        else { f = c ^ (b | ~d); g = (7 * i) & 15; }
+        if (i < 16) { f = __builtin_amdgcn_bitop3_b32(b, c, d, 0xCA); g = i; }
+        else if (i < 32) { f = __builtin_amdgcn_bitop3_b32(b, c, d, 0xE4); g = (5 * i + 1) & 15; }
+        else if (i < 48) { f = __builtin_amdgcn_bitop3_b32(b, c, d, 0x96); g = (3 * i + 5) & 15; }
+        else { f = __builtin_amdgcn_bitop3_b32(b, c, d, 0x39); g = (7 * i) & 15; }
         const uint32_t t = d;
         d = c;
         c = b;
-        b = b + rotl32(a + f + kMd5K[i] + x[g], sh[(i >> 4) * 4 + (i & 3)]);
+        b = b + rotl32(a + f + (x[g] + kMd5K[i]), sh[(i >> 4) * 4 + (i & 3)]);
         a = t;
     }
     h[0] += a; h[1] += b; h[2] += c; h[3] += d;
@@ -694,28 +697,36 @@ __device__ __forceinline__ void md5_load_block(uint32_t x[16], const W *p) {
     }
 }
 
-// nblk blocks, block b loaded from addr(b) (a const W *).
+// nblk blocks, block b loaded from addr(b) (a const W *).  Block j lives in
+// ring slot j % kMd5Ring; the load of block j + kMd5Ahead goes into the slot
+// compressed one step earlier, so the compression reads the loaded registers
+// without copies.  The main loop's loads are unconditional (their wait counts
+// are static); the tail loads only what exists.
+constexpr int kMd5Ring = kMd5Ahead + 1;
 template <typename W, typename Addr>
 __device__ __forceinline__ void md5_blocks_at(uint32_t h[4], Addr addr, uint32_t nblk) {
-    uint32_t buf[kMd5Ahead][16];
+    uint32_t buf[kMd5Ring][16];
 #pragma unroll
     for (int k = 0; k < kMd5Ahead; k++)
         if ((uint32_t)k < nblk) md5_load_block(buf[k], addr((uint32_t)k));
     uint32_t b = 0;
-    for (; b + kMd5Ahead <= nblk; b += kMd5Ahead) {
+    for (; b + kMd5Ring + kMd5Ahead <= nblk; b += kMd5Ring) {
 #pragma unroll
-        for (int k = 0; k < kMd5Ahead; k++) {
-            uint32_t x[16];
-#pragma unroll
-            for (int w = 0; w < 16; w++) x[w] = buf[k][w];
-            const uint32_t nxt = b + k + kMd5Ahead;
-            if (nxt < nblk) md5_load_block(buf[k], addr(nxt));
-            md5_compress(h, x);
+        for (int k = 0; k < kMd5Ring; k++) {
+            md5_load_block(buf[(k + kMd5Ahead) % kMd5Ring], addr(b + k + kMd5Ahead));
+            md5_compress(h, buf[k]);
         }
     }
+    for (; b < nblk; b += kMd5Ring) {
 #pragma unroll
-    for (int k = 0; k < kMd5Ahead; k++)
-        if (b + k < nblk) md5_compress(h, buf[k]);
+        for (int k = 0; k < kMd5Ring; k++) {
+            if (b + k < nblk) {
+                const uint32_t nxt = b + k + kMd5Ahead;
+                if (nxt < nblk) md5_load_block(buf[(k + kMd5Ahead) % kMd5Ring], addr(nxt));
+                md5_compress(h, buf[k]);
+            }
+        }
+    }
 }
 
 // nblk consecutive blocks at p.
@@ -853,10 +864,65 @@ __device__ __forceinline__ void crc_tables_init(CrcLds &L) {
     }
 }
 
-__device__ __forceinline__ uint32_t crc_step4(const CrcLds &L, uint32_t c, uint32_t w) {
+__device__ __forceinline__ uint32_t crc_step4(const uint32_t (*t)[256], uint32_t c, uint32_t w) {
     c ^= w;
-    return L.t[3][c & 0xFFu] ^ L.t[2][(c >> 8) & 0xFFu] ^ L.t[1][(c >> 16) & 0xFFu] ^
-           L.t[0][c >> 24];
+    return t[3][c & 0xFFu] ^ t[2][(c >> 8) & 0xFFu] ^ t[1][(c >> 16) & 0xFFu] ^ t[0][c >> 24];
+}
+__device__ __forceinline__ uint32_t crc_step4(const CrcLds &L, uint32_t c, uint32_t w) {
+    return crc_step4(L.t, c, w);
+}
+
+// The same four tables filled by one wave alone: entry t[k][i] is byte i
+// followed by k zero bytes, i.e. 8 (k + 1) bit steps from i, so each lane
+// computes its own entries and only the wave's LDS order has to hold.
+__device__ __forceinline__ void crc_tables_init_wave(uint32_t (*t)[256]) {
+    const uint32_t lane = threadIdx.x & 63;
+    for (uint32_t e = lane; e < 256; e += 64) {
+        uint32_t c = e;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+#pragma unroll
+            for (int b = 0; b < 8; b++) c = (c & 1u) ? (kCrcPoly ^ (c >> 1)) : (c >> 1);
+            t[k][e] = c;
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// CRC-32 of one zero-padded S-byte slice by one wave: lane l takes bytes
+// [l seg, (l + 1) seg), merged in a shuffle tree as in crc_slice_block.
+// Lane 0 stores the result.
+__device__ __forceinline__ void crc_slice_wave(const uint32_t (*t)[256], const SliceDesc d,
+                                               uint32_t S, bool a16, const CrcPowers &pw,
+                                               uint32_t *out) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t q = a16 ? 16u : 4u;
+    const uint32_t seg = ((S + 63u) / 64u + q - 1) / q * q;
+    const uint32_t lo = min(lane * seg, S), hi = min(lo + seg, S);
+    uint32_t c = 0xFFFFFFFFu;
+    uint32_t o = lo;
+    if (a16) {
+        const uint32_t whole = d.valid > lo ? lo + (min(hi, d.valid) - lo) / 16 * 16 : lo;
+        for (; o < whole; o += 16) {
+            const uint4 v = gload(reinterpret_cast<const uint4 *>(d.ptr + o));
+            c = crc_step4(t, c, v.x);
+            c = crc_step4(t, c, v.y);
+            c = crc_step4(t, c, v.z);
+            c = crc_step4(t, c, v.w);
+        }
+    }
+    for (; o < hi; o += 4) c = crc_step4(t, c, load_word(d.ptr, o, d.valid));
+    c ^= 0xFFFFFFFFu;
+    for (uint32_t step = 1; step < 64; step <<= 1) {
+        const uint32_t other = __shfl_down(c, step, 64);
+        if ((lane & (2 * step - 1)) == 0) {
+            const uint32_t rlo = min((lane + step) * seg, S), rhi = min((lane + 2 * step) * seg, S);
+            c = crc_mulmod(crc_shift_op(pw, rhi - rlo), c) ^ other;
+        }
+    }
+    if (lane == 0) *out = c;
 }
 
 // CRC-32 of one zero-padded S-byte slice by a whole 256-thread workgroup:
@@ -968,6 +1034,48 @@ __device__ __forceinline__ void md5_packet_block(uint32_t h[4], const PacketPref
     md5_compress(h, x);
 }
 
+// Packet blocks k0 .. k0 + nblk - 1 (k0 >= 1) when the block data starts
+// 16-byte aligned and S % 16 == 0.  Block k is data words 16k - 9 .. 16k + 6:
+// the last word of the previous block's aligned 64 bytes, carried, and the
+// first 15 words of data[64k - 32, 64k + 32), loaded as four 16-byte loads
+// (which end at or before byte S, as S is a multiple of 16).
+__device__ __forceinline__ void md5_packet_bulk16(uint32_t h[4], const uint8_t *data, uint32_t k0,
+                                                  uint32_t nblk) {
+    if (nblk == 0) return;
+    uint32_t carry = gload(reinterpret_cast<const uint32_t *>(data + 4 * (16 * (size_t)k0 - 9)));
+    const uint4 *base = reinterpret_cast<const uint4 *>(data + 64 * (size_t)k0 - 32);
+    uint32_t buf[kMd5Ring][16];  // block j in slot j % kMd5Ring, as in md5_blocks_at
+#pragma unroll
+    for (int k = 0; k < kMd5Ahead; k++)
+        if ((uint32_t)k < nblk) md5_load_block(buf[k], base + 4 * k);
+    auto step = [&](int k) {
+        uint32_t x[16];
+        x[0] = carry;
+#pragma unroll
+        for (int w = 1; w < 16; w++) x[w] = buf[k][w - 1];
+        carry = buf[k][15];
+        md5_compress(h, x);
+    };
+    uint32_t b = 0;
+    for (; b + kMd5Ring + kMd5Ahead <= nblk; b += kMd5Ring) {
+#pragma unroll
+        for (int k = 0; k < kMd5Ring; k++) {
+            md5_load_block(buf[(k + kMd5Ahead) % kMd5Ring], base + 4 * (size_t)(b + k + kMd5Ahead));
+            step(k);
+        }
+    }
+    for (; b < nblk; b += kMd5Ring) {
+#pragma unroll
+        for (int k = 0; k < kMd5Ring; k++) {
+            if (b + k < nblk) {
+                const uint32_t nxt = b + k + kMd5Ahead;
+                if (nxt < nblk) md5_load_block(buf[(k + kMd5Ahead) % kMd5Ring], base + 4 * (size_t)nxt);
+                step(k);
+            }
+        }
+    }
+}
+
 __global__ __launch_bounds__(64) void k_recovery_md5(const uint8_t *__restrict__ rec,
                                                      uint32_t count, uint32_t S, uint32_t first_exp,
                                                      PacketPrefix pre, uint8_t *__restrict__ md5_out) {
@@ -980,7 +1088,12 @@ __global__ __launch_bounds__(64) void k_recovery_md5(const uint8_t *__restrict__
     uint32_t h[4];
     md5_init(h);
     md5_packet_block(h, pre, first_exp + r, data, len, 0, nb);
-    if (nfull > 1) md5_bulk<uint32_t>(h, data + 28, nfull - 1);
+    if (nfull > 1) {
+        if (S % 16 == 0 && reinterpret_cast<uintptr_t>(rec) % 16 == 0)
+            md5_packet_bulk16(h, data, 1, nfull - 1);
+        else
+            md5_bulk<uint32_t>(h, data + 28, nfull - 1);
+    }
     for (uint32_t b = nfull > 1 ? nfull : 1; b < nb; b++)
         md5_packet_block(h, pre, first_exp + r, data, len, b, nb);
     store_digest(md5_out + (size_t)r * 16, h);
@@ -1014,16 +1127,18 @@ hipError_t launch_recovery_md5(const uint8_t *d_rec, uint32_t count, uint32_t S,
 // ======================================================================
 //
 // One workgroup per CU.  Workgroups [0, h3) hash recovery packets (one lane
-// per packet), [h3, h3 + h2) hash slices (one lane per slice), the rest take
-// encode units in a fixed stride.  A unit is one row block (8 tiles = 16
-// recovery rows) of one 512-byte column group, all K steps.  Units go band by
-// band (band_cols column groups per band) and row block by row block inside
-// a band, so the workgroups running at one time share the A fragments of one
-// or two row blocks and re-read only the source columns of one band.  Each
-// finished unit is counted in its band's counter (every thread's stores are
-// released at agent scope first); a packet lane hashes the blocks of band b
-// once that count is full (acquire), so the packet hashes run under the
-// encode instead of after it.  The MD5 workgroups keep one CU each, one wave
+// per packet), [h3, h3 + h2) hash slices (one lane per slice), both on wave
+// 0, the other waves of the slice-hash workgroups compute slice CRCs; the
+// rest take encode units in a fixed stride.  A unit is one row block (8
+// tiles = 16 recovery rows) of one 512-byte column group, all K steps.  Units
+// go band by band (next_band_width column groups per band, narrow at both
+// ends) and row block by row block inside a band, so the workgroups
+// running at one time share the A fragments of one or two row blocks and
+// re-read only the source columns of one band.  Finished units are counted
+// in their band's counter (stores released at agent scope first); a packet
+// lane hashes the blocks of band b once that count is full (acquire), so the
+// packet hashes run under the encode and only the last, narrowest band is
+// hashed after it.  The MD5 workgroups keep one CU each, one wave
 // per SIMD, for the serial hashes of their lanes.
 
 struct PassArgs {
@@ -1073,18 +1188,20 @@ __device__ __forceinline__ bool pass_wait(const uint32_t *cnt, uint32_t target, 
 // padding.  Blocks [1, bulk_end) are whole data blocks (md5_bulk).
 __device__ __forceinline__ void pass_packet_lane(const PassArgs &a, uint32_t r) {
     const uint8_t *data = a.rec + (size_t)r * a.S;
+    const bool a16 = a.S % 16 == 0 && reinterpret_cast<uintptr_t>(a.rec) % 16 == 0;
     const uint64_t len = 36ull + a.S;
     const uint32_t nb = md5_blocks(len), nfull = (uint32_t)(len / 64);
     const uint32_t bulk_end = nfull > 1 ? nfull : 1;
     uint32_t h[4];
     md5_init(h);
     uint32_t k = 0;  // next block to compress
+    const uint32_t *bstart = a.sync + kPassBands + a.n_bands;
     for (uint32_t b = 0; b < a.n_bands; b++) {
-        const uint32_t w = min(a.band_cols, a.col_groups - b * a.band_cols);
+        const uint32_t c1 = bstart[b + 1], w = c1 - bstart[b];
         if (!pass_wait(a.sync + kPassBands + b, w * a.row_blocks, a.stall_ticks, a.sync + kPassFlags))
             return;
         // data bytes [0, e) of every block are final; block k needs up to 64 k + 28
-        const uint64_t e = min((uint64_t)(b * a.band_cols + w) * (kMfThreads * 2), (uint64_t)a.S);
+        const uint64_t e = min((uint64_t)c1 * (kMfThreads * 2), (uint64_t)a.S);
         const uint32_t avail = b + 1 == a.n_bands ? nb : e >= 28 ? (uint32_t)((e - 28) / 64) + 1 : 0;
         if (k == 0 && avail > 0) {
             md5_packet_block(h, a.pre, r, data, len, 0, nb);
@@ -1092,7 +1209,8 @@ __device__ __forceinline__ void pass_packet_lane(const PassArgs &a, uint32_t r) 
         }
         const uint32_t stop = min(avail, bulk_end);
         if (stop > k) {
-            md5_bulk<uint32_t>(h, data + 64 * (size_t)k - 36, stop - k);
+            if (a16) md5_packet_bulk16(h, data, k, stop - k);
+            else md5_bulk<uint32_t>(h, data + 64 * (size_t)k - 36, stop - k);
             k = stop;
         }
     }
@@ -1115,19 +1233,30 @@ __device__ __forceinline__ void pass_stamp_end(const PassArgs &a, int role) {
 template <bool FULL>
 __global__ __launch_bounds__(kMfThreads, 1) void k_create_pass(PassArgs a) {
     __shared__ MfLds lds;
-    __shared__ CrcLds crc;
     const uint32_t blk = blockIdx.x, tid = threadIdx.x;
-    // MD5 workgroups run one wave (64 lanes): each lane streams its own
-    // message, and one wave's reads in flight are what a CU's L1 can track.
+    // MD5 workgroups run their hash on wave 0 (64 lanes: each lane streams its
+    // own message).  In the slice-hash workgroups waves 1-3, on the other
+    // SIMDs, take the slice CRCs, one slice per wave, with their tables in the
+    // LDS the encode would use; the packet-hash workgroups keep their CU's
+    // memory path to themselves (they must keep pace with the encode).
+    if (blk < a.h3_blocks && tid >= 64) return;
+    if (blk < a.h3_blocks + a.h2_blocks && tid >= 64) {
+        const uint32_t cw = (blk - a.h3_blocks) * 3 + (tid >> 6) - 1, n_cw = 3 * a.h2_blocks;
+        uint32_t(*t)[256] = reinterpret_cast<uint32_t(*)[256]>(lds.desc) + 4 * ((tid >> 6) - 1);
+        crc_tables_init_wave(t);
+        if (tid % 64 == 0) pass_stamp_start(a, kPassCrc);
+        for (uint32_t sl = cw; sl < a.n; sl += n_cw)
+            crc_slice_wave(t, a.inputs[sl], a.S, a.S % 16 == 0, a.pw, a.slice_crc + sl);
+        if (tid % 64 == 0) pass_stamp_end(a, kPassCrc);
+        return;
+    }
     if (blk < a.h3_blocks) {
-        if (tid >= 64) return;
         if (tid == 0) pass_stamp_start(a, kPassPacket);
         for (uint32_t r = blk * 64 + tid; r < a.R; r += a.h3_blocks * 64) pass_packet_lane(a, r);
         if (tid == 0) pass_stamp_end(a, kPassPacket);
         return;
     }
     if (blk < a.h3_blocks + a.h2_blocks) {
-        if (tid >= 64) return;
         if (tid == 0) pass_stamp_start(a, kPassSlice);
         const uint32_t hb = blk - a.h3_blocks;
         for (uint32_t s = hb * 64 + tid; s < a.n; s += a.h2_blocks * 64) {
@@ -1141,7 +1270,7 @@ __global__ __launch_bounds__(kMfThreads, 1) void k_create_pass(PassArgs a) {
     }
     if (tid == 0) pass_stamp_start(a, kPassEncode);
     mf_load_desc<FULL>(lds, a.inputs, a.n, a.nq);
-    crc_tables_init(crc);  // ends with a barrier: the descriptor table is ready too
+    __syncthreads();  // descriptor table ready
     // Encode workgroup e takes units e, e + E, e + 2E, ... (E encode
     // workgroups): all units of a kind cost the same, the order stays band by
     // band, and the unit index is a scalar every thread computes, so the loop
@@ -1150,29 +1279,28 @@ __global__ __launch_bounds__(kMfThreads, 1) void k_create_pass(PassArgs a) {
     // back to the barrier without that thread).
     const uint32_t first = a.h3_blocks + a.h2_blocks;
     const uint32_t e_blocks = gridDim.x - first;
-    const uint32_t per_band = a.row_blocks * a.band_cols;
+    // Band b holds units [row_blocks * bstart[b], row_blocks * bstart[b + 1]).
+    const uint32_t *bstart = a.sync + kPassBands + a.n_bands;
+    uint32_t band = 0, b_lo = 0, b_hi = a.row_blocks * bstart[1];
     // Finished units are counted per band and published once per band: when
     // the next unit of this workgroup lies in another band, or after the last
     // one.  Publishing = every thread's stores released to the workgroup (they
     // reach L2), a barrier, then one agent-scope release (L2 write-back) and
     // one add by thread 0.
     uint32_t done = 0;  // units of band `band` finished and not yet counted
-    for (uint32_t u = blk - first; u < a.n_units + a.n; u += e_blocks) {
-        if (u >= a.n_units) {  // CRC of slice u - n_units, after the encode units
-            const uint32_t sl = u - a.n_units;
-            if (tid == 0) pass_stamp_start(a, kPassCrc);
-            crc_slice_block(crc, a.inputs[sl], a.S, a.S % 16 == 0, a.pw, a.slice_crc + sl);
-            if (tid == 0) pass_stamp_end(a, kPassCrc);
-            continue;
+    for (uint32_t u = blk - first; u < a.n_units; u += e_blocks) {
+        while (u >= b_hi) {
+            band++;
+            b_lo = b_hi;
+            b_hi = a.row_blocks * bstart[band + 1];
         }
-        const uint32_t band = u / per_band, v = u - band * per_band;
-        const uint32_t w = min(a.band_cols, a.col_groups - band * a.band_cols);
-        const uint32_t y = v / w, x = band * a.band_cols + v % w;
+        const uint32_t c0 = bstart[band], w = bstart[band + 1] - c0;
+        const uint32_t v = u - b_lo, y = v / w, x = c0 + v % w;
         mf_unit<FULL>(lds, a.inputs, a.afrag, a.nq, a.R, a.S, a.rec, 0, y * kMfTiles,
                       x * (kMfThreads * 2));
         done++;
         const uint32_t nxt = u + e_blocks;
-        if (nxt >= a.n_units || nxt / per_band != band) {
+        if (nxt >= a.n_units || nxt >= b_hi) {
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
             __syncthreads();
             if (tid == 0) {
@@ -1186,12 +1314,23 @@ __global__ __launch_bounds__(kMfThreads, 1) void k_create_pass(PassArgs a) {
     }
 }
 
-__global__ void k_pass_reset(uint32_t *sync, uint32_t words) {
+// Zeroes flags and counts (start stamps, the even u64 slots, get the maximum:
+// they are minima over workgroups), and thread 0 writes the band starts.
+__global__ void k_pass_reset(uint32_t *sync, uint32_t n_bands, uint32_t col_groups,
+                             uint32_t band_cols) {
+    const uint32_t words = kPassBands + n_bands;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < words; i += gridDim.x * blockDim.x)
-        // start stamps (even u64 slots) are minima over workgroups
         sync[i] = (i >= kPassTimes && i < kPassBands && ((i - kPassTimes) / 2) % 2 == 0)
                       ? 0xFFFFFFFFu
                       : 0u;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        uint32_t *bstart = sync + kPassBands + n_bands, c = 0;
+        for (uint32_t b = 0; b < n_bands; b++) {
+            bstart[b] = c;
+            c += next_band_width(c, col_groups - c, band_cols);
+        }
+        bstart[n_bands] = c;  // = col_groups
+    }
 }
 
 bool plan_create_pass(uint32_t n, uint32_t R, uint32_t S, uint32_t num_cus, uint32_t band_cols,
@@ -1205,12 +1344,14 @@ bool plan_create_pass(uint32_t n, uint32_t R, uint32_t S, uint32_t num_cus, uint
     p->row_blocks = mfma_row_tiles(R) / kMfTiles;
     p->col_groups = (S + kMfThreads * 2 - 1) / (kMfThreads * 2);
     p->band_cols = std::max<uint32_t>(1, std::min(band_cols ? band_cols : 64u, p->col_groups));
-    p->n_bands = (p->col_groups + p->band_cols - 1) / p->band_cols;
+    p->n_bands = 0;
+    for (uint32_t c = 0; c < p->col_groups; p->n_bands++)
+        c += next_band_width(c, p->col_groups - c, p->band_cols);
     p->n_units = p->row_blocks * p->col_groups;
     return true;
 }
 
-uint32_t pass_sync_words(const PassPlan &p) { return kPassBands + p.n_bands; }
+uint32_t pass_sync_words(const PassPlan &p) { return kPassBands + 2 * p.n_bands + 1; }
 
 hipError_t launch_create_pass(const PassPlan &p, const SliceDesc *d_inputs,
                               const uint4 *d_tail_blocks, uint32_t n,
@@ -1218,9 +1359,9 @@ hipError_t launch_create_pass(const PassPlan &p, const SliceDesc *d_inputs,
                               uint8_t *d_slice_md5, uint32_t *d_slice_crc, const uint8_t set_id[16],
                               uint8_t *d_pkt_md5, uint32_t *d_sync, bool all_full,
                               hipStream_t stream) {
-    const uint32_t words = pass_sync_words(p);
+    const uint32_t words = kPassBands + p.n_bands;
     hipLaunchKernelGGL(k_pass_reset, dim3(std::min<uint32_t>((words + 255) / 256, 64)), dim3(256),
-                       0, stream, d_sync, words);
+                       0, stream, d_sync, p.n_bands, p.col_groups, p.band_cols);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     PassArgs a;
