@@ -155,6 +155,58 @@ int p2g_dev_create_pass(p2g_ctx *ctx, const p2g_dev_file *files, uint32_t nfiles
  * role with no work (or when the pass ran as separate kernels). */
 int p2g_pass_timing(p2g_ctx *ctx, double *ms_out);
 
+/* ---------------------------------------------------------------------
+ * Batched create (no reference counterpart: the reference runs one
+ * run_create at a time, src/creator.rs:188-321).  Many independent sets,
+ * each exactly the work of one p2g_dev_create_pass, share persistent
+ * launches: one table of slices, A fragments and work items (recovery row
+ * block x 512-byte column group) over all jobs, so small jobs fill the
+ * device together and their serial hashes overlap.  For callers that
+ * create many sets at once (a posting service, a backup tool packaging
+ * many directories).
+ * ------------------------------------------------------------------- */
+
+/* One create job of a batch: what p2g_dev_create_pass takes. */
+typedef struct {
+    const p2g_dev_file *files; /* the set's files, device resident, 16-byte aligned */
+    uint64_t slice_size;
+    uint8_t *rec;              /* recovery_count * slice_size bytes, device */
+    uint8_t *slice_md5;        /* 16 B per slice, device */
+    uint32_t *slice_crc;       /* one per slice, device */
+    uint8_t *pkt_md5;          /* 16 B per recovery packet, device */
+    uint32_t nfiles;
+    uint32_t recovery_count;
+    uint8_t set_id[16];
+} p2g_batch_job;
+
+typedef struct p2g_batch p2g_batch;
+
+/* Plans a batch: validates every job (as p2g_dev_create_pass does, plus:
+ * a job needs at least one slice), builds and uploads the descriptor, piece
+ * and work tables and, with use_graph != 0, records the launch sequence as
+ * a HIP graph.  The files and outputs must stay allocated while the plan
+ * is launched; their contents may change between launches. */
+int p2g_batch_plan(p2g_ctx *ctx, const p2g_batch_job *jobs, uint32_t njobs, int use_graph,
+                   p2g_batch **out);
+/* Queues one run of the plan on `stream` (NULL: the context's stream).
+ * Runs of one plan are serialized. */
+int p2g_batch_launch(p2g_batch *b, void *stream);
+/* Waits for the last run; reports a run that could not complete. */
+int p2g_batch_wait(p2g_batch *b);
+/* After p2g_batch_wait: each job's completion time in ms from the start
+ * of the run (device clock), in the order of the jobs given to the plan. */
+int p2g_batch_job_ms(p2g_batch *b, double *ms_out);
+void p2g_batch_free(p2g_batch *b);
+/* Plan, queue and, at the next p2g_synchronize, free: one batched run. */
+int p2g_dev_create_batch(p2g_ctx *ctx, const p2g_batch_job *jobs, uint32_t njobs, void *stream);
+
+/* Synthetic input for benchmarks and tests (no reference counterpart):
+ * fills nbytes of dst (device, 16-byte aligned) with the xorshift64 stream
+ * from `seed` (x ^= x << 13; x ^= x >> 7; x ^= x << 17; each new state
+ * emitted as 8 little-endian bytes), the documented workload generator. */
+int p2g_dev_fill_xorshift64(p2g_ctx *ctx, uint8_t *dst, uint64_t nbytes, uint64_t seed,
+                            void *stream);
+
 /* Waits for all work queued by this context, on its own streams and on the
  * streams callers passed in. */
 int p2g_synchronize(p2g_ctx *ctx);

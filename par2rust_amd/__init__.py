@@ -41,6 +41,8 @@ EXPORTED_SYMBOLS = [
     "p2g_encode_free", "p2g_recovery_packet_md5", "p2g_dev_encode",
     "p2g_dev_slice_checksums", "p2g_dev_recovery_packet_md5", "p2g_dev_create_pass",
     "p2g_pass_timing", "p2g_synchronize", "p2g_run_create",
+    "p2g_batch_plan", "p2g_batch_launch", "p2g_batch_wait", "p2g_batch_job_ms",
+    "p2g_batch_free", "p2g_dev_create_batch", "p2g_dev_fill_xorshift64",
 ]
 
 
@@ -55,6 +57,21 @@ class Par2Error(RuntimeError):
 
 class _DevFile(ctypes.Structure):
     _fields_ = [("data", ctypes.c_void_p), ("len", ctypes.c_uint64)]
+
+
+class _BatchJob(ctypes.Structure):
+    """p2g_batch_job."""
+    _fields_ = [
+        ("files", ctypes.c_void_p),
+        ("slice_size", ctypes.c_uint64),
+        ("rec", ctypes.c_void_p),
+        ("slice_md5", ctypes.c_void_p),
+        ("slice_crc", ctypes.c_void_p),
+        ("pkt_md5", ctypes.c_void_p),
+        ("nfiles", ctypes.c_uint32),
+        ("recovery_count", ctypes.c_uint32),
+        ("set_id", ctypes.c_uint8 * 16),
+    ]
 
 
 class _Scheme(ctypes.Structure):
@@ -97,6 +114,13 @@ def _load() -> ctypes.CDLL:
                                                ctypes.c_uint32, P, P, P, P, P, P]),
         "p2g_pass_timing": (ctypes.c_int, [P, ctypes.POINTER(ctypes.c_double)]),
         "p2g_synchronize": (ctypes.c_int, [P]),
+        "p2g_batch_plan": (ctypes.c_int, [P, P, ctypes.c_uint32, ctypes.c_int, ctypes.POINTER(P)]),
+        "p2g_batch_launch": (ctypes.c_int, [P, P]),
+        "p2g_batch_wait": (ctypes.c_int, [P]),
+        "p2g_batch_job_ms": (ctypes.c_int, [P, ctypes.POINTER(ctypes.c_double)]),
+        "p2g_batch_free": (None, [P]),
+        "p2g_dev_create_batch": (ctypes.c_int, [P, P, ctypes.c_uint32, P]),
+        "p2g_dev_fill_xorshift64": (ctypes.c_int, [P, P, ctypes.c_uint64, ctypes.c_uint64, P]),
         "p2g_run_create": (ctypes.c_int, [P, ctypes.c_char_p, ctypes.c_uint64, ctypes.c_uint32,
                                           ctypes.POINTER(_Scheme), ctypes.c_uint32,
                                           ctypes.POINTER(ctypes.c_char_p),
@@ -195,6 +219,74 @@ def _dev_files(files):
         else:
             items.append(_DevFile(_dev_ptr(f), _dev_len(f)))
     return (_DevFile * max(1, len(items)))(*items), items
+
+
+@dataclass
+class BatchJob:
+    """One create job of a batch: the arguments of dev_create_pass.  Buffers
+    are DeviceBuffers, tensors or device addresses; files as in dev_encode."""
+
+    files: Sequence
+    slice_size: int
+    recovery_count: int
+    set_id: bytes
+    rec: object
+    slice_md5: object
+    slice_crc: object
+    pkt_md5: object
+
+
+def _batch_jobs(jobs: Sequence[BatchJob]):
+    arr = (_BatchJob * max(1, len(jobs)))()
+    keep = []
+    for k, j in enumerate(jobs):
+        files, items = _dev_files(j.files)
+        keep.append(files)
+        b = arr[k]
+        b.files = ctypes.cast(files, ctypes.c_void_p).value
+        b.nfiles = len(items)
+        b.slice_size = j.slice_size
+        b.recovery_count = j.recovery_count
+        b.set_id[:] = list(bytes(j.set_id))
+        b.rec = _dev_ptr(j.rec) if j.recovery_count else None
+        b.slice_md5 = _dev_ptr(j.slice_md5)
+        b.slice_crc = _dev_ptr(j.slice_crc)
+        b.pkt_md5 = _dev_ptr(j.pkt_md5) if j.recovery_count else None
+    return arr, keep
+
+
+class Batch:
+    """A planned batch of create jobs (p2g_batch_plan): launch it any number
+    of times, wait, read per-job completion times."""
+
+    def __init__(self, ctx: "Context", jobs: Sequence[BatchJob], use_graph: bool = True):
+        self.ctx, self.njobs = ctx, len(jobs)
+        arr, self._keep = _batch_jobs(jobs)
+        h = ctypes.c_void_p()
+        ctx._check(lib.p2g_batch_plan(ctx.h, arr, len(jobs), int(use_graph), ctypes.byref(h)))
+        self.h = h
+
+    def launch(self, stream: Optional[int] = None):
+        self.ctx._check(lib.p2g_batch_launch(self.h, ctypes.c_void_p(stream) if stream else None))
+
+    def wait(self):
+        self.ctx._check(lib.p2g_batch_wait(self.h))
+
+    def job_ms(self) -> List[float]:
+        out = (ctypes.c_double * max(1, self.njobs))()
+        self.ctx._check(lib.p2g_batch_job_ms(self.h, out))
+        return list(out[:self.njobs])
+
+    def free(self):
+        if getattr(self, "h", None):
+            lib.p2g_batch_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 class Context:
@@ -321,6 +413,21 @@ class Context:
                                             ctypes.c_void_p(_dev_ptr(slice_crc_out)),
                                             ctypes.c_void_p(_dev_ptr(pkt_md5_out)),
                                             ctypes.c_void_p(stream) if stream else None))
+
+    def dev_create_batch(self, jobs: Sequence[BatchJob], stream: Optional[int] = None):
+        """Every job's dev_create_pass work in shared persistent launches
+        (p2g_dev_create_batch); synchronize() waits for it."""
+        arr, keep = _batch_jobs(jobs)
+        self._check(lib.p2g_dev_create_batch(self.h, arr, len(jobs),
+                                             ctypes.c_void_p(stream) if stream else None))
+
+    def dev_fill_xorshift64(self, dst, nbytes: int, seed: int, stream: Optional[int] = None):
+        """The synthetic workload stream (par2rust_amd.synth) written on the device."""
+        self._check(lib.p2g_dev_fill_xorshift64(self.h, ctypes.c_void_p(_dev_ptr(dst)), nbytes,
+                                                seed, ctypes.c_void_p(stream) if stream else None))
+
+    def batch_plan(self, jobs: Sequence[BatchJob], use_graph: bool = True) -> Batch:
+        return Batch(self, jobs, use_graph)
 
     def pass_timing(self) -> dict:
         """(start, end) in ms of each role of the last fused pass."""

@@ -16,7 +16,8 @@ BUILD = os.path.join(HERE, "_build")
 OUT = os.path.join(HERE, "libpar2gpu.so")
 
 # (source, language)
-SOURCES = [("p2g_kernels.hip", "hip"), ("p2g_abi.hip", "hip"), ("par2_create.cpp", "hip")]
+SOURCES = [("p2g_kernels.hip", "hip"), ("p2g_abi.hip", "hip"), ("p2g_batch.cpp", "hip"),
+           ("par2_create.cpp", "hip")]
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-result", "-Wno-unused-value"]
 HIP_FLAGS = [

@@ -131,6 +131,7 @@ Ctx::~Ctx() {
     hipSetDevice(device);
     if (stream) hipStreamSynchronize(stream);
     if (side) hipStreamSynchronize(side);
+    for (BatchPlan *b : pending_batches) free_batch_plan(b);
     d_antilog.release();
     d_base_logs.release();
     d_mats.release();
@@ -139,6 +140,7 @@ Ctx::~Ctx() {
     d_stage.release();
     d_sync.release();
     d_partials.release();
+    d_xs_jumps.release();
     for (hipEvent_t ev : scratch_done)
         if (ev) hipEventDestroy(ev);
     for (DescSlot *sl : {&enc_descs, &sum_descs}) {
@@ -306,6 +308,55 @@ const char *p2g_last_error(const p2g_ctx *ctx) {
     return ctx ? reinterpret_cast<const Ctx *>(ctx)->err.c_str() : "null context";
 }
 
+// GF(2) 64x64 matrices as 64 column images.
+static uint64_t xs_apply_host(const uint64_t *m, uint64_t v) {
+    uint64_t out = 0;
+    for (int b = 0; b < 64; b++)
+        if ((v >> b) & 1) out ^= m[b];
+    return out;
+}
+
+int p2g_dev_fill_xorshift64(p2g_ctx *ctx, uint8_t *dst, uint64_t nbytes, uint64_t seed,
+                            void *stream) {
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (!c) return P2G_ERR_INVALID_ARGUMENT;
+    if (nbytes == 0) return P2G_OK;
+    if (!dst || reinterpret_cast<uintptr_t>(dst) % 16 != 0)
+        return c->fail(P2G_ERR_INVALID_ARGUMENT, "destination must be 16-byte aligned");
+    hipSetDevice(c->device);
+    if (!c->d_xs_jumps.ptr) {
+        // step^(W 2^b) for b = 0..31, W = kXorshiftLaneWords
+        std::vector<uint64_t> step(64), m(64), t(64), jumps(64 * 32);
+        for (int b = 0; b < 64; b++) {
+            uint64_t x = 1ull << b;
+            x ^= x << 13;
+            x ^= x >> 7;
+            x ^= x << 17;
+            step[b] = x;
+        }
+        m = step;  // step^(2^k), squared until 2^k = W
+        for (uint32_t w = 1; w < kXorshiftLaneWords; w *= 2) {
+            for (int b = 0; b < 64; b++) t[b] = xs_apply_host(m.data(), m[b]);
+            m = t;
+        }
+        for (int k = 0; k < 32; k++) {
+            std::copy(m.begin(), m.end(), jumps.begin() + 64 * k);
+            for (int b = 0; b < 64; b++) t[b] = xs_apply_host(m.data(), m[b]);
+            m = t;
+        }
+        if (c->d_xs_jumps.reserve(jumps.size() * 8) != P2G_OK)
+            return c->fail(P2G_ERR_OUT_OF_MEMORY, "xorshift jump table");
+        hipError_t e = hipMemcpy(c->d_xs_jumps.ptr, jumps.data(), jumps.size() * 8,
+                                 hipMemcpyHostToDevice);
+        if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "upload xorshift jumps", e);
+    }
+    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c->stream;
+    hipError_t e = launch_xorshift_fill(dst, nbytes, seed,
+                                        static_cast<const uint64_t *>(c->d_xs_jumps.ptr), st);
+    if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "xorshift fill kernel", e);
+    return P2G_OK;
+}
+
 int p2g_synchronize(p2g_ctx *ctx) {
     Ctx *c = reinterpret_cast<Ctx *>(ctx);
     if (!c) return P2G_ERR_INVALID_ARGUMENT;
@@ -315,6 +366,14 @@ int p2g_synchronize(p2g_ctx *ctx) {
     for (hipEvent_t ev : c->scratch_done)  // work queued on callers' streams
         if (e == hipSuccess) e = hipEventSynchronize(ev);
     if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "synchronize", e);
+    int brc = P2G_OK;
+    for (BatchPlan *b : c->pending_batches) {
+        const int rc = wait_plan(b);
+        if (brc == P2G_OK) brc = rc;
+        free_batch_plan(b);
+    }
+    c->pending_batches.clear();
+    if (brc != P2G_OK) return brc;
     if (c->pass_pending) {
         c->pass_pending = false;
         uint32_t flags = 0;

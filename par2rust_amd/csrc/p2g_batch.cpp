@@ -1,0 +1,423 @@
+// p2g_batch.cpp -- batched create: many independent create jobs (each the work
+// of p2g_dev_create_pass) in persistent launches that share the GPU.
+//
+// Planning (host): every job's slice descriptors go into one table, ordered by
+// job (longest slices first, so the longest serial hashes start first; then
+// most inputs first).  Partial slices get zero-padded copies, so every
+// descriptor is a whole slice.  Each job's recovery rows are cut into pieces
+// whose A fragments fit the group budget, and pieces are packed in order
+// into groups of at most kBatchAfragBudget bytes of fragments; a group is
+// one k_create_batch launch after one k_batch_afrag launch.  A job's slice
+// checksums run in the group of its first piece.
+//
+// The launch sequence (staging copies, descriptor tails, then per group the
+// fragments and the pass) is recorded once into a HIP graph and replayed by
+// p2g_batch_launch; a plan made with use_graph = 0 issues the launches
+// directly each time.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#include "p2g_context.hpp"
+
+namespace p2g {
+
+// A fragments of one group (P2G_BATCH_AFRAG_BYTES overrides it); one row
+// block of a job is the smallest piece, whatever the budget.
+constexpr uint64_t kBatchAfragBudget = 8ull << 30;
+
+struct BatchPlan {
+    Ctx *c = nullptr;
+    uint32_t njobs = 0;
+    uint32_t n_slices = 0, n_parts = 0, max_part_S = 0;
+    DevBuf d_descs, d_tb, d_staging, d_parts, d_pieces, d_sjobs, d_sync, d_afrag, d_done;
+    std::vector<BatchGroupLaunch> groups;
+    std::vector<uint32_t> sync_off;  // word offset of each group's sync words
+    hipStream_t cap = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t done = nullptr;
+    bool launched = false;
+    std::vector<unsigned long long> h_done;  // completion stamps of the last launch, then start
+
+    int record(hipStream_t st);
+    ~BatchPlan();
+};
+
+BatchPlan::~BatchPlan() {
+    if (c) hipSetDevice(c->device);
+    if (done) hipEventSynchronize(done);
+    if (exec) hipGraphExecDestroy(exec);
+    if (graph) hipGraphDestroy(graph);
+    if (cap) hipStreamDestroy(cap);
+    if (done) hipEventDestroy(done);
+    for (DevBuf *b : {&d_descs, &d_tb, &d_staging, &d_parts, &d_pieces, &d_sjobs, &d_sync,
+                      &d_afrag, &d_done})
+        b->release();
+}
+
+int BatchPlan::record(hipStream_t st) {
+    hipError_t e = launch_stage_parts(static_cast<const StagePart *>(d_parts.ptr), n_parts,
+                                      max_part_S, st);
+    if (e == hipSuccess)
+        e = launch_desc_tails(static_cast<SliceDesc *>(d_descs.ptr), n_slices,
+                              static_cast<uint4 *>(d_tb.ptr), st);
+    for (const BatchGroupLaunch &g : groups) {
+        if (e == hipSuccess)
+            e = launch_batch_afrag(static_cast<const uint16_t *>(c->d_antilog.ptr),
+                                   static_cast<const uint16_t *>(c->d_base_logs.ptr), g, st);
+        if (e == hipSuccess)
+            e = launch_create_batch(g, static_cast<const SliceDesc *>(d_descs.ptr),
+                                    static_cast<const uint4 *>(d_tb.ptr), st);
+    }
+    if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "batched create launches", e);
+    return P2G_OK;
+}
+
+void free_batch_plan(BatchPlan *b) { delete b; }
+
+static void packet_prefix_words(const uint8_t set_id[16], uint32_t out[8]) {
+    static const uint8_t type_tag[16] = {'P', 'A', 'R', ' ', '2', '.', '0', 0,
+                                         'R', 'e', 'c', 'v', 'S', 'l', 'i', 'c'};
+    for (int w = 0; w < 4; w++) {
+        memcpy(&out[w], set_id + 4 * w, 4);  // little-endian host, as the device
+        memcpy(&out[4 + w], type_tag + 4 * w, 4);
+    }
+}
+
+static uint64_t align_up(uint64_t x, uint64_t a) { return (x + a - 1) / a * a; }
+
+template <typename T>
+static int upload(Ctx *c, DevBuf &d, const std::vector<T> &v, const char *what) {
+    if (d.reserve(std::max<size_t>(v.size() * sizeof(T), 16)) != P2G_OK)
+        return c->fail(P2G_ERR_OUT_OF_MEMORY, what);
+    if (v.empty()) return P2G_OK;
+    hipError_t e = hipMemcpy(d.ptr, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, what, e);
+    return P2G_OK;
+}
+
+static int make_plan(Ctx *c, const p2g_batch_job *jobs, uint32_t njobs, bool use_graph,
+                     BatchPlan **out) {
+    *out = nullptr;
+    if (njobs == 0 || !jobs) return c->fail(P2G_ERR_INVALID_ARGUMENT, "empty batch");
+    hipSetDevice(c->device);
+    std::vector<std::vector<SliceDesc>> jd(njobs);
+    for (uint32_t j = 0; j < njobs; j++) {
+        const p2g_batch_job &J = jobs[j];
+        const uint32_t R = J.recovery_count;
+        if ((J.nfiles && !J.files) || !J.slice_md5 || !J.slice_crc || (R && (!J.rec || !J.pkt_md5)))
+            return c->fail(P2G_ERR_INVALID_ARGUMENT, "null argument in a batch job");
+        for (const void *p : {(const void *)J.rec, (const void *)J.slice_md5,
+                              (const void *)J.slice_crc, (const void *)J.pkt_md5})
+            if (reinterpret_cast<uintptr_t>(p) % 4 != 0)
+                return c->fail(P2G_ERR_INVALID_ARGUMENT, "batch outputs must be 4-byte aligned");
+        if (R > P2G_MAX_RECOVERY_BLOCKS)
+            return c->fail(P2G_ERR_TOO_MANY_RECOVERY_BLOCKS, "recovery count above 65535");
+        int rc = check_slice_size(c, J.slice_size);
+        if (rc != P2G_OK) return rc;
+        rc = append_descs(c, J.files, J.nfiles, J.slice_size, jd[j]);
+        if (rc != P2G_OK) return rc;
+        if (jd[j].empty()) return c->fail(P2G_ERR_INVALID_ARGUMENT, "a batch job has no slices");
+        if (jd[j].size() > kMaxInputs)
+            return c->fail(P2G_ERR_TOO_MANY_INPUT_BLOCKS, "more than 32768 input slices");
+    }
+    std::vector<uint32_t> order(njobs);
+    std::iota(order.begin(), order.end(), 0u);
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+        if (jobs[a].slice_size != jobs[b].slice_size) return jobs[a].slice_size > jobs[b].slice_size;
+        return jd[a].size() > jd[b].size();
+    });
+
+    BatchPlan *B = new (std::nothrow) BatchPlan();
+    if (!B) return c->fail(P2G_ERR_OUT_OF_MEMORY, "batch plan");
+    B->c = c;
+    B->njobs = njobs;
+    auto bail = [&](int rc) {
+        delete B;
+        return rc;
+    };
+
+    // staging slots of the partial slices
+    uint64_t stage_bytes = 0, n_slices = 0;
+    for (uint32_t j : order) {
+        const uint32_t S = (uint32_t)jobs[j].slice_size;
+        for (const SliceDesc &d : jd[j])
+            if (d.valid < S) stage_bytes = align_up(stage_bytes, 256) + S;
+        n_slices += jd[j].size();
+    }
+    if (n_slices > 0x7FFFFFFFull) return bail(c->fail(P2G_ERR_INVALID_ARGUMENT, "too many slices"));
+    if (B->d_staging.reserve(std::max<uint64_t>(stage_bytes, 256)) != P2G_OK ||
+        B->d_descs.reserve(n_slices * sizeof(SliceDesc)) != P2G_OK ||
+        B->d_tb.reserve(n_slices * 64) != P2G_OK)
+        return bail(c->fail(P2G_ERR_OUT_OF_MEMORY, "batch descriptor tables"));
+    uint8_t *staging = static_cast<uint8_t *>(B->d_staging.ptr);
+    const SliceDesc *d_descs = static_cast<const SliceDesc *>(B->d_descs.ptr);
+
+    std::vector<SliceDesc> all;
+    all.reserve(n_slices);
+    std::vector<StagePart> parts;
+    std::vector<BatchSlices> sjobs;
+    std::vector<uint32_t> slice0(njobs);
+    uint64_t so = 0;
+    for (uint32_t j : order) {
+        const uint32_t S = (uint32_t)jobs[j].slice_size;
+        slice0[j] = (uint32_t)all.size();
+        for (const SliceDesc &d : jd[j]) {
+            if (d.valid < S) {
+                so = align_up(so, 256);
+                parts.push_back({d.ptr, staging + so, d.valid, S, 0});
+                all.push_back({staging + so, S, 0});
+                B->max_part_S = std::max(B->max_part_S, S);
+                so += S;
+            } else {
+                all.push_back(d);
+            }
+        }
+        sjobs.push_back({jobs[j].slice_md5, jobs[j].slice_crc, slice0[j], (uint32_t)all.size(), S, j});
+    }
+    B->n_slices = (uint32_t)n_slices;
+    B->n_parts = (uint32_t)parts.size();
+
+    // pieces and groups
+    const uint32_t band_default = c->band_cols ? c->band_cols : 64u;
+    uint64_t budget = kBatchAfragBudget;
+    if (const char *v = getenv("P2G_BATCH_AFRAG_BYTES")) budget = std::max<uint64_t>(1, strtoull(v, nullptr, 10));
+    std::vector<BatchPiece> pieces;
+    std::vector<uint64_t> afrag_off;  // of each piece inside its group's fragments
+    struct GroupSpan {
+        uint32_t p_lo, p_hi, s_lo, s_hi;
+        uint64_t afrag;
+    };
+    std::vector<GroupSpan> spans(1, GroupSpan{0, 0, 0, 0, 0});
+    std::vector<uint32_t> job_group(njobs);  // by position in `order`; non-decreasing
+    uint64_t afrag_max = 0;
+    for (uint32_t oi = 0; oi < njobs; oi++) {
+        const uint32_t j = order[oi];
+        const p2g_batch_job &J = jobs[j];
+        const uint32_t S = (uint32_t)J.slice_size, n = (uint32_t)jd[j].size();
+        const uint32_t R = J.recovery_count, nq = mfma_steps(n);
+        const uint64_t per_rb = mfma_afrag_bytes(n, 16);  // fragments of 16 recovery rows
+        const uint32_t max_rb = (uint32_t)std::max<uint64_t>(1, budget / per_rb);
+        job_group[oi] = (uint32_t)spans.size() - 1;
+        for (uint32_t r0 = 0; r0 < R; r0 += 16 * max_rb) {
+            const uint32_t Rp = std::min<uint32_t>(16 * max_rb, R - r0);
+            const uint64_t bytes = mfma_afrag_bytes(n, Rp);
+            if (spans.back().p_hi > spans.back().p_lo && spans.back().afrag + bytes > budget) {
+                const uint32_t np = (uint32_t)pieces.size();
+                spans.push_back({np, np, 0, 0, 0});
+            }
+            if (r0 == 0) job_group[oi] = (uint32_t)spans.size() - 1;
+            BatchPiece P{};
+            P.inputs = d_descs + slice0[j];
+            P.rec = J.rec + (size_t)r0 * S;
+            P.pkt_md5 = J.pkt_md5 + (size_t)r0 * 16;
+            packet_prefix_words(J.set_id, P.pre);
+            P.n = n;
+            P.nq = nq;
+            P.R = Rp;
+            P.S = S;
+            P.first_exp = r0;
+            P.job = j;
+            P.row_blocks = (Rp + 15) / 16;
+            P.col_groups = (S + 511) / 512;
+            P.band_cols = std::max<uint32_t>(1, std::min(band_default, P.col_groups));
+            P.n_bands = 0;
+            for (uint32_t cg = 0; cg < P.col_groups; P.n_bands++)
+                cg += next_band_width(cg, P.col_groups - cg, P.band_cols);
+            pieces.push_back(P);
+            afrag_off.push_back(spans.back().afrag);
+            spans.back().p_hi = (uint32_t)pieces.size();
+            spans.back().afrag += bytes;
+            afrag_max = std::max(afrag_max, spans.back().afrag);
+        }
+    }
+    // the slice jobs of group k: the jobs whose first piece (or, with no
+    // recovery rows, whose place in the order) falls in it
+    for (uint32_t k = 0; k < spans.size(); k++) {
+        spans[k].s_lo = (uint32_t)(std::lower_bound(job_group.begin(), job_group.end(), k) - job_group.begin());
+        spans[k].s_hi = (uint32_t)(std::upper_bound(job_group.begin(), job_group.end(), k) - job_group.begin());
+    }
+
+    if (B->d_afrag.reserve(std::max<uint64_t>(afrag_max, 256)) != P2G_OK)
+        return bail(c->fail(P2G_ERR_OUT_OF_MEMORY, "batch A fragments"));
+    uint8_t *afrag_base = static_cast<uint8_t *>(B->d_afrag.ptr);
+
+    // per group: units, tasks, sync layout
+    uint32_t sync_words_total = 0;
+    std::vector<BatchGroupLaunch> gl(spans.size());
+    for (size_t k = 0; k < spans.size(); k++) {
+        const GroupSpan &sp = spans[k];
+        BatchGroupLaunch &g = gl[k];
+        g = BatchGroupLaunch{};
+        uint32_t units = 0, tasks = 0, bands = 0;
+        for (uint32_t p = sp.p_lo; p < sp.p_hi; p++) {
+            BatchPiece &P = pieces[p];
+            P.afrag = afrag_base + afrag_off[p];
+            P.unit0 = units;
+            units += P.row_blocks * P.col_groups;
+            P.unit_end = units;
+            P.task0 = tasks;
+            tasks += (P.R + 63) / 64;
+            P.task_end = tasks;
+            P.band0 = kPassBands + bands;
+            bands += P.n_bands;
+        }
+        uint32_t starts = kPassBands + bands;
+        for (uint32_t p = sp.p_lo; p < sp.p_hi; p++) {
+            pieces[p].bstart0 = starts;
+            starts += pieces[p].n_bands + 1;
+        }
+        g.n_pieces = sp.p_hi - sp.p_lo;
+        g.n_sjobs = sp.s_hi - sp.s_lo;
+        g.slice_lo = sp.s_lo < sp.s_hi ? sjobs[sp.s_lo].slice0 : 0;
+        g.slice_hi = sp.s_lo < sp.s_hi ? sjobs[sp.s_hi - 1].slice_end : 0;
+        g.n_units = units;
+        g.n_tasks = tasks;
+        g.sync_words = kPassBands + bands;
+        g.n_jobs = njobs;
+        g.first = k == 0;
+        g.afrag_entries = sp.afrag / 16;
+        plan_batch_group(c->num_cus, tasks, g.slice_hi - g.slice_lo, units, &g);
+        B->sync_off.push_back(sync_words_total);
+        sync_words_total += (uint32_t)align_up(starts, 64);
+    }
+    int rc = upload(c, B->d_descs, all, "batch descriptors");
+    if (rc == P2G_OK) rc = upload(c, B->d_parts, parts, "batch staging table");
+    if (rc == P2G_OK) rc = upload(c, B->d_pieces, pieces, "batch pieces");
+    if (rc == P2G_OK) rc = upload(c, B->d_sjobs, sjobs, "batch slice jobs");
+    if (rc != P2G_OK) return bail(rc);
+    if (B->d_sync.reserve((size_t)sync_words_total * 4) != P2G_OK ||
+        B->d_done.reserve((size_t)(njobs + 1) * 8) != P2G_OK)
+        return bail(c->fail(P2G_ERR_OUT_OF_MEMORY, "batch sync words"));
+    for (size_t k = 0; k < gl.size(); k++) {
+        gl[k].pieces = static_cast<const BatchPiece *>(B->d_pieces.ptr) + spans[k].p_lo;
+        gl[k].sjobs = static_cast<const BatchSlices *>(B->d_sjobs.ptr) + spans[k].s_lo;
+        gl[k].sync = static_cast<uint32_t *>(B->d_sync.ptr) + B->sync_off[k];
+        gl[k].job_done = static_cast<unsigned long long *>(B->d_done.ptr);
+    }
+    B->groups = gl;
+    hipError_t e = hipEventCreateWithFlags(&B->done, hipEventDisableTiming);
+    if (e != hipSuccess) return bail(c->fail(P2G_ERR_DEVICE, "hipEventCreate", e));
+    if (use_graph) {
+        e = hipStreamCreateWithFlags(&B->cap, hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipStreamBeginCapture(B->cap, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess) return bail(c->fail(P2G_ERR_DEVICE, "begin graph capture", e));
+        rc = B->record(B->cap);
+        hipGraph_t graph = nullptr;
+        e = hipStreamEndCapture(B->cap, &graph);
+        B->graph = graph;
+        if (rc != P2G_OK) return bail(rc);
+        if (e == hipSuccess) e = hipGraphInstantiate(&B->exec, B->graph, nullptr, nullptr, 0);
+        if (e != hipSuccess) return bail(c->fail(P2G_ERR_DEVICE, "batch graph", e));
+    }
+    *out = B;
+    return P2G_OK;
+}
+
+int launch_plan(BatchPlan *B, hipStream_t st) {
+    Ctx *c = B->c;
+    hipSetDevice(c->device);
+    // launches of one plan share its tables: each waits for the one before
+    hipError_t e = hipStreamWaitEvent(st, B->done, 0);
+    if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "stream wait", e);
+    if (B->exec) {
+        e = hipGraphLaunch(B->exec, st);
+        if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "batch graph launch", e);
+    } else {
+        int rc = B->record(st);
+        if (rc != P2G_OK) return rc;
+    }
+    e = hipEventRecord(B->done, st);
+    if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "event record", e);
+    B->launched = true;
+    return P2G_OK;
+}
+
+int wait_plan(BatchPlan *B) {
+    Ctx *c = B->c;
+    hipSetDevice(c->device);
+    if (!B->launched) return P2G_OK;
+    hipError_t e = hipEventSynchronize(B->done);
+    if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "batch wait", e);
+    B->launched = false;
+    uint32_t bad = 0;
+    for (size_t k = 0; k < B->groups.size(); k++) {
+        uint32_t flags = 0;
+        e = hipMemcpy(&flags, static_cast<uint32_t *>(B->d_sync.ptr) + B->sync_off[k] + kPassFlags,
+                      4, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "read batch flags", e);
+        bad |= flags;
+    }
+    B->h_done.resize(B->njobs + 1);
+    e = hipMemcpy(B->h_done.data(), B->d_done.ptr, (B->njobs + 1) * 8, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "read batch stamps", e);
+    if (bad & kPassErrWait)
+        return c->fail(P2G_ERR_DEVICE,
+                       "batched create: a packet hash stopped waiting for the encode (no progress for 2 s)");
+    return P2G_OK;
+}
+
+}  // namespace p2g
+
+using namespace p2g;
+
+extern "C" {
+
+int p2g_batch_plan(p2g_ctx *ctx, const p2g_batch_job *jobs, uint32_t njobs, int use_graph,
+                   p2g_batch **out) {
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (!c || !out) return P2G_ERR_INVALID_ARGUMENT;
+    if (const char *v = getenv("P2G_BATCH_GRAPH")) use_graph = atoi(v);
+    BatchPlan *B = nullptr;
+    int rc = make_plan(c, jobs, njobs, use_graph != 0, &B);
+    *out = reinterpret_cast<p2g_batch *>(B);
+    return rc;
+}
+
+int p2g_batch_launch(p2g_batch *b, void *stream) {
+    BatchPlan *B = reinterpret_cast<BatchPlan *>(b);
+    if (!B) return P2G_ERR_INVALID_ARGUMENT;
+    return launch_plan(B, stream ? static_cast<hipStream_t>(stream) : B->c->stream);
+}
+
+int p2g_batch_wait(p2g_batch *b) {
+    BatchPlan *B = reinterpret_cast<BatchPlan *>(b);
+    if (!B) return P2G_ERR_INVALID_ARGUMENT;
+    return wait_plan(B);
+}
+
+int p2g_batch_job_ms(p2g_batch *b, double *ms_out) {
+    BatchPlan *B = reinterpret_cast<BatchPlan *>(b);
+    if (!B || !ms_out) return P2G_ERR_INVALID_ARGUMENT;
+    if (B->h_done.size() != B->njobs + 1)
+        return B->c->fail(P2G_ERR_INVALID_ARGUMENT, "no finished batch launch to report");
+    const unsigned long long t0 = B->h_done[B->njobs];
+    for (uint32_t j = 0; j < B->njobs; j++)
+        ms_out[j] = B->h_done[j] >= t0 ? (double)(B->h_done[j] - t0) * 1e-5 : 0.0;
+    return P2G_OK;
+}
+
+void p2g_batch_free(p2g_batch *b) { delete reinterpret_cast<BatchPlan *>(b); }
+
+int p2g_dev_create_batch(p2g_ctx *ctx, const p2g_batch_job *jobs, uint32_t njobs, void *stream) {
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (!c) return P2G_ERR_INVALID_ARGUMENT;
+    BatchPlan *B = nullptr;
+    int rc = make_plan(c, jobs, njobs, false, &B);
+    if (rc != P2G_OK) return rc;
+    rc = launch_plan(B, stream ? static_cast<hipStream_t>(stream) : c->stream);
+    if (rc != P2G_OK) {
+        delete B;
+        return rc;
+    }
+    c->pending_batches.push_back(B);  // p2g_synchronize waits, checks and frees it
+    return P2G_OK;
+}
+
+}  // extern "C"

@@ -46,6 +46,8 @@ struct DescSlot {
 // bit-sliced VALU kernel.  P2G_ENCODE=bitslice|mfma picks one at p2g_open.
 enum { kEncodeMfma = 0, kEncodeBitslice = 1 };
 
+struct BatchPlan;  // p2g_batch.cpp
+
 struct Ctx {
     int device = 0;
     int encode_variant = kEncodeMfma;
@@ -62,6 +64,7 @@ struct Ctx {
     DevBuf d_mats, d_md5, d_crc, d_stage;
     DevBuf d_sync;                 // fused-pass error flags, stamps and band counts
     DevBuf d_partials;             // zero-padded copies of partial slices (fused pass)
+    DevBuf d_xs_jumps;             // xorshift64 jump matrices (synthetic input)
     bool pass_pending = false;     // a fused pass may have set error flags not yet read
     // Shared device buffers and the event recorded after the last kernel that
     // reads them; an entry point that rewrites them first makes its stream
@@ -71,6 +74,7 @@ struct Ctx {
     hipEvent_t scratch_done[2] = {nullptr, nullptr};
     DescSlot enc_descs;  // encode launches
     DescSlot sum_descs;  // slice checksum launches
+    std::vector<BatchPlan *> pending_batches;  // p2g_dev_create_batch, until p2g_synchronize
 
     int open(int dev);
     int side_stream(hipStream_t *out);
@@ -96,6 +100,11 @@ struct Job {
 };
 
 int check_slice_size(Ctx *c, uint64_t S);
+
+// Batched create plans (p2g_batch.cpp): wait for the last launch and read its
+// flags and stamps; free.
+int wait_plan(BatchPlan *b);
+void free_batch_plan(BatchPlan *b);
 
 // Appends the descriptors of every slice of `files` (file order, slice order).
 int append_descs(Ctx *c, const p2g_dev_file *files, uint32_t nfiles, uint64_t S,

@@ -50,14 +50,15 @@ hipError_t launch_desc_tails(SliceDesc *d_descs, uint32_t n, uint4 *d_tail_block
 // d_logbase[0]) and R recovery rows: mfma_row_tiles(R) * mfma_steps(n) * 1 KiB.
 // A K step covers four inputs; the step count is padded to whole groups of
 // four steps with zero fragments (zero A contributes nothing).
-inline uint32_t mfma_row_tiles(uint32_t R) { return (R + 15) / 16 * 8; }
-inline uint32_t mfma_steps(uint32_t n) { return ((n + 3) / 4 + 3) / 4 * 4; }
+__host__ __device__ inline uint32_t mfma_row_tiles(uint32_t R) { return (R + 15) / 16 * 8; }
+__host__ __device__ inline uint32_t mfma_steps(uint32_t n) { return ((n + 3) / 4 + 3) / 4 * 4; }
 constexpr uint32_t kMfmaMaxInputsPerLaunch = 2048;  // LDS descriptor table of k_encode_mfma
 inline size_t mfma_afrag_bytes(uint32_t n, uint32_t R) {
     return (size_t)mfma_row_tiles(R) * mfma_steps(n) * 1024;
 }
 hipError_t launch_mfma_afrag(const uint16_t *d_antilog, const uint16_t *d_logbase, uint32_t n,
                              uint32_t R, uint4 *d_afrag, hipStream_t stream);
+constexpr uint32_t kMfmaChunkInputs = 2048;  // inputs per K chunk of a batched unit
 hipError_t launch_encode_mfma(const SliceDesc *d_inputs, uint32_t n_inputs, const uint4 *d_afrag,
                               uint32_t R, uint32_t S, uint8_t *d_rec, int accumulate,
                               hipStream_t stream);
@@ -101,6 +102,84 @@ hipError_t launch_create_pass(const PassPlan &p, const SliceDesc *d_inputs,
                               uint8_t *d_slice_md5, uint32_t *d_slice_crc, const uint8_t set_id[16],
                               uint8_t *d_pkt_md5, uint32_t *d_sync, bool all_full,
                               hipStream_t stream);
+// ---- batched create pass ------------------------------------------------
+//
+// Many create jobs in one persistent launch per group of jobs.  A job's
+// recovery rows may be split into pieces (16-row multiples) so that the A
+// fragments of a group stay within a budget; every piece has its own
+// fragments, units, band counters and packet tasks.  Tables live in device
+// memory and are read by the kernel.
+
+// One piece: rows [first_exp, first_exp + R) of a job.  Units of the group
+// [unit0, unit_end); packet wave tasks [task0, task_end) (64 packets each);
+// band counters at sync[band0 + b], band starts at sync[bstart0 + b].
+struct BatchPiece {
+    const SliceDesc *inputs;  // the job's n descriptors, all whole slices
+    const void *afrag;        // fragments of these rows (uint4), mfma_steps(n) steps
+    uint8_t *rec;             // block of row 0 of the piece
+    uint8_t *pkt_md5;         // digest of row 0 of the piece
+    uint32_t pre[8];          // set id and packet type words of the packet hash
+    uint32_t n, nq, R, S;
+    uint32_t first_exp, job, row_blocks, col_groups;
+    uint32_t band_cols, n_bands, unit0, unit_end;
+    uint32_t task0, task_end, band0, bstart0;
+};
+static_assert(sizeof(BatchPiece) == 128, "BatchPiece layout");
+
+// One job's slices: global slice indices [slice0, slice_end) of the batch
+// descriptor table.
+struct BatchSlices {
+    uint8_t *md5;
+    uint32_t *crc;
+    uint32_t slice0, slice_end, S, job;
+};
+static_assert(sizeof(BatchSlices) == 32, "BatchSlices layout");
+
+// A partial slice copied into a zero-padded S-byte slot.
+struct StagePart {
+    const uint8_t *src;
+    uint8_t *dst;
+    uint32_t valid, S;
+    uint64_t pad;
+};
+
+struct BatchGroupLaunch {
+    const BatchPiece *pieces;   // the group's pieces (device)
+    uint32_t n_pieces;
+    const BatchSlices *sjobs;   // the group's slice jobs (device)
+    uint32_t n_sjobs;
+    uint32_t slice_lo, slice_hi;  // global slice range of the group
+    uint32_t n_units, n_tasks;
+    uint32_t h2_blocks, h3_blocks, grid;
+    uint32_t *sync;             // group sync words
+    uint32_t sync_words;        // flags, times, counters (zeroed by the reset)
+    uint32_t n_jobs;            // whole batch
+    unsigned long long *job_done;  // n_jobs completion stamps, then the batch start
+    bool first;                 // resets job_done too
+    uint64_t afrag_entries;     // uint4 entries of the group's fragments
+};
+
+// Zero-padded copies of partial slices (one launch for all of them).
+hipError_t launch_stage_parts(const StagePart *d_parts, uint32_t nparts, uint32_t max_S,
+                              hipStream_t stream);
+// A fragments of every piece of a group (pieces[k].afrag), exponent first_exp + r.
+hipError_t launch_batch_afrag(const uint16_t *d_antilog, const uint16_t *d_logbase,
+                              const BatchGroupLaunch &g, hipStream_t stream);
+// Sync reset, band starts, then the persistent launch of one group.
+hipError_t launch_create_batch(const BatchGroupLaunch &g, const SliceDesc *d_descs,
+                               const uint4 *d_tail_blocks, hipStream_t stream);
+// Group sizing: MD5 workgroups for n_tasks packet tasks and n_slices slices.
+void plan_batch_group(uint32_t num_cus, uint32_t n_tasks, uint32_t n_slices, uint32_t n_units,
+                      BatchGroupLaunch *g);
+constexpr uint32_t kBatchProgress = 1;  // sync word: units published so far
+
+// Synthetic input: the xorshift64 stream from `seed` into d_dst.  d_jumps
+// holds 32 GF(2) matrices of 64 columns, step^(kXorshiftLaneWords 2^b); the
+// destination must be 16-byte aligned.
+constexpr uint32_t kXorshiftLaneWords = 2048;
+hipError_t launch_xorshift_fill(uint8_t *d_dst, uint64_t nbytes, uint64_t seed,
+                                const uint64_t *d_jumps, hipStream_t stream);
+
 // Copies the nparts partial slices described by d_parts (ptr, valid, and in
 // `tail` the staging slot) into zero-padded S-byte slots of d_staging.
 hipError_t launch_stage_partials(const SliceDesc *d_parts, uint32_t nparts, uint32_t S,

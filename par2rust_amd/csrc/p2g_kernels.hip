@@ -334,13 +334,12 @@ constexpr int kMfTiles = 8;      // 32-row MFMA tiles per workgroup: 16 recovery
 constexpr int kMfThreads = 256;  // 4 waves, each 64 symbols (two 32-column tiles)
 
 // A fragments: frag (t, g) is 64 lanes x 16 bytes at afrag[(t * nq + g) * 64 + lane].
-// Thread per (t, g, lane).  Rows 2t + rr >= R and inputs >= n are zero.
-__global__ __launch_bounds__(256) void k_mfma_afrag(const uint16_t *__restrict__ antilog,
-                                                    const uint16_t *__restrict__ logbase,
-                                                    uint32_t n, uint32_t R, uint32_t nq,
-                                                    uint32_t tiles, uint4 *__restrict__ afrag) {
-    const uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (uint64_t)tiles * nq * 64) return;
+// Entry idx = (t * nq + g) * 64 + lane.  Row r of the fragments is recovery
+// exponent first_exp + r; rows 2t + rr >= R and inputs >= n are zero.
+__device__ __forceinline__ uint4 mfma_afrag_entry(const uint16_t *__restrict__ antilog,
+                                                  const uint16_t *__restrict__ logbase, uint32_t n,
+                                                  uint32_t R, uint32_t first_exp, uint32_t nq,
+                                                  uint64_t idx) {
     const uint32_t lane = (uint32_t)(idx & 63);
     const uint64_t tg = idx >> 6;
     const uint32_t g = (uint32_t)(tg % nq), t = (uint32_t)(tg / nq);
@@ -350,7 +349,7 @@ __global__ __launch_bounds__(256) void k_mfma_afrag(const uint16_t *__restrict__
     for (uint32_t e = 0; e < 2; e++) {
         const uint32_t i = 4 * g + 2 * h + e;
         if (r >= R || i >= n) continue;
-        uint32_t v = antilog[(uint32_t)(((uint64_t)logbase[i] * r) % kFieldLimit)];
+        uint32_t v = antilog[(uint32_t)(((uint64_t)logbase[i] * (first_exp + r)) % kFieldLimit)];
         uint32_t m = 0;  // row j of the matrix of multiplication by v
         for (int b = 0; b < 16; b++) {
             m |= ((v >> j) & 1u) << b;
@@ -359,7 +358,17 @@ __global__ __launch_bounds__(256) void k_mfma_afrag(const uint16_t *__restrict__
         d[2 * e] = mf_a_word(m, 0);
         d[2 * e + 1] = mf_a_word(m, 1);
     }
-    afrag[idx] = make_uint4(d[0], d[1], d[2], d[3]);
+    return make_uint4(d[0], d[1], d[2], d[3]);
+}
+
+// Thread per fragment entry.
+__global__ __launch_bounds__(256) void k_mfma_afrag(const uint16_t *__restrict__ antilog,
+                                                    const uint16_t *__restrict__ logbase,
+                                                    uint32_t n, uint32_t R, uint32_t nq,
+                                                    uint32_t tiles, uint4 *__restrict__ afrag) {
+    const uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (uint64_t)tiles * nq * 64) return;
+    afrag[idx] = mfma_afrag_entry(antilog, logbase, n, R, 0, nq, idx);
 }
 
 hipError_t launch_mfma_afrag(const uint16_t *d_antilog, const uint16_t *d_logbase, uint32_t n,
@@ -460,14 +469,18 @@ __device__ __forceinline__ void mf_load_desc(MfLds &lds, const SliceDesc *__rest
 }
 
 // One encode unit: row tiles tile0 .. tile0 + 7 (recovery rows 2 tile0 ..
-// 2 tile0 + 15) over the 256 symbols at byte colbase of every slice, all nq K
-// steps, stored to rec (XORed in when accumulate).  All 256 threads call it;
-// it leaves lds.a free for the next unit.
+// 2 tile0 + 15) over the 256 symbols at byte colbase of every slice, nq K
+// steps (the descriptors in lds.desc), stored to rec (XORed in when
+// accumulate).  Fragment (t, g) is afrag[(t * frag_stride + g) * 64 + lane]:
+// frag_stride = nq for a whole launch, the job's step count when this is
+// one K chunk of it.  All 256 threads call it; it leaves lds.a free for the
+// next unit.
 template <bool FULL>
 __device__ __forceinline__ void mf_unit(MfLds &lds, const SliceDesc *__restrict__ inputs,
-                                        const uint4 *__restrict__ afrag, uint32_t nq, uint32_t R,
-                                        uint32_t S, uint8_t *__restrict__ rec, int accumulate,
-                                        uint32_t tile0, uint32_t colbase) {
+                                        const uint4 *__restrict__ afrag, uint32_t nq,
+                                        uint32_t frag_stride, uint32_t R, uint32_t S,
+                                        uint8_t *__restrict__ rec, int accumulate, uint32_t tile0,
+                                        uint32_t colbase) {
     const uint4 *s_desc = lds.desc;
     uint4 (*s_a)[kMfTiles * 64] = lds.a;
     const uint32_t tid = threadIdx.x;
@@ -482,8 +495,8 @@ __device__ __forceinline__ void mf_unit(MfLds &lds, const SliceDesc *__restrict_
 
     // Thread tid stages A entries tid and tid + 256 of each K step: tile
     // (e / 64), lane (e % 64) of fragment (tile0 + tile, g).
-    const uint4 *ga0 = afrag + ((size_t)(tile0 + (tid >> 6)) * nq) * 64 + lane;
-    const uint4 *ga1 = afrag + ((size_t)(tile0 + 4 + (tid >> 6)) * nq) * 64 + lane;
+    const uint4 *ga0 = afrag + ((size_t)(tile0 + (tid >> 6)) * frag_stride) * 64 + lane;
+    const uint4 *ga1 = afrag + ((size_t)(tile0 + 4 + (tid >> 6)) * frag_stride) * 64 + lane;
 
     v16f_t acc[kMfTiles][2];
 #pragma unroll
@@ -599,7 +612,7 @@ __global__ __launch_bounds__(kMfThreads, 1) void k_encode_mfma(
     __shared__ MfLds lds;
     mf_load_desc<false>(lds, inputs, n, nq);
     __syncthreads();
-    mf_unit<false>(lds, inputs, afrag, nq, R, S, rec, accumulate, blockIdx.y * kMfTiles,
+    mf_unit<false>(lds, inputs, afrag, nq, nq, R, S, rec, accumulate, blockIdx.y * kMfTiles,
                    blockIdx.x * (kMfThreads * 2));
 }
 
@@ -1296,7 +1309,7 @@ __global__ __launch_bounds__(kMfThreads, 1) void k_create_pass(PassArgs a) {
         }
         const uint32_t c0 = bstart[band], w = bstart[band + 1] - c0;
         const uint32_t v = u - b_lo, y = v / w, x = c0 + v % w;
-        mf_unit<FULL>(lds, a.inputs, a.afrag, a.nq, a.R, a.S, a.rec, 0, y * kMfTiles,
+        mf_unit<FULL>(lds, a.inputs, a.afrag, a.nq, a.nq, a.R, a.S, a.rec, 0, y * kMfTiles,
                       x * (kMfThreads * 2));
         done++;
         const uint32_t nxt = u + e_blocks;
@@ -1392,6 +1405,393 @@ hipError_t launch_create_pass(const PassPlan &p, const SliceDesc *d_inputs,
         hipLaunchKernelGGL(k_create_pass<true>, dim3(p.grid), dim3(kMfThreads), 0, stream, a);
     else
         hipLaunchKernelGGL(k_create_pass<false>, dim3(p.grid), dim3(kMfThreads), 0, stream, a);
+    return hipGetLastError();
+}
+
+// ======================================================================
+// Batched create pass: many jobs in one persistent launch
+// ======================================================================
+//
+// The roles of k_create_pass over a table of pieces (a job's recovery rows,
+// or a 16-row-aligned part of them) and a table of slice jobs.  Units of all
+// pieces form one list in piece order, band by band inside a piece; encode
+// workgroups take them in a fixed stride and reload the descriptor table
+// from the piece when it changes, in K chunks of kMfmaChunkInputs inputs
+// (the first chunk stores, later ones XOR in).  Packet hashing runs as wave
+// tasks of up to 64 packets of one piece, so a task's band waits are wave
+// uniform; slice MD5 lanes and CRC waves walk the slice jobs.  Every role
+// raises its job's completion stamp (atomicMax on a u64 real-time count),
+// which gives per-job latency inside the batch.
+
+struct BatchArgs {
+    const BatchPiece *pieces;
+    const BatchSlices *sjobs;
+    const SliceDesc *descs;      // whole batch, indexed by global slice index
+    const uint4 *tail_blocks;    // 4 per global slice
+    uint32_t *sync;
+    unsigned long long *job_done;
+    uint32_t n_pieces, n_sjobs, slice_lo, slice_hi;
+    uint32_t n_units, n_tasks, h3_blocks, h2_blocks;
+    uint32_t n_jobs, stall_ticks;
+    CrcPowers pw;
+};
+
+__device__ __forceinline__ void batch_stamp(unsigned long long *slot) {
+    atomicMax(slot, (unsigned long long)__builtin_amdgcn_s_memrealtime());
+}
+
+// pass_wait, but the stall clock also restarts whenever any unit of the
+// launch is published (progress), so a task queued behind a long batch
+// waits as long as the encode keeps moving.
+__device__ __forceinline__ bool batch_wait(const uint32_t *cnt, uint32_t target,
+                                           const uint32_t *progress, uint32_t stall_ticks,
+                                           uint32_t *flags) {
+    uint32_t seen = __builtin_amdgcn_readfirstlane(
+        __hip_atomic_load(cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    uint32_t prog = __builtin_amdgcn_readfirstlane(
+        __hip_atomic_load(progress, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    uint64_t since = __builtin_amdgcn_s_memrealtime();
+    while (seen < target) {
+        __builtin_amdgcn_s_sleep(16);
+        const uint32_t now = __builtin_amdgcn_readfirstlane(
+            __hip_atomic_load(cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        const uint32_t pnow = __builtin_amdgcn_readfirstlane(
+            __hip_atomic_load(progress, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        const uint64_t t = __builtin_amdgcn_s_memrealtime();
+        if (now != seen || pnow != prog) {
+            seen = now;
+            prog = pnow;
+            since = t;
+        } else if (t - since > stall_ticks) {
+            atomicOr(flags, kPassErrWait);
+            return false;
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    return true;
+}
+
+// Packet r of piece P (exponent P.first_exp + r), band by band as in
+// pass_packet_lane.  A lane with store = false hashes along (its wave waits
+// together) and writes nothing.
+__device__ __forceinline__ void batch_packet_lane(const BatchArgs &a, const BatchPiece &P,
+                                                  uint32_t r, bool store) {
+    const uint32_t S = P.S;
+    const uint8_t *data = P.rec + (size_t)r * S;
+    const bool a16 = S % 16 == 0 && reinterpret_cast<uintptr_t>(P.rec) % 16 == 0;
+    const uint64_t len = 36ull + S;
+    const uint32_t nb = md5_blocks(len), nfull = (uint32_t)(len / 64);
+    const uint32_t bulk_end = nfull > 1 ? nfull : 1;
+    const uint32_t exp = P.first_exp + r;
+    PacketPrefix pre;
+#pragma unroll
+    for (int w = 0; w < 8; w++) pre.w[w] = P.pre[w];
+    uint32_t h[4];
+    md5_init(h);
+    uint32_t k = 0;
+    const uint32_t *bstart = a.sync + P.bstart0;
+    for (uint32_t b = 0; b < P.n_bands; b++) {
+        const uint32_t c1 = bstart[b + 1], w = c1 - bstart[b];
+        if (!batch_wait(a.sync + P.band0 + b, w * P.row_blocks, a.sync + kBatchProgress,
+                        a.stall_ticks, a.sync + kPassFlags))
+            return;
+        const uint64_t e = min((uint64_t)c1 * (kMfThreads * 2), (uint64_t)S);
+        const uint32_t avail = b + 1 == P.n_bands ? nb : e >= 28 ? (uint32_t)((e - 28) / 64) + 1 : 0;
+        if (k == 0 && avail > 0) {
+            md5_packet_block(h, pre, exp, data, len, 0, nb);
+            k = 1;
+        }
+        const uint32_t stop = min(avail, bulk_end);
+        if (stop > k) {
+            if (a16) md5_packet_bulk16(h, data, k, stop - k);
+            else md5_bulk<uint32_t>(h, data + 64 * (size_t)k - 36, stop - k);
+            k = stop;
+        }
+    }
+    for (; k < nb; k++) md5_packet_block(h, pre, exp, data, len, k, nb);
+    if (store) store_digest(P.pkt_md5 + (size_t)r * 16, h);
+}
+
+__global__ __launch_bounds__(kMfThreads, 1) void k_create_batch(BatchArgs a) {
+    __shared__ MfLds lds;
+    const uint32_t blk = blockIdx.x, tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    if (tid == 0) {
+        unsigned long long *t0 = a.job_done + a.n_jobs;
+        atomicMin(t0, (unsigned long long)__builtin_amdgcn_s_memrealtime());
+    }
+    if (blk < a.h3_blocks) {
+        // packet tasks: wave task t = 64 packets of one piece
+        uint32_t pi = 0;
+        for (uint32_t t = blk * 4 + wave; t < a.n_tasks; t += a.h3_blocks * 4) {
+            while (t >= a.pieces[pi].task_end) pi++;
+            const BatchPiece &P = a.pieces[pi];
+            const uint32_t r = (t - P.task0) * 64 + lane;
+            batch_packet_lane(a, P, min(r, P.R - 1), r < P.R);
+            if (lane == 0) batch_stamp(a.job_done + P.job);
+        }
+        return;
+    }
+    if (blk < a.h3_blocks + a.h2_blocks) {
+        const uint32_t hb = blk - a.h3_blocks;
+        if (wave >= 2) {
+            // CRC waves, one slice per wave, tables in this wave's part of lds.desc
+            const uint32_t cw = hb * 2 + wave - 2, n_cw = 2 * a.h2_blocks;
+            uint32_t(*t)[256] = reinterpret_cast<uint32_t(*)[256]>(lds.desc) + 4 * (wave - 2);
+            crc_tables_init_wave(t);
+            uint32_t sj = 0;
+            for (uint32_t sl = a.slice_lo + cw; sl < a.slice_hi; sl += n_cw) {
+                while (sl >= a.sjobs[sj].slice_end) sj++;
+                const BatchSlices &J = a.sjobs[sj];
+                crc_slice_wave(t, a.descs[sl], J.S, J.S % 16 == 0, a.pw, J.crc + (sl - J.slice0));
+                const uint32_t nxt = sl + n_cw;
+                if (lane == 0 && (nxt >= a.slice_hi || nxt >= J.slice_end))
+                    batch_stamp(a.job_done + J.job);
+            }
+            return;
+        }
+        // slice MD5 lanes on waves 0 and 1; a lane walks the slice jobs on its own
+        const uint32_t n_lanes = a.h2_blocks * 128;
+        uint32_t sj = 0;
+        for (uint32_t sl = a.slice_lo + (hb * 2 + wave) * 64 + lane; sl < a.slice_hi; sl += n_lanes) {
+            while (sl >= a.sjobs[sj].slice_end) sj++;
+            const BatchSlices J = a.sjobs[sj];
+            const uint4 *tb = a.tail_blocks + 4 * (size_t)sl;
+            uint8_t *out = J.md5 + (size_t)(sl - J.slice0) * 16;
+            if (J.S % 16 == 0) slice_md5_lane<true>(a.descs[sl], tb, J.S, out);
+            else slice_md5_lane<false>(a.descs[sl], tb, J.S, out);
+            const uint32_t nxt = sl + n_lanes;
+            if (nxt >= a.slice_hi || nxt >= J.slice_end) batch_stamp(a.job_done + J.job);
+        }
+        return;
+    }
+    // Encode.  Every value the loop branches on is computed alike by all
+    // threads from the tables (no broadcast), so the barriers inside are
+    // reached by the whole workgroup.
+    const uint32_t first = a.h3_blocks + a.h2_blocks;
+    const uint32_t e_blocks = gridDim.x - first;
+    uint32_t pi = 0;
+    const SliceDesc *loaded = nullptr;  // first descriptor of the chunk in lds.desc
+    uint32_t band = 0, b_lo = 0, b_hi = 0, unit_end = 0;
+    uint32_t done = 0;
+    for (uint32_t u = blk - first; u < a.n_units; u += e_blocks) {
+        if (u >= unit_end) {
+            while (u >= a.pieces[pi].unit_end) pi++;
+            const BatchPiece &P = a.pieces[pi];
+            unit_end = P.unit_end;
+            band = 0;
+            b_lo = P.unit0;
+            b_hi = P.unit0 + P.row_blocks * a.sync[P.bstart0 + 1];
+        }
+        const BatchPiece &P = a.pieces[pi];
+        const uint32_t *bstart = a.sync + P.bstart0;
+        while (u >= b_hi) {
+            band++;
+            b_lo = b_hi;
+            b_hi = P.unit0 + P.row_blocks * bstart[band + 1];
+        }
+        const uint32_t c0 = bstart[band], w = bstart[band + 1] - c0;
+        const uint32_t v = u - b_lo, y = v / w, x = c0 + v % w;
+        const uint4 *afrag = static_cast<const uint4 *>(P.afrag);
+        for (uint32_t k0 = 0; k0 < P.n; k0 += kMfmaChunkInputs) {
+            const SliceDesc *cin = P.inputs + k0;
+            const uint32_t cn = min(P.n - k0, kMfmaChunkInputs);
+            const uint32_t cq = min(P.nq - k0 / 4, kMfmaChunkInputs / 4);
+            if (cin != loaded) {
+                __syncthreads();  // every unit is done reading the old table
+                mf_load_desc<true>(lds, cin, cn, cq);
+                __syncthreads();
+                loaded = cin;
+            }
+            mf_unit<true>(lds, cin, afrag + (size_t)(k0 / 4) * 64, cq, P.nq, P.R, P.S, P.rec,
+                          k0 > 0, y * kMfTiles, x * (kMfThreads * 2));
+        }
+        done++;
+        const uint32_t nxt = u + e_blocks;
+        if (nxt >= a.n_units || nxt >= b_hi) {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+            __syncthreads();
+            if (tid == 0) {
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+                __hip_atomic_fetch_add(a.sync + P.band0 + band, done, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_fetch_add(a.sync + kBatchProgress, done, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+            }
+            done = 0;
+        }
+    }
+}
+
+// Zeroes the group's flags, progress, stamps and band counters, writes each
+// piece's band starts (thread per piece) and, for the first group of a
+// launch, clears the job completion stamps (the batch start slot gets the
+// maximum: it is a minimum over workgroups).
+__global__ void k_batch_reset(uint32_t *sync, uint32_t words, const BatchPiece *__restrict__ pieces,
+                              uint32_t n_pieces, unsigned long long *job_done, uint32_t n_jobs,
+                              int first) {
+    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x, gsz = gridDim.x * blockDim.x;
+    for (uint32_t i = gid; i < words; i += gsz) sync[i] = 0u;
+    if (first)
+        for (uint32_t j = gid; j <= n_jobs; j += gsz) job_done[j] = j == n_jobs ? ~0ull : 0ull;
+    for (uint32_t p = gid; p < n_pieces; p += gsz) {
+        const BatchPiece &P = pieces[p];
+        uint32_t *bs = sync + P.bstart0, c = 0;
+        for (uint32_t b = 0; b < P.n_bands; b++) {
+            bs[b] = c;
+            c += next_band_width(c, P.col_groups - c, P.band_cols);
+        }
+        bs[P.n_bands] = c;
+    }
+}
+
+// Fragments of every piece of a group, grid-stride per piece.
+__global__ __launch_bounds__(256) void k_batch_afrag(const uint16_t *__restrict__ antilog,
+                                                     const uint16_t *__restrict__ logbase,
+                                                     const BatchPiece *__restrict__ pieces,
+                                                     uint32_t n_pieces) {
+    const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t gsz = (uint64_t)gridDim.x * blockDim.x;
+    for (uint32_t p = 0; p < n_pieces; p++) {
+        const BatchPiece &P = pieces[p];
+        const uint64_t total = (uint64_t)mfma_row_tiles(P.R) * P.nq * 64;
+        uint4 *out = static_cast<uint4 *>(const_cast<void *>(P.afrag));
+        for (uint64_t idx = gid; idx < total; idx += gsz)
+            out[idx] = mfma_afrag_entry(antilog, logbase, P.n, P.R, P.first_exp, P.nq, idx);
+    }
+}
+
+// Copies of partial slices: blockIdx.y = part, x strides its words.
+__global__ __launch_bounds__(256) void k_stage_parts(const StagePart *__restrict__ parts) {
+    const StagePart d = parts[blockIdx.y];
+    const uint32_t words = d.S / 4;
+    for (uint32_t w = blockIdx.x * 256 + threadIdx.x; w < words; w += gridDim.x * 256)
+        reinterpret_cast<uint32_t *>(d.dst)[w] = load_word(d.src, 4 * w, d.valid);
+}
+
+hipError_t launch_stage_parts(const StagePart *d_parts, uint32_t nparts, uint32_t max_S,
+                              hipStream_t stream) {
+    for (uint32_t p0 = 0; p0 < nparts; p0 += 65535) {
+        const uint32_t np = std::min<uint32_t>(nparts - p0, 65535);
+        const uint32_t bx = std::min<uint32_t>((max_S / 4 + 255) / 256, 64);
+        hipLaunchKernelGGL(k_stage_parts, dim3(bx, np), dim3(256), 0, stream, d_parts + p0);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+hipError_t launch_batch_afrag(const uint16_t *d_antilog, const uint16_t *d_logbase,
+                              const BatchGroupLaunch &g, hipStream_t stream) {
+    if (g.n_pieces == 0 || g.afrag_entries == 0) return hipSuccess;
+    const uint32_t blocks = (uint32_t)std::min<uint64_t>((g.afrag_entries + 255) / 256, 16384);
+    hipLaunchKernelGGL(k_batch_afrag, dim3(blocks), dim3(256), 0, stream, d_antilog, d_logbase,
+                       g.pieces, g.n_pieces);
+    return hipGetLastError();
+}
+
+void plan_batch_group(uint32_t num_cus, uint32_t n_tasks, uint32_t n_slices, uint32_t n_units,
+                      BatchGroupLaunch *g) {
+    g->grid = num_cus;
+    // four packet waves per packet workgroup, two MD5 waves (128 lanes) and two
+    // CRC waves per slice workgroup; the rest encode
+    g->h3_blocks = std::min<uint32_t>((n_tasks + 3) / 4, 8);
+    g->h2_blocks = std::min<uint32_t>((n_slices + 127) / 128, 16);
+    if (n_units == 0) {
+        g->grid = std::max<uint32_t>(1, g->h2_blocks + g->h3_blocks);
+    } else if (g->h2_blocks + g->h3_blocks + 1 > num_cus) {
+        g->h3_blocks = std::min<uint32_t>(g->h3_blocks, 1);
+        g->h2_blocks = std::min<uint32_t>(g->h2_blocks, 1);
+    }
+}
+
+hipError_t launch_create_batch(const BatchGroupLaunch &g, const SliceDesc *d_descs,
+                               const uint4 *d_tail_blocks, hipStream_t stream) {
+    hipLaunchKernelGGL(k_batch_reset, dim3(64), dim3(256), 0, stream, g.sync, g.sync_words,
+                       g.pieces, g.n_pieces, g.job_done, g.n_jobs, g.first ? 1 : 0);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    BatchArgs a;
+    a.pieces = g.pieces;
+    a.sjobs = g.sjobs;
+    a.descs = d_descs;
+    a.tail_blocks = d_tail_blocks;
+    a.sync = g.sync;
+    a.job_done = g.job_done;
+    a.n_pieces = g.n_pieces;
+    a.n_sjobs = g.n_sjobs;
+    a.slice_lo = g.slice_lo;
+    a.slice_hi = g.slice_hi;
+    a.n_units = g.n_units;
+    a.n_tasks = g.n_tasks;
+    a.h3_blocks = g.h3_blocks;
+    a.h2_blocks = g.h2_blocks;
+    a.n_jobs = g.n_jobs;
+    a.stall_ticks = 200000000u;  // 2 s of the 100 MHz counter without any unit published
+    static const CrcPowers pw = make_crc_powers();
+    a.pw = pw;
+    hipLaunchKernelGGL(k_create_batch, dim3(g.grid), dim3(kMfThreads), 0, stream, a);
+    return hipGetLastError();
+}
+
+// ======================================================================
+// Synthetic input: the xorshift64 stream on the device
+// ======================================================================
+//
+// Word k of the stream is the state after k + 1 steps (x ^= x << 13;
+// x ^= x >> 7; x ^= x << 17) from the seed, stored little-endian.  The step
+// is linear over GF(2), so lane j starts from step^(j W) (seed), built from
+// jumps[b] = step^(W 2^b) (64 columns each) for the set bits b of j, and
+// writes words [j W, (j + 1) W).
+
+__device__ __forceinline__ uint64_t xs_apply(const uint64_t *__restrict__ cols, uint64_t v) {
+    uint64_t out = 0;
+#pragma unroll 8
+    for (int b = 0; b < 64; b++) out ^= cols[b] & (0ull - ((v >> b) & 1ull));
+    return out;
+}
+
+__global__ __launch_bounds__(256) void k_xorshift_fill(uint8_t *__restrict__ dst, uint64_t nbytes,
+                                                       uint64_t seed,
+                                                       const uint64_t *__restrict__ jumps,
+                                                       uint32_t lanes) {
+    const uint32_t j = blockIdx.x * 256 + threadIdx.x;
+    if (j >= lanes) return;
+    uint64_t s = seed;
+    for (int b = 0; b < 32; b++)
+        if ((j >> b) & 1u) s = xs_apply(jumps + 64 * b, s);
+    const uint64_t w0 = (uint64_t)j * kXorshiftLaneWords;
+    const uint64_t words = nbytes / 8;
+    uint64_t *out = reinterpret_cast<uint64_t *>(dst);
+    for (uint32_t k = 0; k < kXorshiftLaneWords; k += 2) {
+        uint64_t a, c;
+        s ^= s << 13; s ^= s >> 7; s ^= s << 17;
+        a = s;
+        s ^= s << 13; s ^= s >> 7; s ^= s << 17;
+        c = s;
+        const uint64_t w = w0 + k;
+        if (w + 2 <= words) {
+            *reinterpret_cast<ulonglong2 *>(out + w) = make_ulonglong2(a, c);
+        } else {
+            // the end of the buffer: whole words, then the bytes of a partial one
+            const uint64_t v[2] = {a, c};
+            for (int q = 0; q < 2; q++) {
+                const uint64_t o = 8 * (w + q);
+                for (int t = 0; t < 8; t++)
+                    if (o + t < nbytes) dst[o + t] = (uint8_t)(v[q] >> (8 * t));
+            }
+            return;
+        }
+    }
+}
+
+hipError_t launch_xorshift_fill(uint8_t *d_dst, uint64_t nbytes, uint64_t seed,
+                                const uint64_t *d_jumps, hipStream_t stream) {
+    const uint64_t words = (nbytes + 7) / 8;
+    const uint64_t lanes = (words + kXorshiftLaneWords - 1) / kXorshiftLaneWords;
+    if (lanes == 0) return hipSuccess;
+    if (lanes > 0xFFFFFFFFull) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_xorshift_fill, dim3((uint32_t)((lanes + 255) / 256)), dim3(256), 0, stream,
+                       d_dst, nbytes, seed, d_jumps, (uint32_t)lanes);
     return hipGetLastError();
 }
 
