@@ -62,6 +62,112 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def run_batch_mix(ctx, n_jobs: int, reps: int) -> dict:
+    """The batched-create comparison: the job mix of synth.batch_mix, every
+    file generated on the device, run (1) one job at a time through the plain
+    device path (p2g_dev_create_pass, then p2g_synchronize, per job) and (2)
+    as one batch (p2g_batch_plan + launch + wait, planning included), then
+    (3) as replays of a planned batch (HIP graph).  Jobs/s = jobs / wall time
+    of the whole run; latency of a job = its completion time from the start
+    of the run, all jobs being submitted together at the start.  Every job's
+    outputs of (2) are compared with those of (1)."""
+    import numpy as np
+
+    import par2rust_amd as p2
+    from par2rust_amd import hipmem, synth
+
+    mix = synth.batch_mix(n_jobs)
+    src_bytes = sum(sz for j in mix for sz, _ in j["files"])
+    t = time.perf_counter()
+    jobs = []
+    for k, j in enumerate(mix):
+        S, R = j["slice_size"], j["recovery"]
+        offs, o = [], 0
+        for sz, _ in j["files"]:
+            offs.append(o)
+            o += (sz + 255) // 256 * 256
+        arena = hipmem.DeviceBuffer(o)
+        for (sz, seed), off in zip(j["files"], offs):
+            ctx.dev_fill_xorshift64(arena.addr + off, sz, seed)
+        n = sum((sz + S - 1) // S for sz, _ in j["files"])
+        files = [(arena.addr + off, sz) for (sz, _), off in zip(j["files"], offs)]
+        outs = {m: [hipmem.DeviceBuffer(R * S), hipmem.DeviceBuffer(16 * n),
+                    hipmem.DeviceBuffer(4 * n), hipmem.DeviceBuffer(16 * R)]
+                for m in ("plain", "batch")}
+        sid = bytes((k * 7 + i) & 0xFF for i in range(16))
+        jobs.append({"S": S, "R": R, "n": n, "files": files, "arena": arena, "outs": outs,
+                     "sid": sid})
+    ctx.synchronize()
+    log(f"bench: batch mix of {n_jobs} jobs, {src_bytes / 1e9:.2f} GB, generated on the device "
+        f"in {time.perf_counter() - t:.1f}s")
+
+    def plain_run():
+        lat = []
+        t0 = time.perf_counter()
+        for j in jobs:
+            o = j["outs"]["plain"]
+            ctx.dev_create_pass(j["files"], j["S"], j["R"], j["sid"], *o)
+            ctx.synchronize()
+            lat.append(time.perf_counter() - t0)
+        return time.perf_counter() - t0, lat
+
+    bjobs = [p2.BatchJob(j["files"], j["S"], j["R"], j["sid"], *j["outs"]["batch"]) for j in jobs]
+
+    def batch_run():
+        t0 = time.perf_counter()
+        b = ctx.batch_plan(bjobs, use_graph=False)
+        t_plan = time.perf_counter() - t0
+        b.launch()
+        b.wait()
+        wall = time.perf_counter() - t0
+        ms = np.array(b.job_ms()) / 1e3
+        b.free()
+        # device completion times, placed so that the last one is the host's end
+        return wall, list(wall - (ms.max() - ms)), t_plan
+
+    plain_run()  # warm-up: scratch buffers reach their sizes
+    plain_wall, plain_lat = plain_run()
+    log(f"bench: plain one job at a time {plain_wall:.3f}s")
+    batch_run()
+    runs = [batch_run() for _ in range(reps)]
+    batch_wall, batch_lat, t_plan = min(runs, key=lambda r: r[0])
+    log(f"bench: batched {batch_wall:.3f}s (planning {t_plan * 1e3:.1f} ms)")
+
+    same = True
+    for j in jobs:
+        a, b = j["outs"]["plain"], j["outs"]["batch"]
+        same &= all(x.download() == y.download() for x, y in zip(a, b))
+    plan = ctx.batch_plan(bjobs, use_graph=True)
+    plan.launch()
+    plan.wait()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        plan.launch()
+    plan.wait()
+    replay_wall = (time.perf_counter() - t0) / reps
+    groups_ms = plan.job_ms()
+    plan.free()
+    log(f"bench: graph replay {replay_wall:.3f}s per batch; batch == plain: {same}")
+
+    pct = lambda v, q: round(float(np.percentile(np.array(v) * 1e3, q)), 2)
+    return {
+        "jobs": n_jobs, "source_bytes": src_bytes,
+        "slice_sizes": {str(S): sum(1 for j in jobs if j["S"] == S) for S in synth.BATCH_SLICE_SIZES},
+        "plain_jobs_per_s": round(n_jobs / plain_wall, 2),
+        "batched_jobs_per_s": round(n_jobs / batch_wall, 2),
+        "speedup": round(plain_wall / batch_wall, 3),
+        "replay_jobs_per_s": round(n_jobs / replay_wall, 2),
+        "plain_latency_ms": {"p50": pct(plain_lat, 50), "p99": pct(plain_lat, 99)},
+        "batched_latency_ms": {"p50": pct(batch_lat, 50), "p99": pct(batch_lat, 99)},
+        "batched_planning_ms": round(t_plan * 1e3, 2),
+        "batched_source_GBps": round(src_bytes / batch_wall / 1e9, 2),
+        "plain_source_GBps": round(src_bytes / plain_wall / 1e9, 2),
+        "replay_last_job_ms": round(max(groups_ms), 2),
+        "bit_exact_vs_plain": bool(same),
+        "latency": "completion time from a common submission at t = 0",
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -71,6 +177,9 @@ def main() -> int:
     ap.add_argument("--mode", default="pass", choices=["pass", "separate"],
                     help="pass: one fused create pass per step; separate: K1, K2, K3 launches")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--batch-jobs", type=int, default=256,
+                    help="jobs of the batched-create comparison (0 skips it)")
+    ap.add_argument("--batch-reps", type=int, default=3)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run one 512 MiB device-to-device copy (the reference reading "
                          "for the FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -216,6 +325,9 @@ def main() -> int:
                        "k3_packet_md5": round(k_ms["k3"], 4)}),
         "roofline": roof,
     }
+
+    if args.batch_jobs > 0:
+        out["batched"] = run_batch_mix(ctx, args.batch_jobs, args.batch_reps)
 
     if not args.no_cpu_baseline:
         from oracle import oracle as O

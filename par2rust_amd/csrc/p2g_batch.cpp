@@ -29,7 +29,7 @@ namespace p2g {
 
 // A fragments of one group (P2G_BATCH_AFRAG_BYTES overrides it); one row
 // block of a job is the smallest piece, whatever the budget.
-constexpr uint64_t kBatchAfragBudget = 8ull << 30;
+constexpr uint64_t kBatchAfragBudget = 16ull << 30;
 
 struct BatchPlan {
     Ctx *c = nullptr;
@@ -203,10 +203,13 @@ static int make_plan(Ctx *c, const p2g_batch_job *jobs, uint32_t njobs, bool use
         const uint32_t S = (uint32_t)J.slice_size, n = (uint32_t)jd[j].size();
         const uint32_t R = J.recovery_count, nq = mfma_steps(n);
         const uint64_t per_rb = mfma_afrag_bytes(n, 16);  // fragments of 16 recovery rows
+        // pieces of equal row-block counts, each within the budget
         const uint32_t max_rb = (uint32_t)std::max<uint64_t>(1, budget / per_rb);
+        const uint32_t rbs = (R + 15) / 16, npieces = (rbs + max_rb - 1) / max_rb;
+        const uint32_t piece_rows = 16 * (npieces ? (rbs + npieces - 1) / npieces : 1);
         job_group[oi] = (uint32_t)spans.size() - 1;
-        for (uint32_t r0 = 0; r0 < R; r0 += 16 * max_rb) {
-            const uint32_t Rp = std::min<uint32_t>(16 * max_rb, R - r0);
+        for (uint32_t r0 = 0; r0 < R; r0 += piece_rows) {
+            const uint32_t Rp = std::min<uint32_t>(piece_rows, R - r0);
             const uint64_t bytes = mfma_afrag_bytes(n, Rp);
             if (spans.back().p_hi > spans.back().p_lo && spans.back().afrag + bytes > budget) {
                 const uint32_t np = (uint32_t)pieces.size();
@@ -353,6 +356,28 @@ int wait_plan(BatchPlan *B) {
                       4, hipMemcpyDeviceToHost);
         if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "read batch flags", e);
         bad |= flags;
+    }
+    if (getenv("P2G_BATCH_VERBOSE")) {
+        // per group: first start and last end of each role, ms from the batch start
+        std::vector<unsigned long long> t0(1);
+        hipMemcpy(t0.data(), static_cast<unsigned long long *>(B->d_done.ptr) + B->njobs, 8,
+                  hipMemcpyDeviceToHost);
+        for (size_t k = 0; k < B->groups.size(); k++) {
+            unsigned long long t[2 * kPassRoles];
+            hipMemcpy(t, static_cast<uint32_t *>(B->d_sync.ptr) + B->sync_off[k] + kPassTimes,
+                      sizeof(t), hipMemcpyDeviceToHost);
+            const BatchGroupLaunch &g = B->groups[k];
+            fprintf(stderr, "batch group %zu: pieces %u units %u tasks %u slices %u h3 %u h2 %u |", k,
+                    g.n_pieces, g.n_units, g.n_tasks, g.slice_hi - g.slice_lo, g.h3_blocks, g.h2_blocks);
+            static const char *names[4] = {"packet", "slice", "encode", "crc"};
+            for (int r = 0; r < (int)kPassRoles; r++) {
+                const unsigned long long a = t[2 * r], b = t[2 * r + 1];
+                if (b == 0) continue;
+                fprintf(stderr, " %s %.2f-%.2f", names[r], (double)(a - t0[0]) * 1e-5,
+                        (double)(b - t0[0]) * 1e-5);
+            }
+            fprintf(stderr, "\n");
+        }
     }
     B->h_done.resize(B->njobs + 1);
     e = hipMemcpy(B->h_done.data(), B->d_done.ptr, (B->njobs + 1) * 8, hipMemcpyDeviceToHost);

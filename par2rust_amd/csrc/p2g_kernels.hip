@@ -518,8 +518,8 @@ __device__ __forceinline__ void mf_unit(MfLds &lds, const SliceDesc *__restrict_
 #define MF_FILL(P)                                                                            \
     do {                                                                                      \
         const uint32_t gp = min((uint32_t)(P), nq - 1);                                       \
-        a0_##P = ga0[(size_t)gp * 64];                                                        \
-        a1_##P = ga1[(size_t)gp * 64];                                                        \
+        a0_##P = gload(ga0 + (size_t)gp * 64);                                                \
+        a1_##P = gload(ga1 + (size_t)gp * 64);                                                \
         mf_load_words<FULL>(s_desc, inputs, gp, h, off_ld, w0_##P, w1_##P);                         \
     } while (0)
     MF_FILL(0);
@@ -536,8 +536,8 @@ __device__ __forceinline__ void mf_unit(MfLds &lds, const SliceDesc *__restrict_
 #define MF_STEP(GG, P, PN)                                                                    \
     do {                                                                                      \
         const uint32_t gf = min((GG) + kMfAhead, nq - 1);                                     \
-        a0_##P = ga0[(size_t)gf * 64];                                                        \
-        a1_##P = ga1[(size_t)gf * 64];                                                        \
+        a0_##P = gload(ga0 + (size_t)gf * 64);                                                \
+        a1_##P = gload(ga1 + (size_t)gf * 64);                                                \
         mf_load_words<FULL>(s_desc, inputs, gf, h, off_ld, w0_##P, w1_##P);                         \
         const uint4 *sa = s_a[(GG) & 1];                                                      \
         v8i_t Bn[2];                                                                          \
@@ -716,8 +716,13 @@ __device__ __forceinline__ void md5_load_block(uint32_t x[16], const W *p) {
 // without copies.  The main loop's loads are unconditional (their wait counts
 // are static); the tail loads only what exists.
 constexpr int kMd5Ring = kMd5Ahead + 1;
-template <typename W, typename Addr>
-__device__ __forceinline__ void md5_blocks_at(uint32_t h[4], Addr addr, uint32_t nblk) {
+struct Md5NoVisit {
+    __device__ void operator()(const uint32_t *) const {}
+};
+// visit(x) sees each block's 16 words before it is compressed.
+template <typename W, typename Addr, typename Visit = Md5NoVisit>
+__device__ __forceinline__ void md5_blocks_at(uint32_t h[4], Addr addr, uint32_t nblk,
+                                              Visit visit = Visit()) {
     uint32_t buf[kMd5Ring][16];
 #pragma unroll
     for (int k = 0; k < kMd5Ahead; k++)
@@ -727,6 +732,7 @@ __device__ __forceinline__ void md5_blocks_at(uint32_t h[4], Addr addr, uint32_t
 #pragma unroll
         for (int k = 0; k < kMd5Ring; k++) {
             md5_load_block(buf[(k + kMd5Ahead) % kMd5Ring], addr(b + k + kMd5Ahead));
+            visit(buf[k]);
             md5_compress(h, buf[k]);
         }
     }
@@ -736,6 +742,7 @@ __device__ __forceinline__ void md5_blocks_at(uint32_t h[4], Addr addr, uint32_t
             if (b + k < nblk) {
                 const uint32_t nxt = b + k + kMd5Ahead;
                 if (nxt < nblk) md5_load_block(buf[(k + kMd5Ahead) % kMd5Ring], addr(nxt));
+                visit(buf[k]);
                 md5_compress(h, buf[k]);
             }
         }
@@ -884,16 +891,28 @@ __device__ __forceinline__ uint32_t crc_step4(const uint32_t (*t)[256], uint32_t
 __device__ __forceinline__ uint32_t crc_step4(const CrcLds &L, uint32_t c, uint32_t w) {
     return crc_step4(L.t, c, w);
 }
+// Sixteen bytes per step (slicing by 16, tables t[0..15]): byte j of the
+// sixteen is followed by 15 - j bytes, so it indexes t[15 - j]; only the
+// first word depends on the running CRC.
+__device__ __forceinline__ uint32_t crc_step16(const uint32_t (*t)[256], uint32_t c, uint32_t w0,
+                                               uint32_t w1, uint32_t w2, uint32_t w3) {
+    c ^= w0;
+    return (t[15][c & 0xFFu] ^ t[14][(c >> 8) & 0xFFu] ^ t[13][(c >> 16) & 0xFFu] ^ t[12][c >> 24]) ^
+           (t[11][w1 & 0xFFu] ^ t[10][(w1 >> 8) & 0xFFu] ^ t[9][(w1 >> 16) & 0xFFu] ^ t[8][w1 >> 24]) ^
+           (t[7][w2 & 0xFFu] ^ t[6][(w2 >> 8) & 0xFFu] ^ t[5][(w2 >> 16) & 0xFFu] ^ t[4][w2 >> 24]) ^
+           (t[3][w3 & 0xFFu] ^ t[2][(w3 >> 8) & 0xFFu] ^ t[1][(w3 >> 16) & 0xFFu] ^ t[0][w3 >> 24]);
+}
 
 // The same four tables filled by one wave alone: entry t[k][i] is byte i
 // followed by k zero bytes, i.e. 8 (k + 1) bit steps from i, so each lane
 // computes its own entries and only the wave's LDS order has to hold.
+template <int NT = 4>
 __device__ __forceinline__ void crc_tables_init_wave(uint32_t (*t)[256]) {
     const uint32_t lane = threadIdx.x & 63;
     for (uint32_t e = lane; e < 256; e += 64) {
         uint32_t c = e;
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
+        for (int k = 0; k < NT; k++) {
 #pragma unroll
             for (int b = 0; b < 8; b++) c = (c & 1u) ? (kCrcPoly ^ (c >> 1)) : (c >> 1);
             t[k][e] = c;
@@ -1419,9 +1438,58 @@ hipError_t launch_create_pass(const PassPlan &p, const SliceDesc *d_inputs,
 // from the piece when it changes, in K chunks of kMfmaChunkInputs inputs
 // (the first chunk stores, later ones XOR in).  Packet hashing runs as wave
 // tasks of up to 64 packets of one piece, so a task's band waits are wave
-// uniform; slice MD5 lanes and CRC waves walk the slice jobs.  Every role
+// uniform; slice lanes (MD5 and CRC-32 of a slice from one read) walk the
+// slice jobs.  Every role
 // raises its job's completion stamp (atomicMax on a u64 real-time count),
 // which gives per-job latency inside the batch.
+
+// MD5 and CRC-32 of one zero-padded S-byte slice by one lane, from one read
+// of its bytes: the CRC takes each message block's words as the MD5 ring
+// hands them over, sixteen bytes per dependent step, then the words of the
+// final partial block below S.  t: this wave's 16 CRC tables
+// (crc_tables_init_wave<16>).
+template <bool A16>
+__device__ __forceinline__ void slice_md5_crc_lane(const SliceDesc d, const uint4 *tb, uint32_t S,
+                                                   const uint32_t (*t)[256], uint8_t *md5_out,
+                                                   uint32_t *crc_out) {
+    using W = typename std::conditional<A16, uint4, uint32_t>::type;
+    const uint32_t nb = md5_blocks(S);
+    const uint32_t nmsg = S / 64;
+    const uint32_t nfull = d.valid / 64;
+    const uint8_t *ptr = d.ptr;
+    uint32_t h[4];
+    md5_init(h);
+    uint32_t c = 0xFFFFFFFFu;
+    md5_blocks_at<W>(
+        h,
+        [=](uint32_t b) {
+            const void *a = b < nfull ? (const void *)(ptr + 64 * (size_t)b)
+                          : b == nfull ? (const void *)tb : (const void *)kZeroBlock;
+            return reinterpret_cast<const W *>(a);
+        },
+        nmsg,
+        [&](const uint32_t *x) {
+#pragma unroll
+            for (int w = 0; w < 16; w += 4) c = crc_step16(t, c, x[w], x[w + 1], x[w + 2], x[w + 3]);
+        });
+    for (uint32_t b = nmsg; b < nb; b++) {
+        const uint32_t base = b * 64u;
+        uint32_t x[16];
+#pragma unroll
+        for (int w = 0; w < 16; w++) {
+            const uint32_t o = base + 4u * w;
+            if (o < S) {
+                x[w] = load_word(d.ptr, o, d.valid);
+                c = crc_step4(t, c, x[w]);
+            } else {
+                x[w] = md5_pad_word(o, S, w, b, nb);
+            }
+        }
+        md5_compress(h, x);
+    }
+    store_digest(md5_out, h);
+    *crc_out = c ^ 0xFFFFFFFFu;
+}
 
 struct BatchArgs {
     const BatchPiece *pieces;
@@ -1433,11 +1501,19 @@ struct BatchArgs {
     uint32_t n_pieces, n_sjobs, slice_lo, slice_hi;
     uint32_t n_units, n_tasks, h3_blocks, h2_blocks;
     uint32_t n_jobs, stall_ticks;
-    CrcPowers pw;
 };
 
 __device__ __forceinline__ void batch_stamp(unsigned long long *slot) {
     atomicMax(slot, (unsigned long long)__builtin_amdgcn_s_memrealtime());
+}
+
+// First start / last end of each role's work in the group's sync words
+// (kPassTimes, as in the fused pass).
+__device__ __forceinline__ void batch_role(uint32_t *sync, int role, bool end) {
+    unsigned long long *t = reinterpret_cast<unsigned long long *>(sync + kPassTimes) + 2 * role;
+    const unsigned long long now = __builtin_amdgcn_s_memrealtime();
+    if (end) atomicMax(t + 1, now);
+    else atomicMin(t, now);
 }
 
 // pass_wait, but the stall clock also restarts whenever any unit of the
@@ -1521,6 +1597,7 @@ __global__ __launch_bounds__(kMfThreads, 1) void k_create_batch(BatchArgs a) {
     }
     if (blk < a.h3_blocks) {
         // packet tasks: wave task t = 64 packets of one piece
+        if (lane == 0) batch_role(a.sync, kPassPacket, false);
         uint32_t pi = 0;
         for (uint32_t t = blk * 4 + wave; t < a.n_tasks; t += a.h3_blocks * 4) {
             while (t >= a.pieces[pi].task_end) pi++;
@@ -1529,39 +1606,32 @@ __global__ __launch_bounds__(kMfThreads, 1) void k_create_batch(BatchArgs a) {
             batch_packet_lane(a, P, min(r, P.R - 1), r < P.R);
             if (lane == 0) batch_stamp(a.job_done + P.job);
         }
+        if (lane == 0) batch_role(a.sync, kPassPacket, true);
         return;
     }
     if (blk < a.h3_blocks + a.h2_blocks) {
+        // slice MD5 + CRC-32 lanes on waves 0 and 1 (CRC tables per wave in
+        // lds.desc); a lane walks the slice jobs on its own
+        if (wave >= 2) return;
         const uint32_t hb = blk - a.h3_blocks;
-        if (wave >= 2) {
-            // CRC waves, one slice per wave, tables in this wave's part of lds.desc
-            const uint32_t cw = hb * 2 + wave - 2, n_cw = 2 * a.h2_blocks;
-            uint32_t(*t)[256] = reinterpret_cast<uint32_t(*)[256]>(lds.desc) + 4 * (wave - 2);
-            crc_tables_init_wave(t);
-            uint32_t sj = 0;
-            for (uint32_t sl = a.slice_lo + cw; sl < a.slice_hi; sl += n_cw) {
-                while (sl >= a.sjobs[sj].slice_end) sj++;
-                const BatchSlices &J = a.sjobs[sj];
-                crc_slice_wave(t, a.descs[sl], J.S, J.S % 16 == 0, a.pw, J.crc + (sl - J.slice0));
-                const uint32_t nxt = sl + n_cw;
-                if (lane == 0 && (nxt >= a.slice_hi || nxt >= J.slice_end))
-                    batch_stamp(a.job_done + J.job);
-            }
-            return;
-        }
-        // slice MD5 lanes on waves 0 and 1; a lane walks the slice jobs on its own
+        static_assert(sizeof(MfLds::desc) >= 2 * 16 * 256 * 4, "two waves of 16 CRC tables");
+        uint32_t(*t)[256] = reinterpret_cast<uint32_t(*)[256]>(lds.desc) + 16 * wave;
+        crc_tables_init_wave<16>(t);
         const uint32_t n_lanes = a.h2_blocks * 128;
+        if (lane == 0) batch_role(a.sync, kPassSlice, false);
         uint32_t sj = 0;
         for (uint32_t sl = a.slice_lo + (hb * 2 + wave) * 64 + lane; sl < a.slice_hi; sl += n_lanes) {
             while (sl >= a.sjobs[sj].slice_end) sj++;
             const BatchSlices J = a.sjobs[sj];
             const uint4 *tb = a.tail_blocks + 4 * (size_t)sl;
-            uint8_t *out = J.md5 + (size_t)(sl - J.slice0) * 16;
-            if (J.S % 16 == 0) slice_md5_lane<true>(a.descs[sl], tb, J.S, out);
-            else slice_md5_lane<false>(a.descs[sl], tb, J.S, out);
+            uint8_t *mo = J.md5 + (size_t)(sl - J.slice0) * 16;
+            uint32_t *co = J.crc + (sl - J.slice0);
+            if (J.S % 16 == 0) slice_md5_crc_lane<true>(a.descs[sl], tb, J.S, t, mo, co);
+            else slice_md5_crc_lane<false>(a.descs[sl], tb, J.S, t, mo, co);
             const uint32_t nxt = sl + n_lanes;
             if (nxt >= a.slice_hi || nxt >= J.slice_end) batch_stamp(a.job_done + J.job);
         }
+        if (lane == 0) batch_role(a.sync, kPassSlice, true);
         return;
     }
     // Encode.  Every value the loop branches on is computed alike by all
@@ -1569,6 +1639,7 @@ __global__ __launch_bounds__(kMfThreads, 1) void k_create_batch(BatchArgs a) {
     // reached by the whole workgroup.
     const uint32_t first = a.h3_blocks + a.h2_blocks;
     const uint32_t e_blocks = gridDim.x - first;
+    if (tid == 0) batch_role(a.sync, kPassEncode, false);
     uint32_t pi = 0;
     const SliceDesc *loaded = nullptr;  // first descriptor of the chunk in lds.desc
     uint32_t band = 0, b_lo = 0, b_hi = 0, unit_end = 0;
@@ -1616,6 +1687,7 @@ __global__ __launch_bounds__(kMfThreads, 1) void k_create_batch(BatchArgs a) {
                                        __HIP_MEMORY_SCOPE_AGENT);
                 __hip_atomic_fetch_add(a.sync + kBatchProgress, done, __ATOMIC_RELAXED,
                                        __HIP_MEMORY_SCOPE_AGENT);
+                batch_role(a.sync, kPassEncode, true);
             }
             done = 0;
         }
@@ -1630,7 +1702,8 @@ __global__ void k_batch_reset(uint32_t *sync, uint32_t words, const BatchPiece *
                               uint32_t n_pieces, unsigned long long *job_done, uint32_t n_jobs,
                               int first) {
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x, gsz = gridDim.x * blockDim.x;
-    for (uint32_t i = gid; i < words; i += gsz) sync[i] = 0u;
+    for (uint32_t i = gid; i < words; i += gsz)
+        sync[i] = (i >= kPassTimes && i < kPassBands && ((i - kPassTimes) / 2) % 2 == 0) ? ~0u : 0u;
     if (first)
         for (uint32_t j = gid; j <= n_jobs; j += gsz) job_done[j] = j == n_jobs ? ~0ull : 0ull;
     for (uint32_t p = gid; p < n_pieces; p += gsz) {
@@ -1692,10 +1765,10 @@ hipError_t launch_batch_afrag(const uint16_t *d_antilog, const uint16_t *d_logba
 void plan_batch_group(uint32_t num_cus, uint32_t n_tasks, uint32_t n_slices, uint32_t n_units,
                       BatchGroupLaunch *g) {
     g->grid = num_cus;
-    // four packet waves per packet workgroup, two MD5 waves (128 lanes) and two
-    // CRC waves per slice workgroup; the rest encode
+    // four packet waves per packet workgroup, two MD5 + CRC-32 waves (128
+    // lanes) per slice workgroup; the rest encode
     g->h3_blocks = std::min<uint32_t>((n_tasks + 3) / 4, 8);
-    g->h2_blocks = std::min<uint32_t>((n_slices + 127) / 128, 16);
+    g->h2_blocks = std::min<uint32_t>((n_slices + 127) / 128, 24);
     if (n_units == 0) {
         g->grid = std::max<uint32_t>(1, g->h2_blocks + g->h3_blocks);
     } else if (g->h2_blocks + g->h3_blocks + 1 > num_cus) {
@@ -1727,8 +1800,6 @@ hipError_t launch_create_batch(const BatchGroupLaunch &g, const SliceDesc *d_des
     a.h2_blocks = g.h2_blocks;
     a.n_jobs = g.n_jobs;
     a.stall_ticks = 200000000u;  // 2 s of the 100 MHz counter without any unit published
-    static const CrcPowers pw = make_crc_powers();
-    a.pw = pw;
     hipLaunchKernelGGL(k_create_batch, dim3(g.grid), dim3(kMfThreads), 0, stream, a);
     return hipGetLastError();
 }

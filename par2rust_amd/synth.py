@@ -8,6 +8,8 @@ with powers of the 64x64 step matrix; the output equals the sequential loop.
 """
 from __future__ import annotations
 
+import math
+
 import numpy as np
 
 _M64 = np.uint64(0xFFFFFFFFFFFFFFFF)
@@ -68,3 +70,56 @@ def xorshift64_bytes(n: int, seed: int, lanes: int = 1 << 14) -> bytes:
         out[k] = s
     stream = out.T.reshape(-1)[:words]  # lane-major: lane j holds words [j*L, (j+1)*L)
     return stream.astype("<u8").tobytes()[:n]
+
+
+class _Uniform:
+    """Uniform doubles in [0, 1) from the xorshift64 stream: the top 53 bits
+    of each new state."""
+
+    def __init__(self, seed: int):
+        self.s = seed & 0xFFFFFFFFFFFFFFFF
+
+    def __call__(self) -> float:
+        s = self.s
+        s ^= (s << 13) & 0xFFFFFFFFFFFFFFFF
+        s ^= s >> 7
+        s ^= (s << 17) & 0xFFFFFFFFFFFFFFFF
+        self.s = s
+        return (s >> 11) / float(1 << 53)
+
+
+BATCH_SLICE_SIZES = (4096, 65536, 786432)
+
+
+def batch_mix(n_jobs: int = 256, seed: int = 3001, max_inputs: int = 32768):
+    """The batched-create job mix: per job a slice size drawn from {4 KiB,
+    64 KiB, 768 KiB}, a total source size log-uniform in [1 MiB, 1 GiB] split
+    into 1-16 files (each at least one byte, sizes proportional to weights
+    drawn from U[0.5, 1.5)), and recovery count ceil(U[1%, 10%] * slices),
+    at least 1.  A draw with more than max_inputs slices (the PAR2 limit;
+    4 KiB slices of more than 128 MiB) is drawn again.  File f of job j is
+    the xorshift64 stream seeded with seed + 64 j + f + 1.
+
+    Returns a list of dicts {slice_size, files: [(length, seed)], recovery}.
+    """
+    u = _Uniform(seed)
+    jobs = []
+    while len(jobs) < n_jobs:
+        S = BATCH_SLICE_SIZES[min(2, int(3 * u()))]
+        total = int(2 ** (20 + 10 * u()))
+        n = -(-total // S)
+        nfiles = 1 + min(15, int(16 * u()))
+        frac = u()
+        if n > max_inputs:
+            continue
+        w = [0.5 + u() for _ in range(nfiles)]
+        sizes = [max(1, int(total * x / sum(w))) for x in w[:-1]]
+        sizes.append(max(1, total - sum(sizes)))
+        j = len(jobs)
+        n = sum(-(-s // S) for s in sizes)
+        if n > max_inputs:
+            continue
+        R = max(1, math.ceil((0.01 + 0.09 * frac) * n))
+        jobs.append({"slice_size": S, "recovery": R,
+                     "files": [(sz, seed + 64 * j + f + 1) for f, sz in enumerate(sizes)]})
+    return jobs
