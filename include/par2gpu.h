@@ -200,6 +200,52 @@ void p2g_batch_free(p2g_batch *b);
 /* Plan, queue and, at the next p2g_synchronize, free: one batched run. */
 int p2g_dev_create_batch(p2g_ctx *ctx, const p2g_batch_job *jobs, uint32_t njobs, void *stream);
 
+/* ---------------------------------------------------------------------
+ * Serving (no reference counterpart): a request loop for a service that
+ * creates PAR2 sets on demand.  Requests are admitted in arrival order
+ * while their footprint (files, recovery blocks, digests) fits an HBM
+ * budget taken at open; admitted requests stay resident and run in
+ * continuous batching steps (batched launches over every admitted job with
+ * work left; a job longer than the step target runs a piece of its
+ * recovery rows per step), so requests that arrive while the device is
+ * busy join the next step.  Each job's results are those of
+ * p2g_dev_create_pass on the same input.
+ * ------------------------------------------------------------------- */
+
+typedef struct p2g_server p2g_server;
+
+/* One create request.  File f is host_data[f] (file_lens[f] bytes, kept
+ * valid until the request is done) or, with host_data NULL, the xorshift64
+ * stream of fill_seeds[f] (p2g_dev_fill_xorshift64) written on the device. */
+typedef struct {
+    const uint64_t *file_lens;
+    const uint8_t *const *host_data;
+    const uint64_t *fill_seeds;
+    uint64_t slice_size;
+    uint32_t nfiles;
+    uint32_t recovery_count;
+    uint8_t set_id[16];
+} p2g_request;
+
+/* hbm_budget: bytes of device memory for admitted requests (allocated at
+ * open); step_ms: target device time of one step (0: 100 ms). */
+int p2g_server_open(p2g_ctx *ctx, uint64_t hbm_budget, double step_ms, p2g_server **out);
+/* Queues a request; id_out identifies it.  Fails at once for a request
+ * that could never be admitted. */
+int p2g_server_submit(p2g_server *s, const p2g_request *req, uint64_t *id_out);
+/* Drives the loop: retires finished steps, admits queued requests, launches
+ * the next steps; returns after wait_ms or as soon as requests are done,
+ * with up to cap of their ids in done_ids. */
+int p2g_server_poll(p2g_server *s, double wait_ms, uint64_t *done_ids, uint32_t cap,
+                    uint32_t *n_done);
+/* A done request's outputs (device pointers, as a p2g_batch_job) and its
+ * submit, admit and done times in ms on the server's clock. */
+int p2g_server_result(p2g_server *s, uint64_t id, p2g_batch_job *out, double times_ms[3]);
+/* Returns a done request's memory to the budget. */
+int p2g_server_release(p2g_server *s, uint64_t id);
+int p2g_server_stats(p2g_server *s, uint32_t *queued, uint32_t *active, uint64_t *hbm_used);
+void p2g_server_close(p2g_server *s);
+
 /* Synthetic input for benchmarks and tests (no reference counterpart):
  * fills nbytes of dst (device, 16-byte aligned) with the xorshift64 stream
  * from `seed` (x ^= x << 13; x ^= x >> 7; x ^= x << 17; each new state

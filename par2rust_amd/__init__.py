@@ -10,6 +10,7 @@ from __future__ import annotations
 
 import ctypes
 import os
+import struct
 from dataclasses import dataclass, field
 from typing import List, Optional, Sequence, Tuple
 
@@ -43,6 +44,8 @@ EXPORTED_SYMBOLS = [
     "p2g_pass_timing", "p2g_synchronize", "p2g_run_create",
     "p2g_batch_plan", "p2g_batch_launch", "p2g_batch_wait", "p2g_batch_job_ms",
     "p2g_batch_free", "p2g_dev_create_batch", "p2g_dev_fill_xorshift64",
+    "p2g_server_open", "p2g_server_submit", "p2g_server_poll", "p2g_server_result",
+    "p2g_server_release", "p2g_server_stats", "p2g_server_close",
 ]
 
 
@@ -68,6 +71,19 @@ class _BatchJob(ctypes.Structure):
         ("slice_md5", ctypes.c_void_p),
         ("slice_crc", ctypes.c_void_p),
         ("pkt_md5", ctypes.c_void_p),
+        ("nfiles", ctypes.c_uint32),
+        ("recovery_count", ctypes.c_uint32),
+        ("set_id", ctypes.c_uint8 * 16),
+    ]
+
+
+class _Request(ctypes.Structure):
+    """p2g_request."""
+    _fields_ = [
+        ("file_lens", ctypes.POINTER(ctypes.c_uint64)),
+        ("host_data", ctypes.POINTER(ctypes.c_void_p)),
+        ("fill_seeds", ctypes.POINTER(ctypes.c_uint64)),
+        ("slice_size", ctypes.c_uint64),
         ("nfiles", ctypes.c_uint32),
         ("recovery_count", ctypes.c_uint32),
         ("set_id", ctypes.c_uint8 * 16),
@@ -121,6 +137,16 @@ def _load() -> ctypes.CDLL:
         "p2g_batch_free": (None, [P]),
         "p2g_dev_create_batch": (ctypes.c_int, [P, P, ctypes.c_uint32, P]),
         "p2g_dev_fill_xorshift64": (ctypes.c_int, [P, P, ctypes.c_uint64, ctypes.c_uint64, P]),
+        "p2g_server_open": (ctypes.c_int, [P, ctypes.c_uint64, ctypes.c_double, ctypes.POINTER(P)]),
+        "p2g_server_submit": (ctypes.c_int, [P, P, ctypes.POINTER(ctypes.c_uint64)]),
+        "p2g_server_poll": (ctypes.c_int, [P, ctypes.c_double, ctypes.POINTER(ctypes.c_uint64),
+                                           ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32)]),
+        "p2g_server_result": (ctypes.c_int, [P, ctypes.c_uint64, P, ctypes.POINTER(ctypes.c_double)]),
+        "p2g_server_release": (ctypes.c_int, [P, ctypes.c_uint64]),
+        "p2g_server_stats": (ctypes.c_int, [P, ctypes.POINTER(ctypes.c_uint32),
+                                            ctypes.POINTER(ctypes.c_uint32),
+                                            ctypes.POINTER(ctypes.c_uint64)]),
+        "p2g_server_close": (None, [P]),
         "p2g_run_create": (ctypes.c_int, [P, ctypes.c_char_p, ctypes.c_uint64, ctypes.c_uint32,
                                           ctypes.POINTER(_Scheme), ctypes.c_uint32,
                                           ctypes.POINTER(ctypes.c_char_p),
@@ -289,6 +315,90 @@ class Batch:
             pass
 
 
+class Server:
+    """The serving loop (p2g_server_*): submit requests, poll for finished
+    ones, read their outputs, release them."""
+
+    def __init__(self, ctx: "Context", hbm_budget: int, step_ms: float = 0.0):
+        self.ctx = ctx
+        h = ctypes.c_void_p()
+        ctx._check(lib.p2g_server_open(ctx.h, hbm_budget, step_ms, ctypes.byref(h)))
+        self.h = h
+        self._keep = {}
+
+    def submit(self, file_lens: Sequence[int], slice_size: int, recovery_count: int,
+               set_id: bytes, host_data: Optional[Sequence[bytes]] = None,
+               seeds: Optional[Sequence[int]] = None) -> int:
+        n = len(file_lens)
+        r = _Request()
+        lens = (ctypes.c_uint64 * max(1, n))(*file_lens)
+        keep = [lens]
+        r.file_lens = lens
+        if host_data is not None:
+            bufs = [ctypes.create_string_buffer(bytes(d), max(1, len(d))) for d in host_data]
+            ptrs = (ctypes.c_void_p * max(1, n))(*[ctypes.addressof(b) for b in bufs])
+            keep += [bufs, ptrs]
+            r.host_data = ptrs
+        if seeds is not None:
+            sd = (ctypes.c_uint64 * max(1, n))(*seeds)
+            keep.append(sd)
+            r.fill_seeds = sd
+        r.slice_size, r.nfiles, r.recovery_count = slice_size, n, recovery_count
+        r.set_id[:] = list(bytes(set_id))
+        rid = ctypes.c_uint64()
+        self.ctx._check(lib.p2g_server_submit(self.h, ctypes.byref(r), ctypes.byref(rid)))
+        self._keep[rid.value] = keep  # host bytes stay valid until release
+        return rid.value
+
+    def poll(self, wait_ms: float = 0.0, cap: int = 4096) -> List[int]:
+        ids = (ctypes.c_uint64 * cap)()
+        n = ctypes.c_uint32()
+        self.ctx._check(lib.p2g_server_poll(self.h, wait_ms, ids, cap, ctypes.byref(n)))
+        return list(ids[:n.value])
+
+    def result(self, rid: int):
+        """(p2g_batch_job with device pointers, (submit, admit, done) ms)."""
+        job = _BatchJob()
+        t = (ctypes.c_double * 3)()
+        self.ctx._check(lib.p2g_server_result(self.h, rid, ctypes.byref(job), t))
+        return job, tuple(t)
+
+    def download(self, rid: int):
+        """(recovery blocks, slice MD5s, slice CRCs, packet MD5s) of a done request."""
+        from . import hipmem
+
+        job, _ = self.result(rid)
+        S, R = job.slice_size, job.recovery_count
+        files = ctypes.cast(job.files, ctypes.POINTER(_DevFile))
+        n = sum((files[f].len + S - 1) // S for f in range(job.nfiles))
+        rec = hipmem.download_ptr(job.rec, R * S) if R else b""
+        md5 = hipmem.download_ptr(job.slice_md5, 16 * n)
+        crc = list(struct.unpack(f"<{n}I", hipmem.download_ptr(job.slice_crc, 4 * n)))
+        pkt = hipmem.download_ptr(job.pkt_md5, 16 * R) if R else b""
+        return rec, md5, crc, pkt
+
+    def release(self, rid: int):
+        self.ctx._check(lib.p2g_server_release(self.h, rid))
+        self._keep.pop(rid, None)
+
+    def stats(self):
+        q, a, u = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint64()
+        self.ctx._check(lib.p2g_server_stats(self.h, ctypes.byref(q), ctypes.byref(a),
+                                             ctypes.byref(u)))
+        return {"queued": q.value, "active": a.value, "hbm_used": u.value}
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib.p2g_server_close(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Context:
     """One p2g context on one GPU (p2g_open / p2g_close)."""
 
@@ -425,6 +535,9 @@ class Context:
         """The synthetic workload stream (par2rust_amd.synth) written on the device."""
         self._check(lib.p2g_dev_fill_xorshift64(self.h, ctypes.c_void_p(_dev_ptr(dst)), nbytes,
                                                 seed, ctypes.c_void_p(stream) if stream else None))
+
+    def server(self, hbm_budget: int, step_ms: float = 0.0) -> Server:
+        return Server(self, hbm_budget, step_ms)
 
     def batch_plan(self, jobs: Sequence[BatchJob], use_graph: bool = True) -> Batch:
         return Batch(self, jobs, use_graph)

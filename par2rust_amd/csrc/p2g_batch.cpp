@@ -23,7 +23,7 @@
 #include <numeric>
 #include <vector>
 
-#include "p2g_context.hpp"
+#include "p2g_batch.hpp"
 
 namespace p2g {
 
@@ -31,33 +31,21 @@ namespace p2g {
 // block of a job is the smallest piece, whatever the budget.
 constexpr uint64_t kBatchAfragBudget = 16ull << 30;
 
-struct BatchPlan {
-    Ctx *c = nullptr;
-    uint32_t njobs = 0;
-    uint32_t n_slices = 0, n_parts = 0, max_part_S = 0;
-    DevBuf d_descs, d_tb, d_staging, d_parts, d_pieces, d_sjobs, d_sync, d_afrag, d_done;
-    std::vector<BatchGroupLaunch> groups;
-    std::vector<uint32_t> sync_off;  // word offset of each group's sync words
-    hipStream_t cap = nullptr;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    hipEvent_t done = nullptr;
-    bool launched = false;
-    std::vector<unsigned long long> h_done;  // completion stamps of the last launch, then start
-
-    int record(hipStream_t st);
-    ~BatchPlan();
-};
+void BatchPlan::drop_graph() {
+    if (exec) hipGraphExecDestroy(exec);
+    if (graph) hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+}
 
 BatchPlan::~BatchPlan() {
     if (c) hipSetDevice(c->device);
     if (done) hipEventSynchronize(done);
-    if (exec) hipGraphExecDestroy(exec);
-    if (graph) hipGraphDestroy(graph);
+    drop_graph();
     if (cap) hipStreamDestroy(cap);
     if (done) hipEventDestroy(done);
-    for (DevBuf *b : {&d_descs, &d_tb, &d_staging, &d_parts, &d_pieces, &d_sjobs, &d_sync,
-                      &d_afrag, &d_done})
+    for (DevBuf *b : {&d_descs, &d_tb, &d_staging, &d_parts, &d_pieces, &d_sjobs, &d_sync, &d_done,
+                      &own_afrag})
         b->release();
 }
 
@@ -102,60 +90,67 @@ static int upload(Ctx *c, DevBuf &d, const std::vector<T> &v, const char *what) 
     return P2G_OK;
 }
 
-static int make_plan(Ctx *c, const p2g_batch_job *jobs, uint32_t njobs, bool use_graph,
-                     BatchPlan **out) {
-    *out = nullptr;
-    if (njobs == 0 || !jobs) return c->fail(P2G_ERR_INVALID_ARGUMENT, "empty batch");
+int check_batch_job(Ctx *c, const p2g_batch_job &J, std::vector<SliceDesc> &descs) {
+    const uint32_t R = J.recovery_count;
+    if ((J.nfiles && !J.files) || !J.slice_md5 || !J.slice_crc || (R && (!J.rec || !J.pkt_md5)))
+        return c->fail(P2G_ERR_INVALID_ARGUMENT, "null argument in a batch job");
+    for (const void *p : {(const void *)J.rec, (const void *)J.slice_md5, (const void *)J.slice_crc,
+                          (const void *)J.pkt_md5})
+        if (reinterpret_cast<uintptr_t>(p) % 4 != 0)
+            return c->fail(P2G_ERR_INVALID_ARGUMENT, "batch outputs must be 4-byte aligned");
+    if (R > P2G_MAX_RECOVERY_BLOCKS)
+        return c->fail(P2G_ERR_TOO_MANY_RECOVERY_BLOCKS, "recovery count above 65535");
+    int rc = check_slice_size(c, J.slice_size);
+    if (rc != P2G_OK) return rc;
+    descs.clear();
+    rc = append_descs(c, J.files, J.nfiles, J.slice_size, descs);
+    if (rc != P2G_OK) return rc;
+    if (descs.empty()) return c->fail(P2G_ERR_INVALID_ARGUMENT, "a batch job has no slices");
+    if (descs.size() > kMaxInputs)
+        return c->fail(P2G_ERR_TOO_MANY_INPUT_BLOCKS, "more than 32768 input slices");
+    return P2G_OK;
+}
+
+// Plans the items into B (its buffers are reused and grown; the previous
+// launch of B must have finished).  Completion stamps are per item.
+int build_plan(Ctx *c, const std::vector<PlanItem> &items, bool use_graph, BatchPlan *B) {
+    const uint32_t nitems = (uint32_t)items.size();
+    if (nitems == 0) return c->fail(P2G_ERR_INVALID_ARGUMENT, "empty batch");
     hipSetDevice(c->device);
-    std::vector<std::vector<SliceDesc>> jd(njobs);
-    for (uint32_t j = 0; j < njobs; j++) {
-        const p2g_batch_job &J = jobs[j];
-        const uint32_t R = J.recovery_count;
-        if ((J.nfiles && !J.files) || !J.slice_md5 || !J.slice_crc || (R && (!J.rec || !J.pkt_md5)))
-            return c->fail(P2G_ERR_INVALID_ARGUMENT, "null argument in a batch job");
-        for (const void *p : {(const void *)J.rec, (const void *)J.slice_md5,
-                              (const void *)J.slice_crc, (const void *)J.pkt_md5})
-            if (reinterpret_cast<uintptr_t>(p) % 4 != 0)
-                return c->fail(P2G_ERR_INVALID_ARGUMENT, "batch outputs must be 4-byte aligned");
-        if (R > P2G_MAX_RECOVERY_BLOCKS)
-            return c->fail(P2G_ERR_TOO_MANY_RECOVERY_BLOCKS, "recovery count above 65535");
-        int rc = check_slice_size(c, J.slice_size);
+    B->c = c;
+    B->njobs = nitems;
+    B->groups.clear();
+    B->sync_off.clear();
+    B->drop_graph();
+    std::vector<std::vector<SliceDesc>> jd(nitems);
+    for (uint32_t j = 0; j < nitems; j++) {
+        const PlanItem &it = items[j];
+        int rc = check_batch_job(c, *it.job, jd[j]);
         if (rc != P2G_OK) return rc;
-        rc = append_descs(c, J.files, J.nfiles, J.slice_size, jd[j]);
-        if (rc != P2G_OK) return rc;
-        if (jd[j].empty()) return c->fail(P2G_ERR_INVALID_ARGUMENT, "a batch job has no slices");
-        if (jd[j].size() > kMaxInputs)
-            return c->fail(P2G_ERR_TOO_MANY_INPUT_BLOCKS, "more than 32768 input slices");
+        if (it.r0 > it.r1 || it.r1 > it.job->recovery_count)
+            return c->fail(P2G_ERR_INVALID_ARGUMENT, "recovery row range of a batch item");
     }
-    std::vector<uint32_t> order(njobs);
+    std::vector<uint32_t> order(nitems);
     std::iota(order.begin(), order.end(), 0u);
     std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
-        if (jobs[a].slice_size != jobs[b].slice_size) return jobs[a].slice_size > jobs[b].slice_size;
+        const uint64_t sa = items[a].job->slice_size, sb = items[b].job->slice_size;
+        if (sa != sb) return sa > sb;
         return jd[a].size() > jd[b].size();
     });
-
-    BatchPlan *B = new (std::nothrow) BatchPlan();
-    if (!B) return c->fail(P2G_ERR_OUT_OF_MEMORY, "batch plan");
-    B->c = c;
-    B->njobs = njobs;
-    auto bail = [&](int rc) {
-        delete B;
-        return rc;
-    };
 
     // staging slots of the partial slices
     uint64_t stage_bytes = 0, n_slices = 0;
     for (uint32_t j : order) {
-        const uint32_t S = (uint32_t)jobs[j].slice_size;
+        const uint32_t S = (uint32_t)items[j].job->slice_size;
         for (const SliceDesc &d : jd[j])
             if (d.valid < S) stage_bytes = align_up(stage_bytes, 256) + S;
         n_slices += jd[j].size();
     }
-    if (n_slices > 0x7FFFFFFFull) return bail(c->fail(P2G_ERR_INVALID_ARGUMENT, "too many slices"));
+    if (n_slices > 0x7FFFFFFFull) return c->fail(P2G_ERR_INVALID_ARGUMENT, "too many slices");
     if (B->d_staging.reserve(std::max<uint64_t>(stage_bytes, 256)) != P2G_OK ||
         B->d_descs.reserve(n_slices * sizeof(SliceDesc)) != P2G_OK ||
         B->d_tb.reserve(n_slices * 64) != P2G_OK)
-        return bail(c->fail(P2G_ERR_OUT_OF_MEMORY, "batch descriptor tables"));
+        return c->fail(P2G_ERR_OUT_OF_MEMORY, "batch descriptor tables");
     uint8_t *staging = static_cast<uint8_t *>(B->d_staging.ptr);
     const SliceDesc *d_descs = static_cast<const SliceDesc *>(B->d_descs.ptr);
 
@@ -163,10 +158,13 @@ static int make_plan(Ctx *c, const p2g_batch_job *jobs, uint32_t njobs, bool use
     all.reserve(n_slices);
     std::vector<StagePart> parts;
     std::vector<BatchSlices> sjobs;
-    std::vector<uint32_t> slice0(njobs);
+    std::vector<uint32_t> slice0(nitems);
+    std::vector<uint8_t> has_slices(nitems);
     uint64_t so = 0;
+    B->max_part_S = 0;
     for (uint32_t j : order) {
-        const uint32_t S = (uint32_t)jobs[j].slice_size;
+        const p2g_batch_job &J = *items[j].job;
+        const uint32_t S = (uint32_t)J.slice_size;
         slice0[j] = (uint32_t)all.size();
         for (const SliceDesc &d : jd[j]) {
             if (d.valid < S) {
@@ -179,7 +177,9 @@ static int make_plan(Ctx *c, const p2g_batch_job *jobs, uint32_t njobs, bool use
                 all.push_back(d);
             }
         }
-        sjobs.push_back({jobs[j].slice_md5, jobs[j].slice_crc, slice0[j], (uint32_t)all.size(), S, j});
+        // an item without its slice checksums has an empty slice range
+        const uint32_t lo = slice0[j], hi = items[j].slices ? (uint32_t)all.size() : lo;
+        sjobs.push_back({J.slice_md5, J.slice_crc, lo, hi, S, j});
     }
     B->n_slices = (uint32_t)n_slices;
     B->n_parts = (uint32_t)parts.size();
@@ -195,27 +195,27 @@ static int make_plan(Ctx *c, const p2g_batch_job *jobs, uint32_t njobs, bool use
         uint64_t afrag;
     };
     std::vector<GroupSpan> spans(1, GroupSpan{0, 0, 0, 0, 0});
-    std::vector<uint32_t> job_group(njobs);  // by position in `order`; non-decreasing
+    std::vector<uint32_t> job_group(nitems);  // by position in `order`; non-decreasing
     uint64_t afrag_max = 0;
-    for (uint32_t oi = 0; oi < njobs; oi++) {
+    for (uint32_t oi = 0; oi < nitems; oi++) {
         const uint32_t j = order[oi];
-        const p2g_batch_job &J = jobs[j];
+        const p2g_batch_job &J = *items[j].job;
         const uint32_t S = (uint32_t)J.slice_size, n = (uint32_t)jd[j].size();
-        const uint32_t R = J.recovery_count, nq = mfma_steps(n);
+        const uint32_t rlo = items[j].r0, rhi = items[j].r1, nq = mfma_steps(n);
         const uint64_t per_rb = mfma_afrag_bytes(n, 16);  // fragments of 16 recovery rows
         // pieces of equal row-block counts, each within the budget
         const uint32_t max_rb = (uint32_t)std::max<uint64_t>(1, budget / per_rb);
-        const uint32_t rbs = (R + 15) / 16, npieces = (rbs + max_rb - 1) / max_rb;
+        const uint32_t rbs = (rhi - rlo + 15) / 16, npieces = (rbs + max_rb - 1) / max_rb;
         const uint32_t piece_rows = 16 * (npieces ? (rbs + npieces - 1) / npieces : 1);
         job_group[oi] = (uint32_t)spans.size() - 1;
-        for (uint32_t r0 = 0; r0 < R; r0 += piece_rows) {
-            const uint32_t Rp = std::min<uint32_t>(piece_rows, R - r0);
+        for (uint32_t r0 = rlo; r0 < rhi; r0 += piece_rows) {
+            const uint32_t Rp = std::min<uint32_t>(piece_rows, rhi - r0);
             const uint64_t bytes = mfma_afrag_bytes(n, Rp);
             if (spans.back().p_hi > spans.back().p_lo && spans.back().afrag + bytes > budget) {
                 const uint32_t np = (uint32_t)pieces.size();
                 spans.push_back({np, np, 0, 0, 0});
             }
-            if (r0 == 0) job_group[oi] = (uint32_t)spans.size() - 1;
+            if (r0 == rlo) job_group[oi] = (uint32_t)spans.size() - 1;
             BatchPiece P{};
             P.inputs = d_descs + slice0[j];
             P.rec = J.rec + (size_t)r0 * S;
@@ -240,16 +240,16 @@ static int make_plan(Ctx *c, const p2g_batch_job *jobs, uint32_t njobs, bool use
             afrag_max = std::max(afrag_max, spans.back().afrag);
         }
     }
-    // the slice jobs of group k: the jobs whose first piece (or, with no
+    // the slice jobs of group k: the items whose first piece (or, with no
     // recovery rows, whose place in the order) falls in it
     for (uint32_t k = 0; k < spans.size(); k++) {
         spans[k].s_lo = (uint32_t)(std::lower_bound(job_group.begin(), job_group.end(), k) - job_group.begin());
         spans[k].s_hi = (uint32_t)(std::upper_bound(job_group.begin(), job_group.end(), k) - job_group.begin());
     }
 
-    if (B->d_afrag.reserve(std::max<uint64_t>(afrag_max, 256)) != P2G_OK)
-        return bail(c->fail(P2G_ERR_OUT_OF_MEMORY, "batch A fragments"));
-    uint8_t *afrag_base = static_cast<uint8_t *>(B->d_afrag.ptr);
+    if (B->afrag->reserve(std::max<uint64_t>(afrag_max, 256)) != P2G_OK)
+        return c->fail(P2G_ERR_OUT_OF_MEMORY, "batch A fragments");
+    uint8_t *afrag_base = static_cast<uint8_t *>(B->afrag->ptr);
 
     // per group: units, tasks, sync layout
     uint32_t sync_words_total = 0;
@@ -283,10 +283,14 @@ static int make_plan(Ctx *c, const p2g_batch_job *jobs, uint32_t njobs, bool use
         g.n_units = units;
         g.n_tasks = tasks;
         g.sync_words = kPassBands + bands;
-        g.n_jobs = njobs;
+        g.n_jobs = nitems;
         g.first = k == 0;
         g.afrag_entries = sp.afrag / 16;
-        plan_batch_group(c->num_cus, tasks, g.slice_hi - g.slice_lo, units, &g);
+        // [slice_lo, slice_hi) may hold the descriptors of items without
+        // checksums; the lanes skip them, and only checksummed slices count
+        uint32_t hashed = 0;
+        for (uint32_t q = sp.s_lo; q < sp.s_hi; q++) hashed += sjobs[q].slice_end - sjobs[q].slice0;
+        plan_batch_group(c->num_cus, tasks, hashed, units, &g);
         B->sync_off.push_back(sync_words_total);
         sync_words_total += (uint32_t)align_up(starts, 64);
     }
@@ -294,10 +298,10 @@ static int make_plan(Ctx *c, const p2g_batch_job *jobs, uint32_t njobs, bool use
     if (rc == P2G_OK) rc = upload(c, B->d_parts, parts, "batch staging table");
     if (rc == P2G_OK) rc = upload(c, B->d_pieces, pieces, "batch pieces");
     if (rc == P2G_OK) rc = upload(c, B->d_sjobs, sjobs, "batch slice jobs");
-    if (rc != P2G_OK) return bail(rc);
+    if (rc != P2G_OK) return rc;
     if (B->d_sync.reserve((size_t)sync_words_total * 4) != P2G_OK ||
-        B->d_done.reserve((size_t)(njobs + 1) * 8) != P2G_OK)
-        return bail(c->fail(P2G_ERR_OUT_OF_MEMORY, "batch sync words"));
+        B->d_done.reserve((size_t)(nitems + 1) * 8) != P2G_OK)
+        return c->fail(P2G_ERR_OUT_OF_MEMORY, "batch sync words");
     for (size_t k = 0; k < gl.size(); k++) {
         gl[k].pieces = static_cast<const BatchPiece *>(B->d_pieces.ptr) + spans[k].p_lo;
         gl[k].sjobs = static_cast<const BatchSlices *>(B->d_sjobs.ptr) + spans[k].s_lo;
@@ -305,19 +309,36 @@ static int make_plan(Ctx *c, const p2g_batch_job *jobs, uint32_t njobs, bool use
         gl[k].job_done = static_cast<unsigned long long *>(B->d_done.ptr);
     }
     B->groups = gl;
-    hipError_t e = hipEventCreateWithFlags(&B->done, hipEventDisableTiming);
-    if (e != hipSuccess) return bail(c->fail(P2G_ERR_DEVICE, "hipEventCreate", e));
+    hipError_t e = hipSuccess;
+    if (!B->done) e = hipEventCreateWithFlags(&B->done, hipEventDisableTiming);
+    if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "hipEventCreate", e);
     if (use_graph) {
-        e = hipStreamCreateWithFlags(&B->cap, hipStreamNonBlocking);
+        if (!B->cap) e = hipStreamCreateWithFlags(&B->cap, hipStreamNonBlocking);
         if (e == hipSuccess) e = hipStreamBeginCapture(B->cap, hipStreamCaptureModeThreadLocal);
-        if (e != hipSuccess) return bail(c->fail(P2G_ERR_DEVICE, "begin graph capture", e));
+        if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "begin graph capture", e);
         rc = B->record(B->cap);
         hipGraph_t graph = nullptr;
         e = hipStreamEndCapture(B->cap, &graph);
         B->graph = graph;
-        if (rc != P2G_OK) return bail(rc);
+        if (rc != P2G_OK) return rc;
         if (e == hipSuccess) e = hipGraphInstantiate(&B->exec, B->graph, nullptr, nullptr, 0);
-        if (e != hipSuccess) return bail(c->fail(P2G_ERR_DEVICE, "batch graph", e));
+        if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "batch graph", e);
+    }
+    return P2G_OK;
+}
+
+static int make_plan(Ctx *c, const p2g_batch_job *jobs, uint32_t njobs, bool use_graph,
+                     BatchPlan **out) {
+    *out = nullptr;
+    if (njobs == 0 || !jobs) return c->fail(P2G_ERR_INVALID_ARGUMENT, "empty batch");
+    std::vector<PlanItem> items(njobs);
+    for (uint32_t j = 0; j < njobs; j++) items[j] = {&jobs[j], 0, jobs[j].recovery_count, true};
+    BatchPlan *B = new (std::nothrow) BatchPlan();
+    if (!B) return c->fail(P2G_ERR_OUT_OF_MEMORY, "batch plan");
+    const int rc = build_plan(c, items, use_graph, B);
+    if (rc != P2G_OK) {
+        delete B;
+        return rc;
     }
     *out = B;
     return P2G_OK;

@@ -1,0 +1,39 @@
+// p2g_batch.hpp -- the batched create plan (p2g_batch.cpp), shared with the
+// serving loop (p2g_serve.cpp).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "p2g_context.hpp"
+
+namespace p2g {
+
+// Tables and launch sequence of one batched run: descriptors of every
+// item's slices (partial slices redirected to zero-padded staging copies),
+// pieces, slice jobs, and per launch group its sync words; the A fragments
+// live in `afrag`, the plan's own buffer or one shared by plans whose runs
+// are ordered on one stream.
+struct BatchPlan {
+    Ctx *c = nullptr;
+    uint32_t njobs = 0;  // items: completion stamps are per item
+    uint32_t n_slices = 0, n_parts = 0, max_part_S = 0;
+    DevBuf d_descs, d_tb, d_staging, d_parts, d_pieces, d_sjobs, d_sync, d_done;
+    DevBuf own_afrag;
+    DevBuf *afrag = &own_afrag;  // or a buffer shared by plans run in one stream order
+    std::vector<BatchGroupLaunch> groups;
+    std::vector<uint32_t> sync_off;  // word offset of each group's sync words
+    hipStream_t cap = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t done = nullptr;
+    bool launched = false;
+    std::vector<unsigned long long> h_done;  // completion stamps of the last launch, then start
+
+    int record(hipStream_t st);
+    void drop_graph();
+    ~BatchPlan();
+};
+
+}  // namespace p2g

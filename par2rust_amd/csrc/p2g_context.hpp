@@ -101,9 +101,17 @@ struct Job {
 
 int check_slice_size(Ctx *c, uint64_t S);
 
-// Batched create plans (p2g_batch.cpp): wait for the last launch and read its
-// flags and stamps; free.
-int wait_plan(BatchPlan *b);
+// Batched create plans (p2g_batch.cpp).  An item is rows [r0, r1) of a job
+// and, with `slices`, the job's slice checksums.
+struct PlanItem {
+    const p2g_batch_job *job;
+    uint32_t r0, r1;
+    bool slices;
+};
+int check_batch_job(Ctx *c, const p2g_batch_job &J, std::vector<SliceDesc> &descs);
+int build_plan(Ctx *c, const std::vector<PlanItem> &items, bool use_graph, BatchPlan *b);
+int launch_plan(BatchPlan *b, hipStream_t st);
+int wait_plan(BatchPlan *b);  // waits for the last launch, reads its flags and stamps
 void free_batch_plan(BatchPlan *b);
 
 // Appends the descriptors of every slice of `files` (file order, slice order).

@@ -1623,6 +1623,7 @@ __global__ __launch_bounds__(kMfThreads, 1) void k_create_batch(BatchArgs a) {
         for (uint32_t sl = a.slice_lo + (hb * 2 + wave) * 64 + lane; sl < a.slice_hi; sl += n_lanes) {
             while (sl >= a.sjobs[sj].slice_end) sj++;
             const BatchSlices J = a.sjobs[sj];
+            if (sl < J.slice0) continue;  // descriptors of an item without slice checksums
             const uint4 *tb = a.tail_blocks + 4 * (size_t)sl;
             uint8_t *mo = J.md5 + (size_t)(sl - J.slice0) * 16;
             uint32_t *co = J.crc + (sl - J.slice0);

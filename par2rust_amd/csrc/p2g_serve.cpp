@@ -1,0 +1,367 @@
+// p2g_serve.cpp -- serving: a request loop that admits create jobs by HBM
+// footprint, keeps their source and outputs resident, and runs them in
+// continuous batching steps.
+//
+// A request brings its files (host bytes to upload, or a seed per file for
+// the synthetic xorshift64 stream written on the device) and its slice size,
+// recovery count and set id.  Requests are admitted in arrival order while
+// their footprint (files, recovery blocks, digests) fits the budget; they
+// take their memory from one pool allocated at open.
+//
+// A step is one batched plan (build_plan) over every admitted job with work
+// left: whole jobs when their estimated device time fits the step target,
+// otherwise a piece of recovery rows sized to the target (and the slice
+// checksums with the first piece).  Steps run in order on one stream, at
+// most two queued at a time (the next one is planned while the current one
+// runs), so requests that arrive while the device is busy join the next
+// step and a long job never holds the others back for more than a step.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <thread>
+#include <vector>
+
+#include "p2g_batch.hpp"
+
+namespace p2g {
+
+constexpr double kServeOpsPerMs = 4.0e12;  // device rate assumed when sizing pieces (4 PF)
+
+// First-fit allocator over one device allocation, 256-byte granules.
+struct Pool {
+    uint8_t *base = nullptr;
+    uint64_t size = 0, used = 0;
+    std::map<uint64_t, uint64_t> free_;  // offset -> length
+    bool alloc(uint64_t n, uint64_t *off) {
+        n = (n + 255) / 256 * 256;
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second < n) continue;
+            *off = it->first;
+            const uint64_t rest = it->second - n, at = it->first + n;
+            free_.erase(it);
+            if (rest) free_[at] = rest;
+            used += n;
+            return true;
+        }
+        return false;
+    }
+    void release(uint64_t off, uint64_t n) {
+        n = (n + 255) / 256 * 256;
+        used -= n;
+        auto it = free_.emplace(off, n).first;
+        auto nx = std::next(it);
+        if (nx != free_.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            free_.erase(nx);
+        }
+        if (it != free_.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+};
+
+struct Request {
+    enum State { kQueued, kActive, kDone };
+    uint64_t id = 0;
+    State state = kQueued;
+    uint64_t S = 0;
+    uint32_t R = 0, n = 0;
+    std::vector<uint64_t> lens, seeds;
+    std::vector<const uint8_t *> host;
+    uint8_t set_id[16] = {};
+    uint64_t footprint = 0, off = 0;
+    std::vector<p2g_dev_file> files;
+    p2g_batch_job job{};
+    uint32_t rows_assigned = 0, rows_done = 0;
+    bool slices_assigned = false, slices_done = false;
+    double ms_per_row = 0;
+    double t_submit = 0, t_admit = 0, t_done = 0;  // ms on the server clock
+};
+
+struct Step {
+    BatchPlan plan;
+    std::vector<std::pair<Request *, PlanItem>> items;
+    bool busy = false;
+};
+
+struct Server {
+    Ctx *c = nullptr;
+    Pool pool;
+    double step_ms = 100.0;
+    uint64_t next_id = 1;
+    std::deque<Request *> queued;
+    std::vector<Request *> active;
+    std::map<uint64_t, std::unique_ptr<Request>> all;
+    std::deque<uint64_t> finished;  // done, not yet reported by poll
+    Step steps[2];
+    std::deque<int> inflight;  // step indices in launch order
+    DevBuf afrag;              // shared by both steps' plans (one stream order)
+    hipStream_t stream = nullptr;
+    std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+
+    double now_ms() const {
+        return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+    int admit();
+    int retire(int k);
+    int launch_step(int k);
+    ~Server();
+};
+
+Server::~Server() {
+    if (c) hipSetDevice(c->device);
+    if (stream) hipStreamSynchronize(stream);
+    for (Step &s : steps) s.plan.afrag = &s.plan.own_afrag;
+    afrag.release();
+    if (pool.base) hipFree(pool.base);
+    if (stream) hipStreamDestroy(stream);
+}
+
+static uint64_t a256(uint64_t x) { return (x + 255) / 256 * 256; }
+
+int Server::admit() {
+    while (!queued.empty()) {
+        Request *q = queued.front();
+        uint64_t off = 0;
+        if (!pool.alloc(q->footprint, &off)) break;  // arrival order: wait for memory
+        queued.pop_front();
+        q->off = off;
+        uint8_t *p = pool.base + off;
+        q->files.resize(q->lens.size());
+        for (size_t f = 0; f < q->lens.size(); f++) {
+            q->files[f] = {p, q->lens[f]};
+            hipError_t e = hipSuccess;
+            if (!q->host.empty() && q->host[f])
+                e = hipMemcpyAsync(p, q->host[f], q->lens[f], hipMemcpyHostToDevice, stream);
+            else if (q->lens[f])
+                e = p2g_dev_fill_xorshift64(reinterpret_cast<p2g_ctx *>(c), p, q->lens[f],
+                                            q->seeds.empty() ? 0 : q->seeds[f], stream) == P2G_OK
+                        ? hipSuccess
+                        : hipErrorUnknown;
+            if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "request source upload", e);
+            p += a256(q->lens[f]);
+        }
+        q->job.files = q->files.data();
+        q->job.nfiles = (uint32_t)q->files.size();
+        q->job.slice_size = q->S;
+        q->job.recovery_count = q->R;
+        memcpy(q->job.set_id, q->set_id, 16);
+        q->job.rec = p;
+        p += a256((uint64_t)q->R * q->S);
+        q->job.slice_md5 = p;
+        p += a256(16ull * q->n);
+        q->job.slice_crc = reinterpret_cast<uint32_t *>(p);
+        p += a256(4ull * q->n);
+        q->job.pkt_md5 = p;
+        q->state = Request::kActive;
+        q->t_admit = now_ms();
+        active.push_back(q);
+    }
+    return P2G_OK;
+}
+
+int Server::launch_step(int k) {
+    Step &st = steps[k];
+    st.items.clear();
+    std::vector<PlanItem> items;
+    for (Request *q : active) {
+        if (q->rows_assigned == q->R && q->slices_assigned) continue;
+        const uint32_t left = q->R - q->rows_assigned;
+        uint32_t take = left;
+        if (left && q->ms_per_row * left > step_ms) {
+            // a piece of whole row blocks sized to the step target
+            const uint32_t rows = (uint32_t)(step_ms / q->ms_per_row);
+            take = std::min(left, std::max<uint32_t>(16, rows / 16 * 16));
+        }
+        PlanItem it{&q->job, q->rows_assigned, q->rows_assigned + take, !q->slices_assigned};
+        q->rows_assigned += take;
+        q->slices_assigned = true;
+        st.items.push_back({q, it});
+        items.push_back(it);
+    }
+    if (items.empty()) return P2G_OK;
+    int rc = build_plan(c, items, false, &st.plan);
+    if (rc == P2G_OK) rc = launch_plan(&st.plan, stream);
+    if (rc != P2G_OK) return rc;
+    st.busy = true;
+    inflight.push_back(k);
+    return P2G_OK;
+}
+
+int Server::retire(int k) {
+    Step &st = steps[k];
+    const int rc = wait_plan(&st.plan);
+    st.busy = false;
+    const double t = now_ms();
+    for (auto &pr : st.items) {
+        Request *q = pr.first;
+        q->rows_done += pr.second.r1 - pr.second.r0;
+        if (pr.second.slices) q->slices_done = true;
+        if (q->rows_done == q->R && q->slices_done && q->state == Request::kActive) {
+            q->state = Request::kDone;
+            q->t_done = t;
+            finished.push_back(q->id);
+            active.erase(std::find(active.begin(), active.end(), q));
+        }
+    }
+    st.items.clear();
+    return rc;
+}
+
+}  // namespace p2g
+
+using namespace p2g;
+
+extern "C" {
+
+int p2g_server_open(p2g_ctx *ctx, uint64_t hbm_budget, double step_ms, p2g_server **out) {
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (!c || !out || hbm_budget == 0) return P2G_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    hipSetDevice(c->device);
+    Server *s = new (std::nothrow) Server();
+    if (!s) return c->fail(P2G_ERR_OUT_OF_MEMORY, "server");
+    s->c = c;
+    if (step_ms > 0) s->step_ms = step_ms;
+    s->pool.size = a256(hbm_budget);
+    void *p = nullptr;
+    hipError_t e = hipMalloc(&p, s->pool.size);
+    if (e != hipSuccess) {
+        delete s;
+        return c->fail(P2G_ERR_OUT_OF_MEMORY, "server HBM pool", e);
+    }
+    s->pool.base = static_cast<uint8_t *>(p);
+    s->pool.free_[0] = s->pool.size;
+    e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete s;
+        return c->fail(P2G_ERR_DEVICE, "server stream", e);
+    }
+    for (Step &st : s->steps) st.plan.afrag = &s->afrag;
+    *out = reinterpret_cast<p2g_server *>(s);
+    return P2G_OK;
+}
+
+int p2g_server_submit(p2g_server *srv, const p2g_request *r, uint64_t *id_out) {
+    Server *s = reinterpret_cast<Server *>(srv);
+    if (!s || !r || !id_out) return P2G_ERR_INVALID_ARGUMENT;
+    Ctx *c = s->c;
+    if (r->nfiles && !r->file_lens) return c->fail(P2G_ERR_INVALID_ARGUMENT, "null file lengths");
+    int rc = check_slice_size(c, r->slice_size);
+    if (rc != P2G_OK) return rc;
+    if (r->recovery_count > P2G_MAX_RECOVERY_BLOCKS)
+        return c->fail(P2G_ERR_TOO_MANY_RECOVERY_BLOCKS, "recovery count above 65535");
+    auto q = std::make_unique<Request>();
+    q->S = r->slice_size;
+    q->R = r->recovery_count;
+    uint64_t n = 0, bytes = 0;
+    for (uint32_t f = 0; f < r->nfiles; f++) {
+        q->lens.push_back(r->file_lens[f]);
+        n += (r->file_lens[f] + q->S - 1) / q->S;
+        bytes += a256(r->file_lens[f]);
+        if (r->host_data) q->host.push_back(r->host_data[f]);
+        q->seeds.push_back(r->fill_seeds ? r->fill_seeds[f] : 0);
+    }
+    if (n == 0) return c->fail(P2G_ERR_INVALID_ARGUMENT, "a request has no slices");
+    if (n > kMaxInputs) return c->fail(P2G_ERR_TOO_MANY_INPUT_BLOCKS, "more than 32768 input slices");
+    q->n = (uint32_t)n;
+    memcpy(q->set_id, r->set_id, 16);
+    q->footprint = bytes + a256((uint64_t)q->R * q->S) + a256(16 * n) + a256(4 * n) + a256(16ull * q->R);
+    if (q->footprint > s->pool.size)
+        return c->fail(P2G_ERR_OUT_OF_MEMORY, "request larger than the server's HBM budget");
+    q->ms_per_row = 256.0 * (double)n * (double)q->S / kServeOpsPerMs;
+    q->id = s->next_id++;
+    q->t_submit = s->now_ms();
+    *id_out = q->id;
+    s->queued.push_back(q.get());
+    s->all[q->id] = std::move(q);
+    return P2G_OK;
+}
+
+int p2g_server_poll(p2g_server *srv, double wait_ms, uint64_t *done_ids, uint32_t cap,
+                    uint32_t *n_done) {
+    Server *s = reinterpret_cast<Server *>(srv);
+    if (!s || (cap && !done_ids)) return P2G_ERR_INVALID_ARGUMENT;
+    hipSetDevice(s->c->device);
+    const double until = s->now_ms() + std::max(0.0, wait_ms);
+    int rc = P2G_OK;
+    for (;;) {
+        // retire finished steps, in launch order
+        while (!s->inflight.empty()) {
+            const int k = s->inflight.front();
+            const hipError_t q = hipEventQuery(s->steps[k].plan.done);
+            if (q == hipErrorNotReady) break;
+            s->inflight.pop_front();
+            const int r = s->retire(k);
+            if (rc == P2G_OK) rc = r;
+        }
+        if (rc == P2G_OK) rc = s->admit();
+        // keep one step running and the next one queued behind it
+        while (rc == P2G_OK && s->inflight.size() < 2) {
+            int k = !s->steps[0].busy ? 0 : !s->steps[1].busy ? 1 : -1;
+            if (k < 0) break;
+            const size_t before = s->inflight.size();
+            rc = s->launch_step(k);
+            if (s->inflight.size() == before) break;  // nothing to run
+        }
+        if (rc != P2G_OK || !s->finished.empty() || s->now_ms() >= until) break;
+        std::this_thread::sleep_for(std::chrono::microseconds(50));
+    }
+    uint32_t k = 0;
+    while (k < cap && !s->finished.empty()) {
+        done_ids[k++] = s->finished.front();
+        s->finished.pop_front();
+    }
+    if (n_done) *n_done = k;
+    return rc;
+}
+
+int p2g_server_result(p2g_server *srv, uint64_t id, p2g_batch_job *out, double times_ms[3]) {
+    Server *s = reinterpret_cast<Server *>(srv);
+    if (!s) return P2G_ERR_INVALID_ARGUMENT;
+    auto it = s->all.find(id);
+    if (it == s->all.end()) return s->c->fail(P2G_ERR_INVALID_ARGUMENT, "unknown request id");
+    Request *q = it->second.get();
+    if (q->state != Request::kDone) return s->c->fail(P2G_ERR_INVALID_ARGUMENT, "request not done");
+    if (out) *out = q->job;
+    if (times_ms) {
+        times_ms[0] = q->t_submit;
+        times_ms[1] = q->t_admit;
+        times_ms[2] = q->t_done;
+    }
+    return P2G_OK;
+}
+
+int p2g_server_release(p2g_server *srv, uint64_t id) {
+    Server *s = reinterpret_cast<Server *>(srv);
+    if (!s) return P2G_ERR_INVALID_ARGUMENT;
+    auto it = s->all.find(id);
+    if (it == s->all.end()) return s->c->fail(P2G_ERR_INVALID_ARGUMENT, "unknown request id");
+    Request *q = it->second.get();
+    if (q->state != Request::kDone) return s->c->fail(P2G_ERR_INVALID_ARGUMENT, "request not done");
+    s->pool.release(q->off, q->footprint);
+    s->all.erase(it);
+    return P2G_OK;
+}
+
+int p2g_server_stats(p2g_server *srv, uint32_t *queued, uint32_t *active, uint64_t *hbm_used) {
+    Server *s = reinterpret_cast<Server *>(srv);
+    if (!s) return P2G_ERR_INVALID_ARGUMENT;
+    if (queued) *queued = (uint32_t)s->queued.size();
+    if (active) *active = (uint32_t)s->active.size();
+    if (hbm_used) *hbm_used = s->pool.used;
+    return P2G_OK;
+}
+
+void p2g_server_close(p2g_server *srv) { delete reinterpret_cast<Server *>(srv); }
+
+}  // extern "C"

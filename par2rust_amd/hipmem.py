@@ -57,6 +57,14 @@ def copy_device(dst: "DeviceBuffer", src: "DeviceBuffer", nbytes: int) -> None:
     _check(_hip.hipMemcpy(dst.ptr, src.ptr, nbytes, D2D), "hipMemcpy D2D")
 
 
+def download_ptr(addr: int, nbytes: int) -> bytes:
+    """nbytes of device memory at a raw device address."""
+    out = ctypes.create_string_buffer(max(1, nbytes))
+    if nbytes:
+        _check(_hip.hipMemcpy(out, _P(addr), nbytes, D2H), "hipMemcpy D2H")
+    return out.raw[:nbytes]
+
+
 class DeviceBuffer:
     """`nbytes` of device memory (hipMalloc: 256-byte aligned)."""
 
