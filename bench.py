@@ -20,7 +20,9 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
+import struct
 import sys
 import time
 
@@ -168,6 +170,114 @@ def run_batch_mix(ctx, n_jobs: int, reps: int) -> dict:
     }
 
 
+def run_serving(ctx, seconds: float, rate: float, budget: int, verify: int = 4) -> dict:
+    """The serving run: requests of the batched job mix (synth.batch_mix,
+    seed 4001) arrive as a Poisson process of `rate` jobs/s for `seconds`,
+    plus one 32 GiB job (one file, slice size 1 MiB: 32768 slices, 3000
+    recovery blocks, seed 4000) at 0.5 s; every file is written on the device
+    by the server (seeded xorshift64).  The server admits requests by
+    footprint within `budget` bytes of HBM and runs continuous batching steps.
+    Latency of a request = time from its arrival to the poll that reports it
+    done.  A sample of `verify` mix jobs is recomputed alone
+    (p2g_dev_create_pass on the same device bytes) and compared."""
+    import numpy as np
+
+    from par2rust_amd import synth
+
+    n_max = int(rate * seconds * 1.5) + 16
+    mix = synth.batch_mix(n_max, seed=4001)
+    u = synth._Uniform(4002)
+    arrivals, t = [], 0.0
+    while True:
+        t += -math.log(1.0 - u()) / rate
+        if t >= seconds or len(arrivals) >= len(mix):
+            break
+        arrivals.append(t)
+    big = {"slice_size": 1 << 20, "recovery": 3000, "files": [(32 << 30, 4000)]}
+    reqs = [(a, m) for a, m in zip(arrivals, mix)]
+    reqs.append((0.5, big))
+    reqs.sort(key=lambda r: r[0])
+    t_open = time.perf_counter()
+    srv = ctx.server(budget)
+    log(f"bench: serving {len(reqs)} requests over {seconds}s (rate {rate:.1f}/s, HBM budget "
+        f"{budget / 1e9:.0f} GB, pool ready in {time.perf_counter() - t_open:.1f}s)")
+    sample = set(range(1, len(reqs), max(1, len(reqs) // max(1, verify))))
+    info, lat, keep, checked, same = {}, {}, [], 0, True
+    max_q, max_used, k = 0, 0, 0
+    src_bytes = sum(sz for _, m in reqs for sz, _ in m["files"])
+    t0 = time.perf_counter()
+    while len(lat) < len(reqs):
+        now = time.perf_counter() - t0
+        while k < len(reqs) and reqs[k][0] <= now:
+            arr, m = reqs[k]
+            sid = bytes((k * 13 + i) & 0xFF for i in range(16))
+            rid = srv.submit([sz for sz, _ in m["files"]], m["slice_size"], m["recovery"], sid,
+                             seeds=[sd for _, sd in m["files"]])
+            info[rid] = (k, arr, m, sid)
+            k += 1
+        wait = 5.0 if k >= len(reqs) else max(0.0, min(5.0, (reqs[k][0] - now) * 1e3))
+        for rid in srv.poll(wait):
+            kk, arr, m, sid = info[rid]
+            lat[rid] = time.perf_counter() - t0 - arr
+            if kk in sample and m is not big:
+                keep.append(rid)  # checked after the run, so the check does not share the GPU
+            else:
+                srv.release(rid)
+        st = srv.stats()
+        max_q, max_used = max(max_q, st["queued"]), max(max_used, st["hbm_used"])
+        if time.perf_counter() - t0 > seconds + 120:
+            raise RuntimeError("serving run did not finish")
+    wall = time.perf_counter() - t0
+    for rid in keep:
+        kk, arr, m, sid = info[rid]
+        same &= _same_alone(ctx, srv, rid, m, sid)
+        checked += 1
+        srv.release(rid)
+    srv.close()
+    big_rid = next(r for r, v in info.items() if v[2] is big)
+    small = [v for r, v in lat.items() if r != big_rid]
+    pct = lambda v, q: round(float(np.percentile(np.array(v) * 1e3, q)), 2)
+    out = {
+        "requests": len(reqs), "arrival_rate_jobs_per_s": round(rate, 1), "arrival_seconds": seconds,
+        "hbm_budget_bytes": budget, "source_bytes": src_bytes,
+        "source_GBps": round(src_bytes / wall / 1e9, 2), "wall_s": round(wall, 3),
+        "latency_ms": {"p50": pct(small, 50), "p99": pct(small, 99), "max": pct(small, 100)},
+        "big_job": {"bytes": 32 << 30, "slice_size": 1 << 20, "slices": 32768, "recovery": 3000,
+                    "arrival_s": 0.5, "latency_ms": round(lat[big_rid] * 1e3, 1)},
+        "max_queued": max_q, "max_hbm_used_bytes": max_used,
+        "bit_exact_vs_alone": {"jobs_checked": checked, "equal": bool(same)},
+        "latency": "arrival to completion reported by the server loop",
+    }
+    log(f"bench: serving done in {wall:.2f}s: {out['source_GBps']} GB/s, p50 "
+        f"{out['latency_ms']['p50']} ms, p99 {out['latency_ms']['p99']} ms, big job "
+        f"{out['big_job']['latency_ms']} ms, checked {checked} equal {same}")
+    return out
+
+
+def _same_alone(ctx, srv, rid, m, sid) -> bool:
+    """The served outputs of request rid equal p2g_dev_create_pass of the
+    same device bytes."""
+    from par2rust_amd import hipmem
+
+    import ctypes
+
+    import par2rust_amd as p2
+
+    job, _ = srv.result(rid)
+    S, R = m["slice_size"], m["recovery"]
+    df = ctypes.cast(job.files, ctypes.POINTER(p2._DevFile))
+    files = [(df[f].data, df[f].len) for f in range(job.nfiles)]
+    n = sum((L + S - 1) // S for _, L in files)
+    outs = [hipmem.DeviceBuffer(R * S), hipmem.DeviceBuffer(16 * n), hipmem.DeviceBuffer(4 * n),
+            hipmem.DeviceBuffer(16 * R)]
+    ctx.dev_create_pass(files, S, R, sid, *outs)
+    ctx.synchronize()
+    served = srv.download(rid)
+    alone = (outs[0].download(), outs[1].download(),
+             list(struct.unpack(f"<{n}I", outs[2].download())), outs[3].download())
+    return served == alone
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -180,6 +290,11 @@ def main() -> int:
     ap.add_argument("--batch-jobs", type=int, default=256,
                     help="jobs of the batched-create comparison (0 skips it)")
     ap.add_argument("--batch-reps", type=int, default=3)
+    ap.add_argument("--serve-seconds", type=float, default=10.0,
+                    help="arrival window of the serving run (0 skips it)")
+    ap.add_argument("--serve-load", type=float, default=0.5,
+                    help="arrival rate as a fraction of the measured batched jobs/s")
+    ap.add_argument("--serve-budget-gb", type=float, default=200.0)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run one 512 MiB device-to-device copy (the reference reading "
                          "for the FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -328,6 +443,10 @@ def main() -> int:
 
     if args.batch_jobs > 0:
         out["batched"] = run_batch_mix(ctx, args.batch_jobs, args.batch_reps)
+    if args.serve_seconds > 0:
+        cap = out["batched"]["batched_jobs_per_s"] if "batched" in out else 400.0
+        out["serving"] = run_serving(ctx, args.serve_seconds, args.serve_load * cap,
+                                     int(args.serve_budget_gb * 1e9))
 
     if not args.no_cpu_baseline:
         from oracle import oracle as O
