@@ -246,6 +246,32 @@ int p2g_server_release(p2g_server *s, uint64_t id);
 int p2g_server_stats(p2g_server *s, uint32_t *queued, uint32_t *active, uint64_t *hbm_used);
 void p2g_server_close(p2g_server *s);
 
+/* ---------------------------------------------------------------------
+ * Autotuner (no reference counterpart; the reference picks its SIMD kernel
+ * by CPU feature, src/galois_simd.rs:605-636).  Per (N, R, slice size)
+ * bucket -- the bit lengths of the three -- the encode variant
+ * (P2G_VARIANT_MFMA: matrix-core fused pass; P2G_VARIANT_BITSLICE: the
+ * bit-sliced VALU kernels) and the column band width of the fused pass that
+ * ran fastest.  p2g_dev_create_pass and the batched planner use a tuned
+ * bucket's choice; with P2G_AUTOTUNE=1 a bucket is tuned on its first
+ * p2g_dev_create_pass call (on that call's data); P2G_TUNE_CACHE names a
+ * file the cache is loaded from at p2g_open and saved to after each tuning.
+ * ------------------------------------------------------------------- */
+enum { P2G_VARIANT_MFMA = 0, P2G_VARIANT_BITSLICE = 1 };
+
+/* The bucket id of a shape: nbits(slices) | nbits(R) << 8 | nbits(S) << 16. */
+uint32_t p2g_tune_bucket(uint64_t slices, uint32_t recovery_count, uint64_t slice_size);
+/* Tunes the bucket of (slices, recovery_count, slice_size) on synthetic
+ * input of that shape and caches the choice; the winner's variant, band
+ * width (0: default) and time in ms are returned when the pointers are set. */
+int p2g_autotune(p2g_ctx *ctx, uint32_t slices, uint32_t recovery_count, uint64_t slice_size,
+                 int *variant_out, uint32_t *band_cols_out, double *ms_out);
+/* The choice p2g_dev_create_pass would use for the shape now. */
+int p2g_tune_get(p2g_ctx *ctx, uint32_t slices, uint32_t recovery_count, uint64_t slice_size,
+                 int *variant_out, uint32_t *band_cols_out);
+int p2g_tune_save(p2g_ctx *ctx, const char *path);
+int p2g_tune_load(p2g_ctx *ctx, const char *path);
+
 /* Synthetic input for benchmarks and tests (no reference counterpart):
  * fills nbytes of dst (device, 16-byte aligned) with the xorshift64 stream
  * from `seed` (x ^= x << 13; x ^= x >> 7; x ^= x << 17; each new state

@@ -46,7 +46,10 @@ EXPORTED_SYMBOLS = [
     "p2g_batch_free", "p2g_dev_create_batch", "p2g_dev_fill_xorshift64",
     "p2g_server_open", "p2g_server_submit", "p2g_server_poll", "p2g_server_result",
     "p2g_server_release", "p2g_server_stats", "p2g_server_close",
+    "p2g_tune_bucket", "p2g_autotune", "p2g_tune_get", "p2g_tune_save", "p2g_tune_load",
 ]
+
+VARIANT_MFMA, VARIANT_BITSLICE = 0, 1
 
 
 class Par2Error(RuntimeError):
@@ -147,6 +150,16 @@ def _load() -> ctypes.CDLL:
                                             ctypes.POINTER(ctypes.c_uint32),
                                             ctypes.POINTER(ctypes.c_uint64)]),
         "p2g_server_close": (None, [P]),
+        "p2g_tune_bucket": (ctypes.c_uint32, [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint64]),
+        "p2g_autotune": (ctypes.c_int, [P, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64,
+                                        ctypes.POINTER(ctypes.c_int),
+                                        ctypes.POINTER(ctypes.c_uint32),
+                                        ctypes.POINTER(ctypes.c_double)]),
+        "p2g_tune_get": (ctypes.c_int, [P, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64,
+                                        ctypes.POINTER(ctypes.c_int),
+                                        ctypes.POINTER(ctypes.c_uint32)]),
+        "p2g_tune_save": (ctypes.c_int, [P, ctypes.c_char_p]),
+        "p2g_tune_load": (ctypes.c_int, [P, ctypes.c_char_p]),
         "p2g_run_create": (ctypes.c_int, [P, ctypes.c_char_p, ctypes.c_uint64, ctypes.c_uint32,
                                           ctypes.POINTER(_Scheme), ctypes.c_uint32,
                                           ctypes.POINTER(ctypes.c_char_p),
@@ -536,6 +549,25 @@ class Context:
         self._check(lib.p2g_dev_fill_xorshift64(self.h, ctypes.c_void_p(_dev_ptr(dst)), nbytes,
                                                 seed, ctypes.c_void_p(stream) if stream else None))
 
+    def autotune(self, slices: int, recovery_count: int, slice_size: int) -> dict:
+        """Tunes the bucket of the shape on synthetic input (p2g_autotune)."""
+        v, b, ms = ctypes.c_int(), ctypes.c_uint32(), ctypes.c_double()
+        self._check(lib.p2g_autotune(self.h, slices, recovery_count, slice_size, ctypes.byref(v),
+                                     ctypes.byref(b), ctypes.byref(ms)))
+        return {"variant": v.value, "band_cols": b.value, "ms": ms.value}
+
+    def tune_get(self, slices: int, recovery_count: int, slice_size: int) -> dict:
+        v, b = ctypes.c_int(), ctypes.c_uint32()
+        self._check(lib.p2g_tune_get(self.h, slices, recovery_count, slice_size, ctypes.byref(v),
+                                     ctypes.byref(b)))
+        return {"variant": v.value, "band_cols": b.value}
+
+    def tune_save(self, path: str):
+        self._check(lib.p2g_tune_save(self.h, os.fsencode(path)))
+
+    def tune_load(self, path: str):
+        self._check(lib.p2g_tune_load(self.h, os.fsencode(path)))
+
     def server(self, hbm_budget: int, step_ms: float = 0.0) -> Server:
         return Server(self, hbm_budget, step_ms)
 
@@ -569,6 +601,11 @@ class Context:
         if rc != P2G_OK:
             raise Par2Error(rc, lib.p2g_last_error(self.h).decode(errors="replace"))
         return [s.decode() for s in out.raw.split(b"\0") if s][: 1 + 65536]
+
+
+def tune_bucket(slices: int, recovery_count: int, slice_size: int) -> int:
+    """The autotuner's bucket id of a shape (p2g_tune_bucket)."""
+    return lib.p2g_tune_bucket(slices, recovery_count, slice_size)
 
 
 def version() -> str:

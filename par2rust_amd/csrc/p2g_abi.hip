@@ -113,6 +113,15 @@ int Ctx::open(int dev) {
     if (e != hipSuccess || cus <= 0) return fail(P2G_ERR_DEVICE, "compute unit count", e);
     num_cus = (uint32_t)cus;
     if (const char *v = getenv("P2G_BAND_COLS")) band_cols = (uint32_t)strtoul(v, nullptr, 10);
+    if (const char *v = getenv("P2G_AUTOTUNE")) autotune = atoi(v) != 0;
+    if (const char *v = getenv("P2G_TUNE_CACHE")) {
+        tune_path = v;
+        if (FILE *f = fopen(v, "r")) {  // a missing file is created at the first tuning
+            fclose(f);
+            int rc = tune_load(this, v);
+            if (rc != P2G_OK) return rc;
+        }
+    }
     if (const char *v = getenv("P2G_ENCODE")) {
         if (!strcmp(v, "bitslice")) encode_variant = kEncodeBitslice;
         else if (!strcmp(v, "mfma")) encode_variant = kEncodeMfma;
@@ -221,7 +230,8 @@ int upload_descs(Ctx *c, DescSlot &slot, const std::vector<SliceDesc> &descs, hi
 
 int dev_encode(Ctx *c, const SliceDesc *d_descs, uint32_t n, uint64_t S, uint32_t first_input,
                uint32_t total_inputs, uint32_t R, uint8_t *d_rec, int accumulate,
-               hipStream_t st) {
+               hipStream_t st, int variant) {
+    if (variant < 0) variant = c->encode_variant;
     if (R > P2G_MAX_RECOVERY_BLOCKS)
         return c->fail(P2G_ERR_TOO_MANY_RECOVERY_BLOCKS, "recovery count above 65535");
     if (total_inputs > kMaxInputs)
@@ -239,7 +249,7 @@ int dev_encode(Ctx *c, const SliceDesc *d_descs, uint32_t n, uint64_t S, uint32_
     }
     const uint16_t *dlog = static_cast<const uint16_t *>(c->d_base_logs.ptr) + first_input;
     const uint16_t *dalog = static_cast<const uint16_t *>(c->d_antilog.ptr);
-    if (c->encode_variant == kEncodeBitslice) {
+    if (variant == kEncodeBitslice) {
         // Inputs go in chunks so the coefficient matrices (32 B per (r, i))
         // stay within kMatsBudget.
         uint64_t ch = kMatsBudget / ((uint64_t)R * 32);
@@ -488,12 +498,15 @@ int p2g_pass_timing(p2g_ctx *ctx, double *ms_out) {
     return P2G_OK;
 }
 
-int p2g_dev_create_pass(p2g_ctx *ctx, const p2g_dev_file *files, uint32_t nfiles,
-                        uint64_t slice_size, uint32_t recovery_count, const uint8_t set_id[16],
-                        uint8_t *rec, uint8_t *slice_md5_out, uint32_t *slice_crc_out,
-                        uint8_t *pkt_md5_out, void *stream) {
-    Ctx *c = reinterpret_cast<Ctx *>(ctx);
-    if (!c) return P2G_ERR_INVALID_ARGUMENT;
+}  // extern "C"
+
+namespace p2g {
+
+// p2g_dev_create_pass with an explicit encode variant and band width.
+int create_pass_impl(Ctx *c, const p2g_dev_file *files, uint32_t nfiles, uint64_t slice_size,
+                     uint32_t recovery_count, const uint8_t set_id[16], uint8_t *rec,
+                     uint8_t *slice_md5_out, uint32_t *slice_crc_out, uint8_t *pkt_md5_out,
+                     hipStream_t st, int variant, uint32_t band_cols) {
     const uint32_t R = recovery_count;
     if ((nfiles && !files) || !set_id || !slice_md5_out || !slice_crc_out ||
         (R && (!rec || !pkt_md5_out)))
@@ -514,7 +527,6 @@ int p2g_dev_create_pass(p2g_ctx *ctx, const p2g_dev_file *files, uint32_t nfiles
         return c->fail(P2G_ERR_TOO_MANY_INPUT_BLOCKS, "more than 32768 input slices");
     const uint32_t n = (uint32_t)descs.size();
     const uint32_t S = (uint32_t)slice_size;
-    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c->stream;
     rc = c->scratch_acquire(Ctx::kScratchEnc, st);
     if (rc != P2G_OK) return rc;
     // Partial slices (the last one of each file) are staged into zero-padded
@@ -548,12 +560,12 @@ int p2g_dev_create_pass(p2g_ctx *ctx, const p2g_dev_file *files, uint32_t nfiles
     hipError_t e;
 
     PassPlan plan;
-    const bool fused = c->encode_variant == kEncodeMfma &&
-                       plan_create_pass(n, R, S, c->num_cus, c->band_cols, &plan) &&
+    const bool fused = variant == kEncodeMfma &&
+                       plan_create_pass(n, R, S, c->num_cus, band_cols, &plan) &&
                        mfma_afrag_bytes(n, R) <= kPassAfragBudget;
     if (!fused) {
         // The same results from the separate kernels, in stream order.
-        rc = dev_encode(c, dd, n, slice_size, 0, n, R, rec, 0, st);
+        rc = dev_encode(c, dd, n, slice_size, 0, n, R, rec, 0, st, variant);
         if (rc != P2G_OK) return rc;
         if (n) {
             e = launch_desc_tails(dd, n, tb, st);
@@ -580,6 +592,34 @@ int p2g_dev_create_pass(p2g_ctx *ctx, const p2g_dev_file *files, uint32_t nfiles
     if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "create pass kernels", e);
     c->pass_pending = true;
     return c->scratch_release(Ctx::kScratchEnc, st);
+}
+
+}  // namespace p2g
+
+extern "C" {
+
+int p2g_dev_create_pass(p2g_ctx *ctx, const p2g_dev_file *files, uint32_t nfiles,
+                        uint64_t slice_size, uint32_t recovery_count, const uint8_t set_id[16],
+                        uint8_t *rec, uint8_t *slice_md5_out, uint32_t *slice_crc_out,
+                        uint8_t *pkt_md5_out, void *stream) {
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (!c) return P2G_ERR_INVALID_ARGUMENT;
+    if ((nfiles && !files) || slice_size == 0)
+        return create_pass_impl(c, files, nfiles, slice_size, recovery_count, set_id, rec,
+                                slice_md5_out, slice_crc_out, pkt_md5_out, c->stream,
+                                c->encode_variant, c->band_cols);  // reports the argument error
+    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c->stream;
+    uint64_t n = 0;
+    for (uint32_t f = 0; f < nfiles; f++) n += (files[f].len + slice_size - 1) / slice_size;
+    auto run = [&](int variant, uint32_t band_cols) {
+        return create_pass_impl(c, files, nfiles, slice_size, recovery_count, set_id, rec,
+                                slice_md5_out, slice_crc_out, pkt_md5_out, st, variant, band_cols);
+    };
+    TuneChoice ch;
+    if (tune_lookup(c, n, recovery_count, slice_size, &ch)) return run(ch.variant, ch.band_cols);
+    if (c->autotune && n && recovery_count)
+        return tune_and_run(c, n, recovery_count, slice_size, st, run, nullptr);
+    return run(c->encode_variant, c->band_cols);
 }
 
 // ---- host-buffer entry points ----------------------------------------

@@ -229,7 +229,12 @@ int build_plan(Ctx *c, const std::vector<PlanItem> &items, bool use_graph, Batch
             P.job = j;
             P.row_blocks = (Rp + 15) / 16;
             P.col_groups = (S + 511) / 512;
-            P.band_cols = std::max<uint32_t>(1, std::min(band_default, P.col_groups));
+            TuneChoice ch;
+            const uint32_t band = tune_lookup(c, n, J.recovery_count, S, &ch) &&
+                                          ch.variant == kEncodeMfma && ch.band_cols
+                                      ? ch.band_cols
+                                      : band_default;
+            P.band_cols = std::max<uint32_t>(1, std::min(band, P.col_groups));
             P.n_bands = 0;
             for (uint32_t cg = 0; cg < P.col_groups; P.n_bands++)
                 cg += next_band_width(cg, P.col_groups - cg, P.band_cols);

@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <functional>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -48,6 +50,13 @@ enum { kEncodeMfma = 0, kEncodeBitslice = 1 };
 
 struct BatchPlan;  // p2g_batch.cpp
 
+// Autotuner choice for one (N, R, slice size) bucket (p2g_tune.cpp).
+struct TuneChoice {
+    int variant = kEncodeMfma;
+    uint32_t band_cols = 0;  // 0: the default band width
+    double ms = 0;           // the winning time when it was measured
+};
+
 struct Ctx {
     int device = 0;
     int encode_variant = kEncodeMfma;
@@ -75,6 +84,9 @@ struct Ctx {
     DescSlot enc_descs;  // encode launches
     DescSlot sum_descs;  // slice checksum launches
     std::vector<BatchPlan *> pending_batches;  // p2g_dev_create_batch, until p2g_synchronize
+    std::map<uint32_t, TuneChoice> tune;       // autotuner cache by bucket
+    bool autotune = false;                     // P2G_AUTOTUNE=1: tune a bucket on its first call
+    std::string tune_path;                     // P2G_TUNE_CACHE: cache file, loaded at open
 
     int open(int dev);
     int side_stream(hipStream_t *out);
@@ -124,8 +136,27 @@ int upload_descs(Ctx *c, DescSlot &slot, const std::vector<SliceDesc> &descs, hi
 
 // rec (^)= encode of the n slices described by d_descs (device), whose global
 // input indices start at first_input.
+// variant: kEncodeMfma / kEncodeBitslice, or -1 for the context's.
 int dev_encode(Ctx *c, const SliceDesc *d_descs, uint32_t n, uint64_t S, uint32_t first_input,
                uint32_t total_inputs, uint32_t R, uint8_t *d_rec, int accumulate,
-               hipStream_t st);
+               hipStream_t st, int variant = -1);
+
+// p2g_dev_create_pass with an explicit encode variant and band width
+// (band_cols 0: the default).
+int create_pass_impl(Ctx *c, const p2g_dev_file *files, uint32_t nfiles, uint64_t slice_size,
+                     uint32_t recovery_count, const uint8_t set_id[16], uint8_t *rec,
+                     uint8_t *slice_md5_out, uint32_t *slice_crc_out, uint8_t *pkt_md5_out,
+                     hipStream_t st, int variant, uint32_t band_cols);
+
+// Autotuner (p2g_tune.cpp).  Buckets: bit lengths of N, R and S.
+uint32_t tune_bucket(uint64_t n, uint32_t R, uint64_t S);
+bool tune_lookup(Ctx *c, uint64_t n, uint32_t R, uint64_t S, TuneChoice *out);
+// Runs every candidate (variant x band width) through `run` on stream st,
+// timing each, stores the fastest for the bucket (and the cache file), and
+// leaves the outputs of a run of the fastest.
+int tune_and_run(Ctx *c, uint64_t n, uint32_t R, uint64_t S, hipStream_t st,
+                 const std::function<int(int, uint32_t)> &run, TuneChoice *out);
+int tune_load(Ctx *c, const char *path);
+int tune_save(Ctx *c, const char *path);
 
 }  // namespace p2g

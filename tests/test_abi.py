@@ -45,3 +45,12 @@ def test_status_codes_match_header():
     assert set(codes) == set(p2.ERRORS)
     assert codes[1] == "P2G_ERR_IO" and p2.ERRORS[1] == "Io"
     assert codes[8] == "P2G_ERR_TOO_MANY_INPUT_BLOCKS" and p2.ERRORS[8] == "TooManyInputBlocks"
+
+
+def test_tune_bucket_is_bit_lengths():
+    import par2rust_amd as p2
+
+    # the headline shape: 1027 slices (11 bits), 200 blocks (8), 524288 bytes (20)
+    assert p2.tune_bucket(1027, 200, 524288) == 11 | 8 << 8 | 20 << 16
+    assert p2.tune_bucket(1024, 128, 1 << 19) == p2.tune_bucket(2047, 255, (1 << 20) - 4)
+    assert p2.tune_bucket(2048, 200, 524288) != p2.tune_bucket(2047, 200, 524288)
