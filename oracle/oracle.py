@@ -53,6 +53,7 @@ def _load():
         "or_file_hashes": (None, [P, u64, P, P]),
         "or_file_id": (None, [P, u64, P, ctypes.c_size_t, P]),
         "or_encode": (ctypes.c_int, [P, P, u32, u64, u32, P]),
+        "or_encode_range": (ctypes.c_int, [P, P, u32, u64, u32, u32, P]),
         "or_encode_simd": (ctypes.c_int, [P, P, u32, u64, u32, P, u32, ctypes.c_int]),
         "or_simd_detect": (ctypes.c_int, []),
         "or_simd_dispatch_name": (ctypes.c_char_p, [ctypes.c_int]),
@@ -181,6 +182,16 @@ def encode(files: Sequence[bytes], S: int, R: int) -> bytes:
     if rc != 0:
         raise RuntimeError(f"or_encode rc={rc}")
     return out.raw[:R * S]
+
+
+def encode_rows(files: Sequence[bytes], S: int, first_exp: int, count: int) -> bytes:
+    """Recovery blocks of exponents first_exp .. first_exp + count - 1 only."""
+    ptrs, lens, keep = _files(files)
+    out = ctypes.create_string_buffer(max(1, count * S))
+    rc = L.or_encode_range(ptrs, lens, len(files), S, first_exp, count, out)
+    if rc != 0:
+        raise RuntimeError(f"or_encode_range rc={rc}")
+    return out.raw[:count * S]
 
 
 def encode_simd(files: Sequence[bytes], S: int, R: int, threads: int = 0,

@@ -293,6 +293,17 @@ void or_file_id(const uint8_t hash16k[16], uint64_t len, const uint8_t *name,
 
 int or_encode(const uint8_t *const *files, const uint64_t *lens, uint32_t nfiles,
               uint64_t slice_size, uint32_t recovery_count, uint8_t *rec_out) {
+    return or_encode_range(files, lens, nfiles, slice_size, 0, recovery_count, rec_out);
+}
+
+/* Recovery blocks of exponents first_exp .. first_exp + count - 1: block k
+ * uses coef = gf_pow(base_i, first_exp + k), as RsEncoder::new(first_exp, ..)
+ * (src/reedsolomon.rs:39-60) would; the reference's create always starts at
+ * exponent 0 (src/creator.rs:915), so block k here equals block
+ * first_exp + k of a create with more recovery blocks. */
+int or_encode_range(const uint8_t *const *files, const uint64_t *lens, uint32_t nfiles,
+                    uint64_t slice_size, uint32_t first_exp, uint32_t count, uint8_t *rec_out) {
+    const uint32_t recovery_count = count;
     uint64_t total = 0;
     for (uint32_t f = 0; f < nfiles; f++) total += or_slice_count(lens[f], slice_size);
     memset(rec_out, 0, (size_t)recovery_count * slice_size);
@@ -310,7 +321,7 @@ int or_encode(const uint8_t *const *files, const uint64_t *lens, uint32_t nfiles
             memcpy(slice, files[f] + off, have);
             memset(slice + have, 0, slice_size - have); /* src/creator.rs:79-85 */
             for (uint32_t r = 0; r < recovery_count; r++)
-                or_gf_mul_xor_scalar(or_coefficient(bases, r, (uint32_t)input_idx), slice,
+                or_gf_mul_xor_scalar(or_coefficient(bases, first_exp + r, (uint32_t)input_idx), slice,
                                      rec_out + (size_t)r * slice_size, slice_size);
         }
     }

@@ -84,6 +84,9 @@ void or_file_id(const uint8_t hash16k[16], uint64_t len, const uint8_t *name,
  * blocks of slice_size bytes, exponent-major. */
 int or_encode(const uint8_t *const *files, const uint64_t *lens, uint32_t nfiles,
               uint64_t slice_size, uint32_t recovery_count, uint8_t *rec_out);
+/* Blocks of exponents first_exp .. first_exp + count - 1 only. */
+int or_encode_range(const uint8_t *const *files, const uint64_t *lens, uint32_t nfiles,
+                    uint64_t slice_size, uint32_t first_exp, uint32_t count, uint8_t *rec_out);
 /* The reference's SIMD CPU path (GFNI-AVX512 > GFNI > AVX2 > table) with the
  * recovery buffers split in ceil(R/T) chunks over T threads and a barrier per
  * input slice.  force_dispatch < 0 picks by CPU features (or_simd_detect);

@@ -123,3 +123,27 @@ def batch_mix(n_jobs: int = 256, seed: int = 3001, max_inputs: int = 32768):
         jobs.append({"slice_size": S, "recovery": R,
                      "files": [(sz, seed + 64 * j + f + 1) for f, sz in enumerate(sizes)]})
     return jobs
+
+
+def config2_files(seed: int = 2001):
+    """The many-files workload: 64 files named f00.bin .. f63.bin, sizes
+    floor(2^(10 + 16 u)) (log-uniform in [1 KiB, 64 MiB]); the files at the
+    first 4 places of a shuffle (64 more draws, sorted) are zero-length and
+    the next 8 all-zero bytes; file k otherwise holds the xorshift64 stream
+    seeded with seed + 100 + k.  Returns [(name, size, kind, stream seed)]
+    with kind 'random', 'zero' or 'empty'."""
+    u = _Uniform(seed)
+    sizes = [int(2 ** (10 + 16 * u())) for _ in range(64)]
+    keys = [u() for _ in range(64)]
+    order = sorted(range(64), key=lambda k: keys[k])
+    empty, zero = set(order[:4]), set(order[4:12])
+    out = []
+    for k in range(64):
+        kind = "empty" if k in empty else "zero" if k in zero else "random"
+        out.append((f"f{k:02d}.bin", 0 if kind == "empty" else sizes[k], kind, seed + 100 + k))
+    return out
+
+
+def config2_bytes(entry) -> bytes:
+    name, size, kind, sd = entry
+    return xorshift64_bytes(size, sd) if kind == "random" else bytes(size)

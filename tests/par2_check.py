@@ -81,12 +81,15 @@ def check_set(files: dict, sources: list, S: int, R: int):
         first, count = map(int, vname.rsplit(".vol", 1)[1].split(".")[0].split("+"))
         assert exps == list(range(first, first + count))
     if R:
+        import numpy as np
+
         assert sorted(seen) == list(range(R))
-        parity = bytearray(S)
+        parity = np.zeros(S, dtype=np.uint8)
         for _, data in sources:
-            for off in range(0, len(data), S):
-                chunk = data[off:off + S]
-                for k in range(len(chunk)):
-                    parity[k] ^= chunk[k]
-        assert seen[0] == bytes(parity)
+            n = (len(data) + S - 1) // S
+            if n:
+                a = np.zeros(n * S, dtype=np.uint8)
+                a[:len(data)] = np.frombuffer(data, dtype=np.uint8)
+                parity ^= np.bitwise_xor.reduce(a.reshape(n, S), axis=0)
+        assert seen[0] == parity.tobytes()
     return seen

@@ -198,6 +198,32 @@ def test_encode_matches_textbook_definition():
     assert O.encode(files, S, R) == b"".join(expect)
 
 
+def test_encode_rows_are_rows_of_the_full_encode():
+    S = 64
+    files = [deterministic(21, 300), deterministic(22, 77)]
+    full = O.encode(files, S, 12)
+    for first, count in [(0, 12), (0, 1), (5, 3), (11, 1)]:
+        assert O.encode_rows(files, S, first, count) == full[first * S:(first + count) * S]
+
+
+def test_encode_rows_at_a_high_exponent_matches_textbook():
+    # exponent 999 (the last block of a 1000-block create), by square-and-multiply
+    S = 32
+    files = [deterministic(31, 70)]
+    slices = [files[0][o:o + S].ljust(S, b"\0") for o in range(0, 70, S)]
+    bases = O.input_bases(len(slices))
+    acc = bytearray(S)
+    for i, sl in enumerate(slices):
+        c, b, e = 1, bases[i], 999
+        while e:
+            if e & 1:
+                c = clmul_mod(c, b)
+            b = clmul_mod(b, b)
+            e >>= 1
+        acc = bytearray(x ^ y for x, y in zip(acc, scalar_mul_xor_bytes(c, sl)))
+    assert O.encode_rows(files, S, 999, 1) == bytes(acc)
+
+
 def test_encode_is_linear_and_row0_is_parity():
     S, R = 256, 6
     a = deterministic(11, 1000)
