@@ -11,9 +11,12 @@
 
 #include <algorithm>
 #include <cerrno>
+#include <condition_variable>
 #include <cstdio>
 #include <cstring>
+#include <deque>
 #include <fcntl.h>
+#include <mutex>
 #include <sys/stat.h>
 #include <thread>
 #include <unistd.h>
@@ -306,10 +309,10 @@ int scan(p2g_ctx *ctx, const std::string &path, const std::vector<uint8_t> &name
     return P2G_OK;
 }
 
-int run_create(p2g_ctx *ctx, const CreateOptions &opts, const std::vector<SourceFile> &sources,
-               std::vector<std::string> &written, Error &err) {
-    written.clear();
-    // validation order, src/creator.rs:193-217
+namespace {
+
+// run_create's validation, src/creator.rs:193-217 (after the scans).
+int check_create(const CreateOptions &opts, const std::vector<SourceFile> &sources, Error &err) {
     if (sources.empty()) return set_err(err, P2G_ERR_NO_INPUT_FILES, "no input files supplied");
     if (sources.size() > P2G_MAX_FILES)
         return set_err(err, P2G_ERR_TOO_MANY_FILES, "too many input files");
@@ -321,7 +324,7 @@ int run_create(p2g_ctx *ctx, const CreateOptions &opts, const std::vector<Source
         return set_err(err, P2G_ERR_INVALID_SLICE_SIZE, "invalid slice size " + std::to_string(S));
     uint64_t total_blocks = 0, total_bytes = 0;
     for (auto &s : sources) {
-        total_blocks += s.slice_checksums.size();
+        total_blocks += (s.length + S - 1) / S;
         total_bytes += s.length;
     }
     if (total_blocks > P2G_MAX_INPUT_BLOCKS) {
@@ -332,20 +335,31 @@ int run_create(p2g_ctx *ctx, const CreateOptions &opts, const std::vector<Source
                            std::to_string(S) + "; retry with slice_size >= " +
                            std::to_string(suggested));
     }
+    return P2G_OK;
+}
 
-    // Main packet, src/packet/main_packet.rs:25-42: file ids sorted.
+// set_id: MD5 of the main packet body, src/packet/main_packet.rs:25-42 (file
+// ids sorted).
+void main_body(uint64_t S, const std::vector<SourceFile> &sources, std::vector<uint8_t> &body,
+               uint8_t set_id[16]) {
     std::vector<std::array<uint8_t, 16>> ids(sources.size());
     for (size_t f = 0; f < sources.size(); f++) memcpy(ids[f].data(), sources[f].file_id, 16);
     std::sort(ids.begin(), ids.end());
-    std::vector<uint8_t> body;
+    body.clear();
     put_le64(body, S);
     put_le32(body, (uint32_t)sources.size());
     for (auto &id : ids) body.insert(body.end(), id.begin(), id.end());
-    uint8_t set_id[16];
     HostMd5::digest(body.data(), body.size(), set_id);
-    std::vector<uint8_t> critical;
+}
+
+// The critical packets: Main, FileDesc + IFSC per source in caller order
+// (src/creator.rs:227-230), Creator.
+void make_critical(uint64_t S, const std::vector<SourceFile> &sources, uint8_t set_id[16],
+                   std::vector<uint8_t> &critical) {
+    std::vector<uint8_t> body;
+    main_body(S, sources, body, set_id);
+    critical.clear();
     append_packet(critical, set_id, kTypeMain, body);
-    // FileDesc + IFSC per source in caller order, src/creator.rs:227-230
     for (auto &s : sources) {
         body.clear();  // src/packet/file_desc.rs:17-28
         body.insert(body.end(), s.file_id, s.file_id + 16);
@@ -366,8 +380,59 @@ int run_create(p2g_ctx *ctx, const CreateOptions &opts, const std::vector<Source
     body.assign(kCreatorString, kCreatorString + sizeof(kCreatorString) - 1);
     body.resize((body.size() + 3) & ~size_t(3), 0);  // src/packet/creator.rs:8-14
     append_packet(critical, set_id, kTypeCreator, body);
+}
 
-    int rc = write_atomic(opts.output, {&critical}, {}, err);
+// Volumes, src/creator.rs:279-317 and build_volume_packet :1025-1042:
+// critical || RecvSlic packets in exponent order || critical.  Packet hashes
+// come from the GPU; the headers are assembled here.
+int write_volumes(const CreateOptions &opts, const std::vector<uint32_t> &sizes,
+                  const std::vector<uint8_t> &critical, const uint8_t set_id[16],
+                  const uint8_t *rec, const uint8_t *pkt_md5, std::vector<std::string> &written,
+                  Error &err) {
+    const uint64_t S = opts.slice_size;
+    uint32_t first = 0;
+    for (uint32_t count : sizes) {
+        std::vector<std::vector<uint8_t>> hdrs(count);
+        for (uint32_t k = 0; k < count; k++) {
+            const uint32_t exp = first + k;
+            auto &h = hdrs[k];
+            h.insert(h.end(), kMagic, kMagic + 8);
+            put_le64(h, 64 + 4 + S);
+            h.insert(h.end(), &pkt_md5[16 * (size_t)exp], &pkt_md5[16 * (size_t)exp] + 16);
+            h.insert(h.end(), set_id, set_id + 16);
+            h.insert(h.end(), kTypeRecv, kTypeRecv + 16);
+            put_le32(h, exp);
+        }
+        std::vector<std::pair<const uint8_t *, size_t>> parts;
+        for (uint32_t k = 0; k < count; k++) {
+            parts.push_back({hdrs[k].data(), hdrs[k].size()});
+            parts.push_back({rec + (size_t)(first + k) * S, (size_t)S});
+        }
+        parts.push_back({critical.data(), critical.size()});
+        const std::string vname = volume_name(opts.output, first, count);
+        int rc = write_atomic(vname, {&critical}, parts, err);
+        if (rc != P2G_OK) return rc;
+        written.push_back(vname);
+        first += count;
+    }
+    return P2G_OK;
+}
+
+}  // namespace
+
+int run_create(p2g_ctx *ctx, const CreateOptions &opts, const std::vector<SourceFile> &sources,
+               std::vector<std::string> &written, Error &err) {
+    written.clear();
+    int rc = check_create(opts, sources, err);
+    if (rc != P2G_OK) return rc;
+    const uint64_t S = opts.slice_size;
+    uint64_t total_blocks = 0;
+    for (auto &s : sources) total_blocks += s.slice_checksums.size();
+    uint8_t set_id[16];
+    std::vector<uint8_t> critical;
+    make_critical(S, sources, set_id, critical);
+
+    rc = write_atomic(opts.output, {&critical}, {}, err);
     if (rc != P2G_OK) return rc;
     written.push_back(opts.output);
     const uint32_t R = opts.recovery_block_count;
@@ -377,8 +442,8 @@ int run_create(p2g_ctx *ctx, const CreateOptions &opts, const std::vector<Source
     rc = resolve_volume_sizes(opts.volume_scheme, R, sizes, err);
     if (rc != P2G_OK) return rc;  // after the index is written, as src/creator.rs:237 then :244
 
-    // Single-pass encode, src/creator.rs:883-1018: every file read once, its
-    // slices pushed at their running global input index.
+    // Encode, src/creator.rs:883-1018: every file read again, its slices
+    // pushed at their running global input index.
     p2g_job *job = nullptr;
     rc = p2g_encode_begin(ctx, S, (uint32_t)total_blocks, R, &job);
     if (rc != P2G_OK) return set_err(err, rc, std::string("encode: ") + p2g_last_error(ctx));
@@ -420,36 +485,236 @@ int run_create(p2g_ctx *ctx, const CreateOptions &opts, const std::vector<Source
     rc = p2g_encode_finish(job, rec.p, set_id, pkt_md5.data());
     p2g_encode_free(job);
     if (rc != P2G_OK) return set_err(err, rc, std::string("encode: ") + p2g_last_error(ctx));
+    return write_volumes(opts, sizes, critical, set_id, rec.p, pkt_md5.data(), written, err);
+}
 
-    // Volumes, src/creator.rs:279-317 and build_volume_packet :1025-1042:
-    // critical || RecvSlic packets in exponent order || critical.  Packet
-    // hashes come from the GPU; the headers are assembled here.
-    uint32_t first = 0;
-    for (uint32_t count : sizes) {
-        std::vector<std::vector<uint8_t>> hdrs(count);
-        for (uint32_t k = 0; k < count; k++) {
-            const uint32_t exp = first + k;
-            auto &h = hdrs[k];
-            h.insert(h.end(), kMagic, kMagic + 8);
-            put_le64(h, 64 + 4 + S);
-            h.insert(h.end(), &pkt_md5[16 * (size_t)exp], &pkt_md5[16 * (size_t)exp] + 16);
-            h.insert(h.end(), set_id, set_id + 16);
-            h.insert(h.end(), kTypeRecv, kTypeRecv + 16);
-            put_le32(h, exp);
-        }
-        std::vector<std::pair<const uint8_t *, size_t>> parts;
-        for (uint32_t k = 0; k < count; k++) {
-            parts.push_back({hdrs[k].data(), hdrs[k].size()});
-            parts.push_back({rec.p + (size_t)(first + k) * S, (size_t)S});
-        }
-        parts.push_back({critical.data(), critical.size()});
-        const std::string vname = volume_name(opts.output, first, count);
-        rc = write_atomic(vname, {&critical}, parts, err);
-        if (rc != P2G_OK) return rc;
-        written.push_back(vname);
-        first += count;
+// ---- single-read create ----------------------------------------------
+//
+// The fused shape of src/creator.rs:447-700: each file is read once, in
+// 64 MiB chunks through a ring of pinned buffers; every chunk is copied to
+// the set's HBM arena on a copy stream while a host thread streams it
+// through hash_full / hash16k (src/source.rs:223-228).  Once every file is
+// resident, one p2g_dev_create_pass computes the slice checksums, the
+// recovery blocks and the packet hashes; the index and volumes are written
+// as run_create writes them.  The per-file checks run first, in caller
+// order and in scan's order (name, slice size, stat, empty), so a set fails
+// with the error the scans would report (a read that fails part way is the
+// one case reported after later files' checks rather than before).
+int create_resident(p2g_ctx *ctx, const CreateOptions &opts, const std::vector<std::string> &paths,
+                    const std::vector<std::vector<uint8_t>> &names, std::vector<std::string> &written,
+                    Error &err, bool *declined) {
+    written.clear();
+    *declined = false;
+    const uint64_t S = opts.slice_size;
+    const size_t nf = paths.size();
+    std::vector<SourceFile> sources(nf);
+    for (size_t f = 0; f < nf; f++) {
+        const std::vector<uint8_t> &name = names[f];
+        if (name.empty() || std::find(name.begin(), name.end(), 0) != name.end())
+            return set_err(err, P2G_ERR_INVALID_FILE_NAME,
+                           "filename " + paths[f] + " is empty or contains a NUL byte");
+        if (S == 0 || S % 4 != 0)
+            return set_err(err, P2G_ERR_INVALID_SLICE_SIZE, "invalid slice size " + std::to_string(S));
+        struct stat st;
+        if (::stat(paths[f].c_str(), &st) != 0)
+            return set_err(err, P2G_ERR_IO, "cannot stat " + paths[f] + ": " + strerror(errno));
+        if (st.st_size == 0)
+            return set_err(err, P2G_ERR_EMPTY_FILE, "input file " + paths[f] + " is empty");
+        sources[f].name = name;
+        sources[f].path = paths[f];
+        sources[f].length = (uint64_t)st.st_size;
     }
-    return P2G_OK;
+    int rc = check_create(opts, sources, err);
+    if (rc != P2G_OK) return rc;
+    const uint32_t R = opts.recovery_block_count;
+    std::vector<uint32_t> sizes;
+    Error scheme_err;
+    const int scheme_rc = R ? resolve_volume_sizes(opts.volume_scheme, R, sizes, scheme_err) : P2G_OK;
+
+    // HBM for the whole set: files (256-byte aligned), recovery blocks,
+    // digests; declined (the caller streams instead) when it would not
+    // leave room for the pass's own buffers.
+    uint64_t n = 0, arena = 0;
+    std::vector<uint64_t> off(nf);
+    for (size_t f = 0; f < nf; f++) {
+        off[f] = arena;
+        arena += (sources[f].length + 255) / 256 * 256;
+        n += (sources[f].length + S - 1) / S;
+    }
+    const uint64_t out_bytes = (uint64_t)R * S + 20 * n + 16ull * R + 1024;
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess ||
+        arena + out_bytes + (8ull << 30) > (uint64_t)free_b) {
+        *declined = true;
+        return P2G_OK;
+    }
+    uint8_t *dev = nullptr;
+    if (hipMalloc(reinterpret_cast<void **>(&dev), arena + out_bytes) != hipSuccess) {
+        *declined = true;
+        return P2G_OK;
+    }
+    struct DevFree {
+        uint8_t *p;
+        ~DevFree() { hipFree(p); }
+    } dev_free{dev};
+    uint8_t *d_rec = dev + arena;
+    uint8_t *d_md5 = d_rec + (uint64_t)R * S;
+    uint32_t *d_crc = reinterpret_cast<uint32_t *>(d_md5 + 16 * n);
+    uint8_t *d_pkt = reinterpret_cast<uint8_t *>(d_crc + n);
+
+    // read once: pinned ring -> HBM, and the whole-file hashes on a thread
+    constexpr int kSlots = 4;
+    Pinned slot[kSlots];
+    hipEvent_t copied[kSlots] = {};
+    hipStream_t copy = nullptr;
+    hipError_t he = hipStreamCreateWithFlags(&copy, hipStreamNonBlocking);
+    for (int k = 0; k < kSlots && he == hipSuccess; k++)
+        he = hipEventCreateWithFlags(&copied[k], hipEventDisableTiming);
+    struct Cleanup {
+        hipStream_t &s;
+        hipEvent_t *e;
+        ~Cleanup() {
+            if (s) hipStreamSynchronize(s);
+            for (int k = 0; k < kSlots; k++)
+                if (e[k]) hipEventDestroy(e[k]);
+            if (s) hipStreamDestroy(s);
+        }
+    } cleanup{copy, copied};
+    if (he != hipSuccess) return set_err(err, P2G_ERR_DEVICE, "copy stream");
+    struct Item {
+        int slot;
+        size_t f, got;
+        uint64_t at;
+    };
+    std::vector<HostMd5> full(nf), first16k(nf);
+    std::mutex mu;
+    std::condition_variable cv;
+    std::deque<Item> queue;
+    uint64_t hashed = 0;  // items hashed so far
+    bool stop = false;
+    std::thread hasher([&]() {
+        for (;;) {
+            Item it;
+            {
+                std::unique_lock<std::mutex> lk(mu);
+                cv.wait(lk, [&] { return stop || !queue.empty(); });
+                if (queue.empty()) return;
+                it = queue.front();
+                queue.pop_front();
+            }
+            const uint8_t *p = slot[it.slot].p;
+            full[it.f].update(p, it.got);
+            if (it.at < 16384)
+                first16k[it.f].update(p, (size_t)std::min<uint64_t>(it.got, 16384 - it.at));
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                hashed++;
+            }
+            cv.notify_all();
+        }
+    });
+    auto finish_hasher = [&]() {
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            stop = true;
+        }
+        cv.notify_all();
+        hasher.join();
+    };
+    const uint64_t chunk = kChunkBytes;
+    uint64_t items = 0;
+    for (size_t f = 0; f < nf && rc == P2G_OK; f++) {
+        int fd = ::open(paths[f].c_str(), O_RDONLY);
+        if (fd < 0) {
+            rc = set_err(err, P2G_ERR_IO, "cannot open " + paths[f] + ": " + strerror(errno));
+            break;
+        }
+        for (uint64_t at = 0; at < sources[f].length; at += chunk, items++) {
+            const int k = (int)(items % kSlots);
+            {
+                // the slot's previous chunk must be hashed and copied
+                std::unique_lock<std::mutex> lk(mu);
+                cv.wait(lk, [&] { return hashed + kSlots > items; });
+            }
+            if (hipEventSynchronize(copied[k]) != hipSuccess) {
+                rc = set_err(err, P2G_ERR_DEVICE, "chunk copy");
+                break;
+            }
+            const size_t want = (size_t)std::min<uint64_t>(chunk, sources[f].length - at);
+            size_t got = 0;
+            if (!slot[k].reserve((size_t)chunk) || !read_full(fd, slot[k].p, want, got) ||
+                got != want) {
+                rc = set_err(err, P2G_ERR_IO, "short read on " + paths[f]);
+                break;
+            }
+            if (hipMemcpyAsync(dev + off[f] + at, slot[k].p, got, hipMemcpyHostToDevice, copy) !=
+                    hipSuccess ||
+                hipEventRecord(copied[k], copy) != hipSuccess) {
+                rc = set_err(err, P2G_ERR_DEVICE, "chunk copy");
+                break;
+            }
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                queue.push_back({k, f, got, at});
+            }
+            cv.notify_all();
+        }
+        ::close(fd);
+    }
+    finish_hasher();
+    if (rc != P2G_OK) return rc;
+    if (hipStreamSynchronize(copy) != hipSuccess) return set_err(err, P2G_ERR_DEVICE, "chunk copies");
+    for (size_t f = 0; f < nf; f++) {
+        SourceFile &s = sources[f];
+        full[f].finish(s.hash_full);
+        first16k[f].finish(s.hash16k);
+        HostMd5 id;  // compute_file_id, src/source.rs:303-309
+        id.update(s.hash16k, 16);
+        uint8_t l[8];
+        for (int i = 0; i < 8; i++) l[i] = (uint8_t)(s.length >> (8 * i));
+        id.update(l, 8);
+        id.update(s.name.data(), s.name.size());
+        id.finish(s.file_id);
+    }
+    uint8_t set_id[16];
+    std::vector<uint8_t> body;
+    main_body(S, sources, body, set_id);
+
+    // one device pass: slice checksums, recovery blocks, packet hashes
+    std::vector<p2g_dev_file> files(nf);
+    for (size_t f = 0; f < nf; f++) files[f] = {dev + off[f], sources[f].length};
+    rc = p2g_dev_create_pass(ctx, files.data(), (uint32_t)nf, S, R, set_id, d_rec, d_md5, d_crc,
+                             d_pkt, nullptr);
+    if (rc == P2G_OK) rc = p2g_synchronize(ctx);
+    if (rc != P2G_OK) return set_err(err, rc, std::string("create pass: ") + p2g_last_error(ctx));
+    std::vector<uint8_t> md5s(16 * n), pkt_md5(16ull * R);
+    std::vector<uint32_t> crcs(n);
+    Pinned rec;
+    if (R && !rec.reserve((size_t)R * S)) return set_err(err, P2G_ERR_OUT_OF_MEMORY, "pinned recovery buffer");
+    he = hipMemcpy(md5s.data(), d_md5, 16 * n, hipMemcpyDeviceToHost);
+    if (he == hipSuccess) he = hipMemcpy(crcs.data(), d_crc, 4 * n, hipMemcpyDeviceToHost);
+    if (he == hipSuccess && R) he = hipMemcpy(pkt_md5.data(), d_pkt, 16ull * R, hipMemcpyDeviceToHost);
+    if (he == hipSuccess && R) he = hipMemcpy(rec.p, d_rec, (size_t)R * S, hipMemcpyDeviceToHost);
+    if (he != hipSuccess) return set_err(err, P2G_ERR_DEVICE, "read back the create pass");
+    for (size_t f = 0, sl = 0; f < nf; f++) {
+        SourceFile &s = sources[f];
+        s.slice_checksums.resize((s.length + S - 1) / S);
+        for (auto &c : s.slice_checksums) {
+            memcpy(c.md5, &md5s[16 * sl], 16);
+            c.crc32 = crcs[sl++];
+        }
+    }
+    std::vector<uint8_t> critical;
+    make_critical(S, sources, set_id, critical);
+    rc = write_atomic(opts.output, {&critical}, {}, err);
+    if (rc != P2G_OK) return rc;
+    written.push_back(opts.output);
+    if (R == 0) return P2G_OK;
+    if (scheme_rc != P2G_OK) {  // after the index is written, as src/creator.rs:237 then :244
+        err = scheme_err;
+        return scheme_rc;
+    }
+    return write_volumes(opts, sizes, critical, set_id, rec.p, pkt_md5.data(), written, err);
 }
 
 }  // namespace par2
@@ -467,15 +732,6 @@ extern "C" int p2g_run_create(p2g_ctx *ctx, const char *output, uint64_t slice_s
         return P2G_ERR_INVALID_ARGUMENT;
     if (names_out && names_cap) names_out[0] = 0;
     par2::Error err;
-    std::vector<par2::SourceFile> sources(nfiles);
-    for (uint32_t f = 0; f < nfiles; f++) {
-        std::vector<uint8_t> name(names[f], names[f] + name_lens[f]);
-        int rc = par2::scan(ctx, paths[f], name, slice_size, sources[f], err);
-        if (rc != P2G_OK) {
-            reinterpret_cast<p2g::Ctx *>(ctx)->err = err.message;
-            return rc;
-        }
-    }
     par2::CreateOptions opts;
     opts.output = output;
     opts.slice_size = slice_size;
@@ -488,7 +744,22 @@ extern "C" int p2g_run_create(p2g_ctx *ctx, const char *output, uint64_t slice_s
         opts.volume_scheme.explicit_sizes.assign(scheme->explicit_sizes,
                                                  scheme->explicit_sizes + scheme->n_explicit);
     std::vector<std::string> written;
-    int rc = par2::run_create(ctx, opts, sources, written, err);
+    std::vector<std::string> path_v(paths, paths + nfiles);
+    std::vector<std::vector<uint8_t>> name_v(nfiles);
+    for (uint32_t f = 0; f < nfiles; f++) name_v[f].assign(names[f], names[f] + name_lens[f]);
+    // single read with the set resident in HBM; the streaming scan + encode
+    // when the set does not fit (or P2G_CREATE_STREAM=1)
+    bool declined = true;
+    int rc = P2G_OK;
+    const char *stream_env = getenv("P2G_CREATE_STREAM");
+    if (!(stream_env && atoi(stream_env)))
+        rc = par2::create_resident(ctx, opts, path_v, name_v, written, err, &declined);
+    if (declined) {
+        std::vector<par2::SourceFile> sources(nfiles);
+        for (uint32_t f = 0; f < nfiles && rc == P2G_OK; f++)
+            rc = par2::scan(ctx, paths[f], name_v[f], slice_size, sources[f], err);
+        if (rc == P2G_OK) rc = par2::run_create(ctx, opts, sources, written, err);
+    }
     size_t used = 0;
     for (auto &w : written) {
         if (names_out && used + w.size() + 1 <= names_cap) {

@@ -61,6 +61,13 @@ int scan(p2g_ctx *ctx, const std::string &path, const std::vector<uint8_t> &name
 int run_create(p2g_ctx *ctx, const CreateOptions &opts, const std::vector<SourceFile> &sources,
                std::vector<std::string> &written, Error &err);
 
+// scan + run_create with every file read once into HBM and one fused device
+// pass (the fused shape of src/creator.rs:447-700).  *declined: the set does
+// not fit the free HBM; nothing was written and the caller streams instead.
+int create_resident(p2g_ctx *ctx, const CreateOptions &opts, const std::vector<std::string> &paths,
+                    const std::vector<std::vector<uint8_t>> &names, std::vector<std::string> &written,
+                    Error &err, bool *declined);
+
 // resolve_volume_sizes, src/creator.rs:325-431.
 int resolve_volume_sizes(const VolumeScheme &s, uint32_t total, std::vector<uint32_t> &out,
                          Error &err);

@@ -11,6 +11,17 @@ from oracle import oracle as O
 pytestmark = pytest.mark.gpu
 
 
+@pytest.fixture(params=["resident", "stream"])
+def create_path(request, monkeypatch):
+    """p2g_run_create's two pipelines: every file read once into HBM (the
+    default when the set fits), or scan then a second read for the encode."""
+    if request.param == "stream":
+        monkeypatch.setenv("P2G_CREATE_STREAM", "1")
+    else:
+        monkeypatch.delenv("P2G_CREATE_STREAM", raising=False)
+    return request.param
+
+
 def write(path, b):
     with open(path, "wb") as f:
         f.write(b)
@@ -30,7 +41,7 @@ def both(tmp_path, inputs, S, R, scheme: p2.VolumeScheme, ctx, out_name="recover
     return got
 
 
-def test_config0_exponential(tmp_path, ctx):
+def test_config0_exponential(tmp_path, ctx, create_path):
     src = write(tmp_path / "input.bin", O.xorshift64_bytes(4 << 20, 1))
     got = both(tmp_path, [(str(src), b"input.bin")], 4096, 10, p2.VolumeScheme.exponential(), ctx)
     assert [os.path.basename(p) for p in got] == [
@@ -45,13 +56,13 @@ def test_small_single_volume(tmp_path, ctx):
     assert [os.path.basename(p) for p in got] == ["recovery.par2", "recovery.vol0+2.par2"]
 
 
-def test_index_only(tmp_path, ctx):
+def test_index_only(tmp_path, ctx, create_path):
     src = write(tmp_path / "a.bin", b"hello par2 world")
     got = both(tmp_path, [(str(src), b"a.bin")], 4, 0, p2.VolumeScheme.single(), ctx)
     assert len(got) == 1
 
 
-def test_multi_file_uniform_with_zero_files(tmp_path, ctx):
+def test_multi_file_uniform_with_zero_files(tmp_path, ctx, create_path):
     names = []
     for i, n in enumerate([70000, 65536, 1, 200000, 65537]):
         b = bytes(n) if i == 1 else O.xorshift64_bytes(n, 1000 + i)
@@ -59,7 +70,7 @@ def test_multi_file_uniform_with_zero_files(tmp_path, ctx):
     both(tmp_path, names, 65536, 20, p2.VolumeScheme.uniform(3), ctx)
 
 
-def test_explicit_and_limited(tmp_path, ctx):
+def test_explicit_and_limited(tmp_path, ctx, create_path):
     src = write(tmp_path / "x.bin", O.xorshift64_bytes(30000, 5))
     got = both(tmp_path, [(str(src), b"x.bin")], 1024, 6, p2.VolumeScheme.explicit([3, 3]), ctx)
     assert [os.path.basename(p) for p in got][1:] == ["recovery.vol0+3.par2", "recovery.vol3+3.par2"]
@@ -82,7 +93,7 @@ def expect_error(fn, kind):
     return e.value
 
 
-def test_errors(tmp_path, ctx):
+def test_errors(tmp_path, ctx, create_path):
     out = str(tmp_path / "r.par2")
     empty = write(tmp_path / "empty.bin", b"")
     ok = write(tmp_path / "ok.bin", b"hi there")
