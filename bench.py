@@ -2,6 +2,11 @@
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload headline]
 
+Besides the headline line it reports, in the same JSON object: the whole
+create's wall time on the headline file (end_to_end_create), the batched
+job mix against the same jobs run one at a time (batched), and a serving run
+with Poisson arrivals and a 32 GiB job (serving); flags skip each.
+
 A step is one pass of the device hot path over the workload resident in
 HBM: the GF(2^16) recovery encode (K1), the per-slice MD5 + CRC-32 of the IFSC
 packet (K2) and the recovery-packet MD5s (K3).  By default a step is one
@@ -278,6 +283,58 @@ def _same_alone(ctx, srv, rid, m, sid) -> bool:
     return served == alone
 
 
+def run_end_to_end(ctx, host: bytes, S: int, R: int, cpu: bool) -> dict:
+    """Wall time of the whole create (SourceFile::scan of the file, then
+    run_create writing the index and the volumes) on the headline file, read
+    from the page cache of a temporary directory: p2g_run_create with every
+    file read once into HBM, p2g_run_create streaming (scan, then a second
+    read for the encode), and, with cpu, the oracle's run_create (the
+    reference's CPU pipeline restated: SIMD encode on all host threads)."""
+    import shutil
+    import tempfile
+
+    import par2rust_amd as p2
+
+    tmp = tempfile.mkdtemp(prefix="p2g_e2e_")
+    try:
+        src = os.path.join(tmp, "input.bin")
+        with open(src, "wb") as f:
+            f.write(host)
+        opts = p2.CreateOptions(os.path.join(tmp, "out", "recovery.par2"), S, R,
+                                p2.VolumeScheme.exponential())
+        os.makedirs(os.path.join(tmp, "out"))
+        out = {"file_bytes": len(host), "slice_size": S, "recovery": R,
+               "source": "page-cached file in a temporary directory"}
+        for mode in ("resident", "stream"):
+            if mode == "stream":
+                os.environ["P2G_CREATE_STREAM"] = "1"
+            best = 1e9
+            for _ in range(2):
+                t = time.perf_counter()
+                ctx.run_create(opts, [(src, b"input.bin")])
+                best = min(best, time.perf_counter() - t)
+            os.environ.pop("P2G_CREATE_STREAM", None)
+            out[f"gpu_{mode}_s"] = round(best, 3)
+            out[f"gpu_{mode}_GBps"] = round(len(host) / best / 1e9, 3)
+        if cpu:
+            from oracle import oracle as O
+
+            t = time.perf_counter()
+            O.run_create(os.path.join(tmp, "out", "cpu.par2"), S, R, [(src, b"input.bin")],
+                         O.SCHEME_EXPONENTIAL)
+            ct = time.perf_counter() - t
+            out["cpu_reference_port_s"] = round(ct, 3)
+            out["cpu_threads"] = os.cpu_count()
+            same = all(open(os.path.join(tmp, "out", a), "rb").read() ==
+                       open(os.path.join(tmp, "out", a.replace("recovery", "cpu")), "rb").read()
+                       for a in os.listdir(os.path.join(tmp, "out")) if a.startswith("recovery"))
+            out["bit_exact_vs_cpu"] = bool(same)
+        log(f"bench: end to end {out}")
+        return out
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -295,6 +352,8 @@ def main() -> int:
     ap.add_argument("--serve-load", type=float, default=0.5,
                     help="arrival rate as a fraction of the measured batched jobs/s")
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
+    ap.add_argument("--no-end-to-end", action="store_true",
+                    help="skip the whole-create wall-time comparison")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run one 512 MiB device-to-device copy (the reference reading "
                          "for the FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -441,6 +500,8 @@ def main() -> int:
         "roofline": roof,
     }
 
+    if not args.no_end_to_end:
+        out["end_to_end_create"] = run_end_to_end(ctx, host, S, R, not args.no_cpu_baseline)
     if args.batch_jobs > 0:
         out["batched"] = run_batch_mix(ctx, args.batch_jobs, args.batch_reps)
     if args.serve_seconds > 0:
