@@ -460,10 +460,10 @@ struct MfLds {
 // (valid = 0), or, FULL, copies of input 0 (read, never counted).  The caller
 // synchronizes before the first unit.
 template <bool FULL>
-__device__ __forceinline__ void mf_load_desc(MfLds &lds, const SliceDesc *__restrict__ inputs,
+__device__ __forceinline__ void mf_load_desc(uint4 *desc, const SliceDesc *__restrict__ inputs,
                                              uint32_t n, uint32_t nq) {
     for (uint32_t i = threadIdx.x; i < 4 * nq; i += kMfThreads)
-        lds.desc[i] = i < n  ? gload(reinterpret_cast<const uint4 *>(inputs) + i)
+        desc[i] = i < n  ? gload(reinterpret_cast<const uint4 *>(inputs) + i)
                       : FULL ? gload(reinterpret_cast<const uint4 *>(inputs))
                              : make_uint4(0, 0, 0, 0);
 }
@@ -606,11 +606,181 @@ __device__ __forceinline__ void mf_unit(MfLds &lds, const SliceDesc *__restrict_
     }
 }
 
+// The same unit for whole-slice descriptors with every operand loaded
+// straight into LDS (global_load_lds): the A fragments (dwordx4) and the
+// source words (dword) of step g land in ring buffer g % 4 three steps
+// ahead, take no registers and cannot be moved late by the scheduler.  A
+// ring buffer is kMfRing uint4: the 512 A entries, then 8 runs of 64 words
+// (wave w, word j at run 2 w + j, lane l's word at position l).  The four
+// buffers are distinct __shared__ objects, so the compiler does not make
+// LDS reads wait for the DMA into the other buffers; the waits are explicit
+// instead.  Every thread issues the same loads in the same order on every
+// step (clamped past nq; a compiler fence between steps' groups): per step
+// A(g + 3) x 2 then W(g + 3) x 2, after a prologue A0 W0 A1 W1 A2 W2.  At
+// the top of step g, group g + 1 (A and W of step g + 1) is complete once
+// at most the 4 loads of step g - 1 are in flight: vmcnt(4) (with
+// lgkmcnt(0): this wave's LDS reads of step g - 1 are done), then one
+// s_barrier (no fence: its release would wait for every load in flight)
+// publishes every wave's share.  Step g then reads A(g + 1) and W(g + 1)
+// from LDS while its MFMAs run on A(g) and B(g), read and built in step
+// g - 1, so no MFMA waits for LDS.  The buffer that step g's DMA overwrites,
+// (g + 3) % 4, was last read in step g - 2.  Scheduling barriers keep each
+// step's instructions between its barriers (MFMAs have no memory operands,
+// so the compiler fences alone would let them sink past later barriers).
+constexpr int kMfRing = kMfTiles * 64 + 128;  // uint4 per ring buffer: A, then 512 words
+constexpr int kWaitVmcnt4 = 0x0074;  // s_waitcnt vmcnt(4) lgkmcnt(0) (this wave's LDS reads done)
+constexpr int kWaitVmcnt8 = 0x0F78;
+constexpr int kWaitVmcnt0 = 0x0070;  // vmcnt(0) lgkmcnt(0)
+
+// Addresses of the two source words of K step g of this lane (whole slices).
+__device__ __forceinline__ void mf_word_addrs(const uint4 *s_desc, uint32_t g, uint32_t h,
+                                              uint32_t off, const uint32_t *&p0,
+                                              const uint32_t *&p1) {
+    const uint2 d0 = reinterpret_cast<const uint2 *>(s_desc + 4 * g + 2 * h)[0];
+    const uint2 d1 = reinterpret_cast<const uint2 *>(s_desc + 4 * g + 2 * h + 1)[0];
+    p0 = reinterpret_cast<const uint32_t *>(((uint64_t)d0.y << 32 | d0.x) + off);
+    p1 = reinterpret_cast<const uint32_t *>(((uint64_t)d1.y << 32 | d1.x) + off);
+}
+
+__device__ __forceinline__ void mf_unit_dma(const uint4 *s_desc, uint4 *A0, uint4 *A1, uint4 *A2,
+                                            uint4 *A3, const SliceDesc *__restrict__ inputs,
+                                            const uint4 *__restrict__ afrag, uint32_t nq,
+                                            uint32_t frag_stride, uint32_t R, uint32_t S,
+                                            uint8_t *__restrict__ rec, int accumulate,
+                                            uint32_t tile0, uint32_t colbase) {
+    (void)inputs;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t lane = tid & 63, w = tid >> 6;
+    const uint32_t c = lane & 31, h = lane >> 5;
+    const uint32_t off = colbase + w * 128 + 4 * c;
+    const bool col_ok = off < S;
+    const uint32_t off_ld = min(off, S - 4);
+    const uint4 *ga0 = afrag + ((size_t)(tile0 + w) * frag_stride) * 64 + lane;
+    const uint4 *ga1 = afrag + ((size_t)(tile0 + 4 + w) * frag_stride) * 64 + lane;
+    // this lane's words in a ring buffer: run 2 w (word 0) and 2 w + 1 (word 1)
+    const uint32_t wrd0 = kMfTiles * 64 * 4 + (2 * w) * 64 + lane;
+
+    v16f_t acc[kMfTiles][2];
+#pragma unroll
+    for (int a = 0; a < kMfTiles; a++)
+#pragma unroll
+        for (int cb = 0; cb < 2; cb++)
+#pragma unroll
+            for (int v = 0; v < 16; v++) acc[a][cb][v] = 0.0f;
+
+    // word addresses of the next DMA, read from the descriptors one step ahead
+    const uint32_t *pw0, *pw1;
+#define MF_DMA(G, BUF)                                                                        \
+    do {                                                                                      \
+        const uint32_t gq = min((uint32_t)(G), nq - 1);                                       \
+        __builtin_amdgcn_global_load_lds(ga0 + (size_t)gq * 64, (BUF) + 64 * w, 16, 0, 0);    \
+        __builtin_amdgcn_global_load_lds(ga1 + (size_t)gq * 64, (BUF) + 256 + 64 * w, 16, 0, 0); \
+        __builtin_amdgcn_global_load_lds(pw0, reinterpret_cast<uint32_t *>(BUF) + wrd0 - lane, 4, 0, 0); \
+        __builtin_amdgcn_global_load_lds(pw1, reinterpret_cast<uint32_t *>(BUF) + wrd0 + 64 - lane, 4, 0, 0); \
+        asm volatile("" ::: "memory");                                                        \
+        mf_word_addrs(s_desc, min((uint32_t)(G) + 1, nq - 1), h, off_ld, pw0, pw1);           \
+    } while (0)
+#define MF_B(BUF, OUT)                                                                        \
+    mf_make_b(reinterpret_cast<const uint32_t *>(BUF)[wrd0],                                  \
+              reinterpret_cast<const uint32_t *>(BUF)[wrd0 + 64], OUT)
+    mf_word_addrs(s_desc, 0, h, off_ld, pw0, pw1);
+    MF_DMA(0, A0);
+    MF_DMA(1, A1);
+    MF_DMA(2, A2);
+    v8i_t Bc[2];
+    uint4 Ac[kMfTiles];
+    __builtin_amdgcn_s_waitcnt(kWaitVmcnt8);  // group 0 of this wave
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+#pragma unroll
+    for (int a = 0; a < kMfTiles; a++) Ac[a] = A0[a * 64 + lane];
+    MF_B(A0, Bc);
+
+    // step GG: MFMAs on Ac = A(GG), Bc = B(GG); DMA(GG + 3) into BUFW;
+    // A(GG + 1) and B(GG + 1) from BUFN.
+#define MF_STEP_DMA(GG, BUFN, BUFW)                                                           \
+    do {                                                                                      \
+        __builtin_amdgcn_sched_barrier(0);                                                    \
+        __builtin_amdgcn_s_waitcnt(kWaitVmcnt4);                                              \
+        asm volatile("" ::: "memory");                                                        \
+        __builtin_amdgcn_s_barrier();                                                         \
+        asm volatile("" ::: "memory");                                                        \
+        __builtin_amdgcn_sched_barrier(0);                                                    \
+        MF_DMA((GG) + 3, BUFW);                                                               \
+        uint4 An[kMfTiles];                                                                   \
+        _Pragma("unroll") for (int a = 0; a < kMfTiles; a++) An[a] = (BUFN)[a * 64 + lane];  \
+        v8i_t Bn[2];                                                                          \
+        MF_B(BUFN, Bn);                                                                       \
+        _Pragma("unroll") for (int a = 0; a < kMfTiles; a++) {                                \
+            const uint4 A = Ac[a];                                                            \
+            const v8i_t av = v8i_t{(int)A.x, (int)A.y, (int)A.z, (int)A.w, 0, 0, 0, 0};       \
+            _Pragma("unroll") for (int cb = 0; cb < 2; cb++) acc[a][cb] =                     \
+                __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(av, Bc[cb], acc[a][cb], 4, 4, \
+                                                                0, kE8M0One, 0, kE8M0One);    \
+        }                                                                                     \
+        /* LDS reads first, then half the MFMAs while they land (the B build */            \
+        /* after them waits for every LDS read), then MFMAs with the VALU */                 \
+        __builtin_amdgcn_sched_group_barrier(0x100, 11, 0);                                   \
+        __builtin_amdgcn_sched_group_barrier(0x008, kMfTiles, 0);                             \
+        _Pragma("unroll") for (int k = 0; k < kMfTiles; k++) {                                \
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);                                \
+            __builtin_amdgcn_sched_group_barrier(0x002, 5, 0);                                \
+        }                                                                                     \
+        Bc[0] = Bn[0];                                                                        \
+        Bc[1] = Bn[1];                                                                        \
+        _Pragma("unroll") for (int a = 0; a < kMfTiles; a++) Ac[a] = An[a];                   \
+    } while (0)
+
+    for (uint32_t g = 0; g < nq; g += 4) {
+        MF_STEP_DMA(g, A1, A3);
+        MF_STEP_DMA(g + 1, A2, A0);
+        MF_STEP_DMA(g + 2, A3, A1);
+        MF_STEP_DMA(g + 3, A0, A2);
+    }
+#undef MF_STEP_DMA
+#undef MF_B
+#undef MF_DMA
+    // the clamped DMAs of steps past nq, and this wave's last LDS reads, are
+    // done before anything reuses the ring
+    __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+
+#pragma unroll
+    for (int a = 0; a < kMfTiles; a++) {
+        uint32_t sym[2];
+#pragma unroll
+        for (int cb = 0; cb < 2; cb++) {
+            uint32_t lo = 0, hi = 0;
+#pragma unroll
+            for (int v = 0; v < 16; v++) {
+                const uint32_t bit = ((uint32_t)acc[a][cb][v]) & 1u;
+                const int pos = (v & 3) + ((v >> 2) & 1) * 8;
+                if (v < 8) lo |= bit << pos;
+                else hi |= bit << pos;
+            }
+            lo <<= 4 * h;
+            hi <<= 4 * h;
+            const uint32_t give = h ? lo : hi;
+            const uint32_t got = (uint32_t)__shfl_xor((int)give, 32, 64);
+            sym[cb] = (h ? hi : lo) | got;
+        }
+        const uint32_t r = 2 * (tile0 + a) + h;
+        if (r < R && col_ok) {
+            uint32_t *o = reinterpret_cast<uint32_t *>(rec + (size_t)r * S + off);
+            const uint32_t v = sym[0] | (sym[1] << 16);
+            *o = accumulate ? (*o ^ v) : v;
+        }
+    }
+}
+
 __global__ __launch_bounds__(kMfThreads, 1) void k_encode_mfma(
     const SliceDesc *__restrict__ inputs, uint32_t n, const uint4 *__restrict__ afrag,
     uint32_t nq, uint32_t R, uint32_t S, uint8_t *__restrict__ rec, int accumulate) {
     __shared__ MfLds lds;
-    mf_load_desc<false>(lds, inputs, n, nq);
+    mf_load_desc<false>(lds.desc, inputs, n, nq);
     __syncthreads();
     mf_unit<false>(lds, inputs, afrag, nq, nq, R, S, rec, accumulate, blockIdx.y * kMfTiles,
                    blockIdx.x * (kMfThreads * 2));
@@ -1265,6 +1435,8 @@ __device__ __forceinline__ void pass_stamp_end(const PassArgs &a, int role) {
 template <bool FULL>
 __global__ __launch_bounds__(kMfThreads, 1) void k_create_pass(PassArgs a) {
     __shared__ MfLds lds;
+    // the A ring of mf_unit_dma: four separate objects (see there)
+    __shared__ uint4 s_a0[kMfRing], s_a1[kMfRing], s_a2[kMfRing], s_a3[kMfRing];
     const uint32_t blk = blockIdx.x, tid = threadIdx.x;
     // MD5 workgroups run their hash on wave 0 (64 lanes: each lane streams its
     // own message).  In the slice-hash workgroups waves 1-3, on the other
@@ -1301,7 +1473,7 @@ __global__ __launch_bounds__(kMfThreads, 1) void k_create_pass(PassArgs a) {
         return;
     }
     if (tid == 0) pass_stamp_start(a, kPassEncode);
-    mf_load_desc<FULL>(lds, a.inputs, a.n, a.nq);
+    mf_load_desc<FULL>(lds.desc, a.inputs, a.n, a.nq);
     __syncthreads();  // descriptor table ready
     // Encode workgroup e takes units e, e + E, e + 2E, ... (E encode
     // workgroups): all units of a kind cost the same, the order stays band by
@@ -1328,8 +1500,12 @@ __global__ __launch_bounds__(kMfThreads, 1) void k_create_pass(PassArgs a) {
         }
         const uint32_t c0 = bstart[band], w = bstart[band + 1] - c0;
         const uint32_t v = u - b_lo, y = v / w, x = c0 + v % w;
-        mf_unit<FULL>(lds, a.inputs, a.afrag, a.nq, a.nq, a.R, a.S, a.rec, 0, y * kMfTiles,
-                      x * (kMfThreads * 2));
+        if (FULL)
+            mf_unit_dma(lds.desc, s_a0, s_a1, s_a2, s_a3, a.inputs, a.afrag, a.nq, a.nq, a.R, a.S,
+                        a.rec, 0, y * kMfTiles, x * (kMfThreads * 2));
+        else
+            mf_unit<FULL>(lds, a.inputs, a.afrag, a.nq, a.nq, a.R, a.S, a.rec, 0, y * kMfTiles,
+                          x * (kMfThreads * 2));
         done++;
         const uint32_t nxt = u + e_blocks;
         if (nxt >= a.n_units || nxt >= b_hi) {
@@ -1590,6 +1766,7 @@ __device__ __forceinline__ void batch_packet_lane(const BatchArgs &a, const Batc
 
 __global__ __launch_bounds__(kMfThreads, 1) void k_create_batch(BatchArgs a) {
     __shared__ MfLds lds;
+    __shared__ uint4 s_a0[kMfRing], s_a1[kMfRing], s_a2[kMfRing], s_a3[kMfRing];
     const uint32_t blk = blockIdx.x, tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     if (tid == 0) {
         unsigned long long *t0 = a.job_done + a.n_jobs;
@@ -1670,11 +1847,11 @@ __global__ __launch_bounds__(kMfThreads, 1) void k_create_batch(BatchArgs a) {
             const uint32_t cq = min(P.nq - k0 / 4, kMfmaChunkInputs / 4);
             if (cin != loaded) {
                 __syncthreads();  // every unit is done reading the old table
-                mf_load_desc<true>(lds, cin, cn, cq);
+                mf_load_desc<true>(lds.desc, cin, cn, cq);
                 __syncthreads();
                 loaded = cin;
             }
-            mf_unit<true>(lds, cin, afrag + (size_t)(k0 / 4) * 64, cq, P.nq, P.R, P.S, P.rec,
+            mf_unit_dma(lds.desc, s_a0, s_a1, s_a2, s_a3, cin, afrag + (size_t)(k0 / 4) * 64, cq, P.nq, P.R, P.S, P.rec,
                           k0 > 0, y * kMfTiles, x * (kMfThreads * 2));
         }
         done++;
