@@ -536,7 +536,9 @@ int create_pass_impl(Ctx *c, const p2g_dev_file *files, uint32_t nfiles, uint64_
     std::vector<SliceDesc> parts;
     for (uint32_t i = 0; i < n; i++)
         if (descs[i].valid < S) parts.push_back({descs[i].ptr, descs[i].valid, (uint32_t)parts.size()});
-    const bool stage = !parts.empty() && (uint64_t)parts.size() * S <= kStageBudget;
+    uint64_t stage_budget = kStageBudget;
+    if (const char *v = getenv("P2G_STAGE_BYTES")) stage_budget = strtoull(v, nullptr, 10);
+    const bool stage = !parts.empty() && (uint64_t)parts.size() * S <= stage_budget;
     if (stage) {
         if (c->d_partials.reserve(parts.size() * (size_t)S) != P2G_OK)
             return c->fail(P2G_ERR_OUT_OF_MEMORY, "partial slice staging");

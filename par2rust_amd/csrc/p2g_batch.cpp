@@ -177,9 +177,11 @@ int build_plan(Ctx *c, const std::vector<PlanItem> &items, bool use_graph, Batch
                 all.push_back(d);
             }
         }
-        // an item without its slice checksums has an empty slice range
-        const uint32_t lo = slice0[j], hi = items[j].slices ? (uint32_t)all.size() : lo;
-        sjobs.push_back({J.slice_md5, J.slice_crc, lo, hi, S, j});
+        // the item's slice range (possibly empty) within the job's slices
+        const uint32_t n = (uint32_t)jd[j].size();
+        const uint32_t s0 = std::min(items[j].s0, n), s1 = std::max(s0, std::min(items[j].s1, n));
+        sjobs.push_back({J.slice_md5 + 16ull * s0, J.slice_crc + s0, slice0[j] + s0, slice0[j] + s1,
+                         S, j});
     }
     B->n_slices = (uint32_t)n_slices;
     B->n_parts = (uint32_t)parts.size();
@@ -337,7 +339,7 @@ static int make_plan(Ctx *c, const p2g_batch_job *jobs, uint32_t njobs, bool use
     *out = nullptr;
     if (njobs == 0 || !jobs) return c->fail(P2G_ERR_INVALID_ARGUMENT, "empty batch");
     std::vector<PlanItem> items(njobs);
-    for (uint32_t j = 0; j < njobs; j++) items[j] = {&jobs[j], 0, jobs[j].recovery_count, true};
+    for (uint32_t j = 0; j < njobs; j++) items[j] = {&jobs[j], 0, jobs[j].recovery_count, 0, UINT32_MAX};
     BatchPlan *B = new (std::nothrow) BatchPlan();
     if (!B) return c->fail(P2G_ERR_OUT_OF_MEMORY, "batch plan");
     const int rc = build_plan(c, items, use_graph, B);

@@ -17,7 +17,7 @@ constexpr uint32_t kMaxInputs = 32768;            // MAX_INPUT_BLOCKS, src/creat
 constexpr uint64_t kMaxSliceBytes = 1ull << 31;   // kernels index slices with u32
 constexpr uint64_t kMatsBudget = 256ull << 20;    // coefficient matrices per launch chunk
 constexpr uint64_t kPassAfragBudget = 4ull << 30; // A fragments of one fused create pass
-constexpr uint64_t kStageBudget = 256ull << 20;   // zero-padded copies of partial slices
+constexpr uint64_t kStageBudget = 256ull << 20;   // zero-padded copies of partial slices (P2G_STAGE_BYTES overrides)
 
 // Grow-only device allocation.
 struct DevBuf {
@@ -113,12 +113,13 @@ struct Job {
 
 int check_slice_size(Ctx *c, uint64_t S);
 
-// Batched create plans (p2g_batch.cpp).  An item is rows [r0, r1) of a job
-// and, with `slices`, the job's slice checksums.
+// Batched create plans (p2g_batch.cpp).  An item is recovery rows [r0, r1)
+// of a job and the checksums of its slices [s0, s1) (s1 past the job's
+// slice count means all of them).
 struct PlanItem {
     const p2g_batch_job *job;
     uint32_t r0, r1;
-    bool slices;
+    uint32_t s0, s1;
 };
 int check_batch_job(Ctx *c, const p2g_batch_job &J, std::vector<SliceDesc> &descs);
 int build_plan(Ctx *c, const std::vector<PlanItem> &items, bool use_graph, BatchPlan *b);

@@ -10,8 +10,9 @@
 //
 // A step is one batched plan (build_plan) over every admitted job with work
 // left: whole jobs when their estimated device time fits the step target,
-// otherwise a piece of recovery rows sized to the target (and the slice
-// checksums with the first piece).  Steps run in order on one stream, at
+// otherwise a piece of recovery rows sized to the target; a job's slice
+// checksums likewise go in as many rounds of the slice lanes as fit the
+// target.  Steps run in order on one stream, at
 // most two queued at a time (the next one is planned while the current one
 // runs), so requests that arrive while the device is busy join the next
 // step and a long job never holds the others back for more than a step.
@@ -31,6 +32,8 @@
 namespace p2g {
 
 constexpr double kServeOpsPerMs = 4.0e12;  // device rate assumed when sizing pieces (4 PF)
+constexpr double kServeMd5BlockMs = 1.1e-3;  // one lane's MD5 + CRC-32 of a 64-byte block
+constexpr uint32_t kServeSliceLanes = 24 * 128;  // slice-hash lanes of a batched launch
 
 // First-fit allocator over one device allocation, 256-byte granules.
 struct Pool {
@@ -82,8 +85,8 @@ struct Request {
     std::vector<p2g_dev_file> files;
     p2g_batch_job job{};
     uint32_t rows_assigned = 0, rows_done = 0;
-    bool slices_assigned = false, slices_done = false;
-    double ms_per_row = 0;
+    uint32_t slices_assigned = 0, slices_done = 0;
+    double ms_per_row = 0, ms_per_slice = 0;  // estimated device time (encode row; one slice's hash)
     double t_submit = 0, t_admit = 0, t_done = 0;  // ms on the server clock
 };
 
@@ -174,7 +177,7 @@ int Server::launch_step(int k) {
     st.items.clear();
     std::vector<PlanItem> items;
     for (Request *q : active) {
-        if (q->rows_assigned == q->R && q->slices_assigned) continue;
+        if (q->rows_assigned == q->R && q->slices_assigned == q->n) continue;
         const uint32_t left = q->R - q->rows_assigned;
         uint32_t take = left;
         if (left && q->ms_per_row * left > step_ms) {
@@ -182,9 +185,14 @@ int Server::launch_step(int k) {
             const uint32_t rows = (uint32_t)(step_ms / q->ms_per_row);
             take = std::min(left, std::max<uint32_t>(16, rows / 16 * 16));
         }
-        PlanItem it{&q->job, q->rows_assigned, q->rows_assigned + take, !q->slices_assigned};
+        // slice checksums: as many rounds of the slice lanes as fit the target
+        const uint32_t sleft = q->n - q->slices_assigned;
+        const uint32_t rounds = std::max<uint32_t>(1, (uint32_t)(step_ms / q->ms_per_slice));
+        const uint32_t stake = (uint32_t)std::min<uint64_t>(sleft, (uint64_t)rounds * kServeSliceLanes);
+        PlanItem it{&q->job, q->rows_assigned, q->rows_assigned + take, q->slices_assigned,
+                    q->slices_assigned + stake};
         q->rows_assigned += take;
-        q->slices_assigned = true;
+        q->slices_assigned += stake;
         st.items.push_back({q, it});
         items.push_back(it);
     }
@@ -205,8 +213,8 @@ int Server::retire(int k) {
     for (auto &pr : st.items) {
         Request *q = pr.first;
         q->rows_done += pr.second.r1 - pr.second.r0;
-        if (pr.second.slices) q->slices_done = true;
-        if (q->rows_done == q->R && q->slices_done && q->state == Request::kActive) {
+        q->slices_done += pr.second.s1 - pr.second.s0;
+        if (q->rows_done == q->R && q->slices_done == q->n && q->state == Request::kActive) {
             q->state = Request::kDone;
             q->t_done = t;
             finished.push_back(q->id);
@@ -279,6 +287,7 @@ int p2g_server_submit(p2g_server *srv, const p2g_request *r, uint64_t *id_out) {
     if (q->footprint > s->pool.size)
         return c->fail(P2G_ERR_OUT_OF_MEMORY, "request larger than the server's HBM budget");
     q->ms_per_row = 256.0 * (double)n * (double)q->S / kServeOpsPerMs;
+    q->ms_per_slice = (double)((q->S + 8) / 64 + 1) * kServeMd5BlockMs;
     q->id = s->next_id++;
     q->t_submit = s->now_ms();
     *id_out = q->id;

@@ -117,3 +117,13 @@ def test_create_pass_twice_reuses_state(ctx):
     b = [data(50000, 94)]
     assert run_pass(ctx, a, 8192, 12, sid) == expected(a, 8192, 12, sid)
     assert run_pass(ctx, b, 4096, 5, sid) == expected(b, 4096, 5, sid)
+
+
+@pytest.mark.parametrize("lens,S,R", [([70000, 4100], 4096, 7), ([777], 12, 4), ([1 << 20, 9], 16384, 33)])
+def test_create_pass_partial_slices_in_place(ctx, monkeypatch, lens, S, R):
+    # no staging budget: partial slices are read in place (the general
+    # instantiation of the pass, with tail words and masked word addresses)
+    monkeypatch.setenv("P2G_STAGE_BYTES", "0")
+    files = [data(n, 500 + i) for i, n in enumerate(lens)]
+    sid = data(16, 79)
+    assert run_pass(ctx, files, S, R, sid) == expected(files, S, R, sid)

@@ -35,6 +35,7 @@ REQS = [
     ([5000], 64, 0),
     ([16 * 3000 + 9], 16, 50),
     ([300000], 65536, 70),
+    ([4 * 7000], 4, 3),              # 7000 slices: checksums over several steps at a tiny target
 ]
 
 
