@@ -175,7 +175,8 @@ def run_batch_mix(ctx, n_jobs: int, reps: int) -> dict:
     }
 
 
-def run_serving(ctx, seconds: float, rate: float, budget: int, verify: int = 4) -> dict:
+def run_serving(ctx, seconds: float, rate: float, budget: int, verify: int = 4,
+                step_ms: float = 0.0) -> dict:
     """The serving run: requests of the batched job mix (synth.batch_mix,
     seed 4001) arrive as a Poisson process of `rate` jobs/s for `seconds`,
     plus one 32 GiB job (one file, slice size 1 MiB: 32768 slices, 3000
@@ -203,7 +204,7 @@ def run_serving(ctx, seconds: float, rate: float, budget: int, verify: int = 4) 
     reqs.append((0.5, big))
     reqs.sort(key=lambda r: r[0])
     t_open = time.perf_counter()
-    srv = ctx.server(budget)
+    srv = ctx.server(budget, step_ms)
     log(f"bench: serving {len(reqs)} requests over {seconds}s (rate {rate:.1f}/s, HBM budget "
         f"{budget / 1e9:.0f} GB, pool ready in {time.perf_counter() - t_open:.1f}s)")
     sample = set(range(1, len(reqs), max(1, len(reqs) // max(1, verify))))
@@ -244,7 +245,7 @@ def run_serving(ctx, seconds: float, rate: float, budget: int, verify: int = 4) 
     pct = lambda v, q: round(float(np.percentile(np.array(v) * 1e3, q)), 2)
     out = {
         "requests": len(reqs), "arrival_rate_jobs_per_s": round(rate, 1), "arrival_seconds": seconds,
-        "hbm_budget_bytes": budget, "source_bytes": src_bytes,
+        "hbm_budget_bytes": budget, "step_target_ms": step_ms or 100.0, "source_bytes": src_bytes,
         "source_GBps": round(src_bytes / wall / 1e9, 2), "wall_s": round(wall, 3),
         "latency_ms": {"p50": pct(small, 50), "p99": pct(small, 99), "max": pct(small, 100)},
         "big_job": {"bytes": 32 << 30, "slice_size": 1 << 20, "slices": 32768, "recovery": 3000,
@@ -352,6 +353,8 @@ def main() -> int:
     ap.add_argument("--serve-load", type=float, default=0.5,
                     help="arrival rate as a fraction of the measured batched jobs/s")
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
+    ap.add_argument("--serve-step-ms", type=float, default=0.0,
+                    help="the server's step target (0: its default, 100 ms)")
     ap.add_argument("--no-end-to-end", action="store_true",
                     help="skip the whole-create wall-time comparison")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -507,7 +510,7 @@ def main() -> int:
     if args.serve_seconds > 0:
         cap = out["batched"]["batched_jobs_per_s"] if "batched" in out else 400.0
         out["serving"] = run_serving(ctx, args.serve_seconds, args.serve_load * cap,
-                                     int(args.serve_budget_gb * 1e9))
+                                     int(args.serve_budget_gb * 1e9), step_ms=args.serve_step_ms)
 
     if not args.no_cpu_baseline:
         from oracle import oracle as O

@@ -99,7 +99,7 @@ struct Step {
 struct Server {
     Ctx *c = nullptr;
     Pool pool;
-    double step_ms = 100.0;
+    double step_ms = 50.0;
     uint64_t next_id = 1;
     std::deque<Request *> queued;
     std::vector<Request *> active;
