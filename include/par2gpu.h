@@ -228,7 +228,7 @@ typedef struct {
 } p2g_request;
 
 /* hbm_budget: bytes of device memory for admitted requests (allocated at
- * open); step_ms: target device time of one step (0: 100 ms). */
+ * open); step_ms: target device time of one step (0: 50 ms). */
 int p2g_server_open(p2g_ctx *ctx, uint64_t hbm_budget, double step_ms, p2g_server **out);
 /* Queues a request; id_out identifies it.  Fails at once for a request
  * that could never be admitted. */

@@ -9,13 +9,14 @@
 // take their memory from one pool allocated at open.
 //
 // A step is one batched plan (build_plan) over every admitted job with work
-// left: whole jobs when their estimated device time fits the step target,
-// otherwise a piece of recovery rows sized to the target; a job's slice
-// checksums likewise go in as many rounds of the slice lanes as fit the
-// target.  Steps run in order on one stream, at
-// most two queued at a time (the next one is planned while the current one
-// runs), so requests that arrive while the device is busy join the next
-// step and a long job never holds the others back for more than a step.
+// left: whole jobs when their estimated device time fits the step target
+// (50 ms unless the caller sets one), otherwise a piece of recovery rows
+// sized to the target; a job's slice checksums likewise go in as many
+// rounds of the slice lanes as fit the target.  Steps run in order on one
+// stream, at most two queued at a time (the next one is planned while the
+// current one runs), so requests that arrive while the device is busy join
+// the next step and a long job never holds the others back for more than a
+// step.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
