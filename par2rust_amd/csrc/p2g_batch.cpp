@@ -236,10 +236,12 @@ int build_plan(Ctx *c, const std::vector<PlanItem> &items, bool use_graph, Batch
                                           ch.variant == kEncodeMfma && ch.band_cols
                                       ? ch.band_cols
                                       : band_default;
-            P.band_cols = std::max<uint32_t>(1, std::min(band, P.col_groups));
+            // rows of at most 32 column groups (slices up to 16 KiB) are one band
+            P.band_cols = P.col_groups <= 32 ? P.col_groups
+                                             : std::max<uint32_t>(1, std::min(band, P.col_groups));
             P.n_bands = 0;
             for (uint32_t cg = 0; cg < P.col_groups; P.n_bands++)
-                cg += next_band_width(cg, P.col_groups - cg, P.band_cols);
+                cg += batch_band_width(cg, P.col_groups - cg, P.band_cols);
             pieces.push_back(P);
             afrag_off.push_back(spans.back().afrag);
             spans.back().p_hi = (uint32_t)pieces.size();

@@ -84,6 +84,13 @@ __host__ __device__ inline uint32_t next_band_width(uint32_t done, uint32_t rema
     return ramp < tail ? ramp : tail;
 }
 
+// Band widths of a batched piece: one band when W covers every column
+// group (a short row, whose packet hashes gain nothing from early bands),
+// else next_band_width.
+__host__ __device__ inline uint32_t batch_band_width(uint32_t done, uint32_t remaining, uint32_t W) {
+    return done == 0 && W >= remaining ? remaining : next_band_width(done, remaining, W);
+}
+
 struct PassPlan {
     uint32_t grid = 0, h2_blocks = 0, h3_blocks = 0;
     uint32_t row_blocks = 0, col_groups = 0, band_cols = 0, n_bands = 0, n_units = 0;

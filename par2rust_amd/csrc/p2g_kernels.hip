@@ -1818,11 +1818,19 @@ __global__ __launch_bounds__(kMfThreads, 1) void k_create_batch(BatchArgs a) {
     const uint32_t first = a.h3_blocks + a.h2_blocks;
     const uint32_t e_blocks = gridDim.x - first;
     if (tid == 0) batch_role(a.sync, kPassEncode, false);
+    // Workgroups are dealt to the 8 XCDs round-robin (XCD = blk % 8).  Each
+    // round of e_blocks units is taken in XCD order: this workgroup's rank
+    // is the encode workgroups of lower XCDs, then those of its own XCD
+    // before it, so units adjacent in order (one row block's column groups)
+    // run on one XCD and share its L2.
+    uint32_t rank = 0;
+    for (uint32_t b = first; b < gridDim.x; b++)
+        if (b % 8 < blk % 8 || (b % 8 == blk % 8 && b < blk)) rank++;
     uint32_t pi = 0;
     const SliceDesc *loaded = nullptr;  // first descriptor of the chunk in lds.desc
     uint32_t band = 0, b_lo = 0, b_hi = 0, unit_end = 0;
     uint32_t done = 0;
-    for (uint32_t u = blk - first; u < a.n_units; u += e_blocks) {
+    for (uint32_t u = rank; u < a.n_units; u += e_blocks) {
         if (u >= unit_end) {
             while (u >= a.pieces[pi].unit_end) pi++;
             const BatchPiece &P = a.pieces[pi];
@@ -1889,7 +1897,7 @@ __global__ void k_batch_reset(uint32_t *sync, uint32_t words, const BatchPiece *
         uint32_t *bs = sync + P.bstart0, c = 0;
         for (uint32_t b = 0; b < P.n_bands; b++) {
             bs[b] = c;
-            c += next_band_width(c, P.col_groups - c, P.band_cols);
+            c += batch_band_width(c, P.col_groups - c, P.band_cols);
         }
         bs[P.n_bands] = c;
     }
