@@ -893,17 +893,40 @@ struct Md5NoVisit {
 template <typename W, typename Addr, typename Visit = Md5NoVisit>
 __device__ __forceinline__ void md5_blocks_at(uint32_t h[4], Addr addr, uint32_t nblk,
                                               Visit visit = Visit()) {
+    // Fewer blocks than one turn of the ring plus the look-ahead: one load
+    // and compression after the other.  Longer messages fill the ring with
+    // unconditional loads in block order, so that the compiler's wait
+    // counts at the main loop's top stay 8 blocks deep (after loads that
+    // may have been skipped or were reordered, it waits there until every
+    // load but the newest is done, which drains the ring once a turn).
+    if (nblk < (uint32_t)(kMd5Ring + kMd5Ahead)) {
+        for (uint32_t b = 0; b < nblk; b++) {
+            uint32_t x[16];
+            md5_load_block(x, addr(b));
+            visit(x);
+            md5_compress(h, x);
+        }
+        return;
+    }
     uint32_t buf[kMd5Ring][16];
+    // in block order: block 0 must be the oldest load when it is compressed
 #pragma unroll
-    for (int k = 0; k < kMd5Ahead; k++)
-        if ((uint32_t)k < nblk) md5_load_block(buf[k], addr((uint32_t)k));
+    for (int k = 0; k < kMd5Ahead; k++) {
+        md5_load_block(buf[k], addr((uint32_t)k));
+        __builtin_amdgcn_sched_barrier(0);
+    }
     uint32_t b = 0;
     for (; b + kMd5Ring + kMd5Ahead <= nblk; b += kMd5Ring) {
 #pragma unroll
         for (int k = 0; k < kMd5Ring; k++) {
             md5_load_block(buf[(k + kMd5Ahead) % kMd5Ring], addr(b + k + kMd5Ahead));
+            // keeps each load at the top of its block: left free, the
+            // scheduler gathers a whole ring's loads at the loop's end and
+            // the loop top then waits a full memory latency for them
+            __builtin_amdgcn_sched_barrier(0);
             visit(buf[k]);
             md5_compress(h, buf[k]);
+            __builtin_amdgcn_sched_barrier(0);
         }
     }
     for (; b < nblk; b += kMd5Ring) {
@@ -1246,24 +1269,37 @@ __device__ __forceinline__ void md5_packet_bulk16(uint32_t h[4], const uint8_t *
     if (nblk == 0) return;
     uint32_t carry = gload(reinterpret_cast<const uint32_t *>(data + 4 * (16 * (size_t)k0 - 9)));
     const uint4 *base = reinterpret_cast<const uint4 *>(data + 64 * (size_t)k0 - 32);
-    uint32_t buf[kMd5Ring][16];  // block j in slot j % kMd5Ring, as in md5_blocks_at
-#pragma unroll
-    for (int k = 0; k < kMd5Ahead; k++)
-        if ((uint32_t)k < nblk) md5_load_block(buf[k], base + 4 * k);
-    auto step = [&](int k) {
+    auto step_words = [&](const uint32_t y[16]) {
         uint32_t x[16];
         x[0] = carry;
 #pragma unroll
-        for (int w = 1; w < 16; w++) x[w] = buf[k][w - 1];
-        carry = buf[k][15];
+        for (int w = 1; w < 16; w++) x[w] = y[w - 1];
+        carry = y[15];
         md5_compress(h, x);
     };
+    if (nblk < (uint32_t)(kMd5Ring + kMd5Ahead)) {  // short: as in md5_blocks_at
+        for (uint32_t k = 0; k < nblk; k++) {
+            uint32_t y[16];
+            md5_load_block(y, base + 4 * (size_t)k);
+            step_words(y);
+        }
+        return;
+    }
+    uint32_t buf[kMd5Ring][16];  // block j in slot j % kMd5Ring, as in md5_blocks_at
+#pragma unroll
+    for (int k = 0; k < kMd5Ahead; k++) {  // in block order, as in md5_blocks_at
+        md5_load_block(buf[k], base + 4 * k);
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    auto step = [&](int k) { step_words(buf[k]); };
     uint32_t b = 0;
     for (; b + kMd5Ring + kMd5Ahead <= nblk; b += kMd5Ring) {
 #pragma unroll
         for (int k = 0; k < kMd5Ring; k++) {
             md5_load_block(buf[(k + kMd5Ahead) % kMd5Ring], base + 4 * (size_t)(b + k + kMd5Ahead));
+            __builtin_amdgcn_sched_barrier(0);  // as in md5_blocks_at
             step(k);
+            __builtin_amdgcn_sched_barrier(0);
         }
     }
     for (; b < nblk; b += kMd5Ring) {
