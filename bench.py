@@ -3,9 +3,11 @@
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload headline]
 
 Besides the headline line it reports, in the same JSON object: the whole
-create's wall time on the headline file (end_to_end_create), the batched
-job mix against the same jobs run one at a time (batched), and a serving run
-with Poisson arrivals and a 32 GiB job (serving); flags skip each.
+create's wall time on the headline file (end_to_end_create), the device pass
+over the many-files workload (64 files, S = 64 KiB, R = 1000: many_files),
+the batched job mix against the same jobs run one at a time (batched), and a
+serving run with Poisson arrivals and a 32 GiB job (serving); flags skip
+each.
 
 A step is one pass of the device hot path over the workload resident in
 HBM: the GF(2^16) recovery encode (K1), the per-slice MD5 + CRC-32 of the IFSC
@@ -67,6 +69,17 @@ def measured_traffic(kernel: str):
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def run_batch_mix(ctx, n_jobs: int, reps: int) -> dict:
@@ -284,6 +297,49 @@ def _same_alone(ctx, srv, rid, m, sid) -> bool:
     return served == alone
 
 
+def run_many_files(ctx, reps: int = 3) -> dict:
+    """The many-files workload (synth.config2_files: 64 files log-uniform in
+    [1 KiB, 64 MiB], 8 all-zero, 4 zero-length and left out as the
+    reference's scan rejects them; S = 64 KiB, R = 1000), written on the
+    device and run through p2g_dev_create_pass: more than 2048 slices, so a
+    one-job batched launch.  Bytes against the oracle are checked by
+    tests/test_gpu_fullsize.py."""
+    from par2rust_amd import hipmem, synth
+
+    S, R = 65536, 1000
+    ents = [e for e in synth.config2_files() if e[2] != "empty"]
+    offs, o = [], 0
+    for _, sz, _, _ in ents:
+        offs.append(o)
+        o += (sz + 255) // 256 * 256
+    arena = hipmem.DeviceBuffer(o)
+    arena.zero()
+    for (_, sz, kind, seed), off in zip(ents, offs):
+        if kind == "random":
+            ctx.dev_fill_xorshift64(arena.addr + off, sz, seed)
+    files = [(arena.addr + off, sz) for (_, sz, _, _), off in zip(ents, offs)]
+    n = sum(-(-sz // S) for _, sz, _, _ in ents)
+    src = sum(sz for _, sz, _, _ in ents)
+    outs = [hipmem.DeviceBuffer(R * S), hipmem.DeviceBuffer(16 * n), hipmem.DeviceBuffer(4 * n),
+            hipmem.DeviceBuffer(16 * R)]
+    sid = bytes(range(16))
+    ctx.dev_create_pass(files, S, R, sid, *outs)
+    ctx.synchronize()
+    t = time.perf_counter()
+    for _ in range(reps):
+        ctx.dev_create_pass(files, S, R, sid, *outs)
+        ctx.synchronize()
+    dt = (time.perf_counter() - t) / reps
+    ops = 256.0 * n * R * S
+    log(f"bench: many files {len(files)} files {src / 1e6:.1f} MB N={n} R={R}: {dt * 1e3:.2f} ms")
+    return {"files": len(files), "source_bytes": src, "slice_size": S, "slices": n, "recovery": R,
+            "ms_per_pass": round(dt * 1e3, 3), "source_GBps": round(src / dt / 1e9, 2),
+            "gf_mul_xor_per_s": round(n * R * (S / 2) / dt, 1),
+            "fp4_TFLOPs": round(ops / dt / 1e12, 1), "fp4_frac": round(ops / dt / 1e12 / FP4_DENSE_PEAK_TFLOPS, 3),
+            "path": "p2g_dev_create_pass (one-job batched launch: N > 2048)",
+            "timing": "host wall time per call, synchronized, inputs resident in HBM"}
+
+
 def run_end_to_end(ctx, host: bytes, S: int, R: int, cpu: bool) -> dict:
     """Wall time of the whole create (SourceFile::scan of the file, then
     run_create writing the index and the volumes) on the headline file, read
@@ -355,6 +411,8 @@ def main() -> int:
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
     ap.add_argument("--serve-step-ms", type=float, default=0.0,
                     help="the server's step target (0: its default, 50 ms)")
+    ap.add_argument("--no-many-files", action="store_true",
+                    help="skip the many-files (64 files, R = 1000) pass timing")
     ap.add_argument("--no-end-to-end", action="store_true",
                     help="skip the whole-create wall-time comparison")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -505,6 +563,8 @@ def main() -> int:
 
     if not args.no_end_to_end:
         out["end_to_end_create"] = run_end_to_end(ctx, host, S, R, not args.no_cpu_baseline)
+    if not args.no_many_files and args.workload == "headline":
+        out["many_files"] = run_many_files(ctx)
     if args.batch_jobs > 0:
         out["batched"] = run_batch_mix(ctx, args.batch_jobs, args.batch_reps)
     if args.serve_seconds > 0:
@@ -521,11 +581,22 @@ def main() -> int:
         cpu_md5, cpu_crc = O.slice_checksums_mt(host, S, threads)
         cpu_pkt = O.recovery_md5_mt(cpu_rec, R, S, 0, set_id, threads)
         ct = time.perf_counter() - t
+        # the same step on one core (the encode is the whole step's bulk and
+        # scales with the thread count; the check below still uses the
+        # multi-threaded outputs)
+        t = time.perf_counter()
+        O.encode_simd([host], S, R, 1)
+        O.slice_checksums_mt(host, S, 1)
+        O.recovery_md5_mt(cpu_rec, R, S, 0, set_id, 1)
+        ct1 = time.perf_counter() - t
         out["cpu_baseline"] = {
             "value": round(nbytes / ct / 1e9, 4), "unit": "GB/s", "cores": threads,
             "kind": "port",
             "sample": f"the full {args.workload} step once (encode {O.simd_dispatch()} + "
                       f"slice md5/crc32 + packet md5) on {threads} threads, {ct:.2f}s",
+            "single_core": {"value": round(nbytes / ct1 / 1e9, 4), "unit": "GB/s",
+                            "seconds": round(ct1, 2)},
+            "host": {"cpu_model": _cpu_model(), "logical_cpus": os.cpu_count()},
         }
         out["parity"] = {
             "recovery_blocks": gpu_rec == cpu_rec,

@@ -142,7 +142,10 @@ int p2g_dev_recovery_packet_md5(p2g_ctx *ctx, const uint8_t *rec, uint32_t count
  * hashes of build_volume_packet (src/creator.rs:1031-1041) together.
  * Outputs: rec (R * slice_size), slice_md5_out (16 B per slice),
  * slice_crc_out (one per slice), pkt_md5_out (16 B per packet), all device
- * memory.  p2g_synchronize reports a pass that could not complete. */
+ * memory.  A set of more than 2048 slices (or whose coefficient fragments
+ * exceed the pass's budget) runs as a one-job batched launch
+ * (p2g_dev_create_batch), with the same results.  p2g_synchronize reports a
+ * pass that could not complete. */
 int p2g_dev_create_pass(p2g_ctx *ctx, const p2g_dev_file *files, uint32_t nfiles,
                         uint64_t slice_size, uint32_t recovery_count, const uint8_t set_id[16],
                         uint8_t *rec, uint8_t *slice_md5_out, uint32_t *slice_crc_out,
@@ -152,7 +155,8 @@ int p2g_dev_create_pass(p2g_ctx *ctx, const p2g_dev_file *files, uint32_t nfiles
  * of this context has finished (it waits for that), the first start and the
  * last end of its packet-MD5, slice-MD5, encode and CRC work, in ms from the
  * earliest start: ms_out[2k] start and ms_out[2k + 1] end of role k, 0 for a
- * role with no work (or when the pass ran as separate kernels). */
+ * role with no work (all 0 when the last pass ran as separate kernels or as
+ * a batched launch, or when none has run). */
 int p2g_pass_timing(p2g_ctx *ctx, double *ms_out);
 
 /* ---------------------------------------------------------------------

@@ -482,7 +482,10 @@ int p2g_dev_recovery_packet_md5(p2g_ctx *ctx, const uint8_t *rec, uint32_t count
 int p2g_pass_timing(p2g_ctx *ctx, double *ms_out) {
     Ctx *c = reinterpret_cast<Ctx *>(ctx);
     if (!c || !ms_out) return P2G_ERR_INVALID_ARGUMENT;
-    if (!c->d_sync.ptr) return c->fail(P2G_ERR_INVALID_ARGUMENT, "no fused create pass has run");
+    if (!c->last_pass_fused || !c->d_sync.ptr) {  // separate kernels, batched, or none yet
+        for (uint32_t k = 0; k < 2 * kPassRoles; k++) ms_out[k] = 0.0;
+        return P2G_OK;
+    }
     hipSetDevice(c->device);
     hipError_t e = hipEventSynchronize(c->scratch_done[Ctx::kScratchEnc]);
     unsigned long long t[2 * kPassRoles];
@@ -527,6 +530,28 @@ int create_pass_impl(Ctx *c, const p2g_dev_file *files, uint32_t nfiles, uint64_
         return c->fail(P2G_ERR_TOO_MANY_INPUT_BLOCKS, "more than 32768 input slices");
     const uint32_t n = (uint32_t)descs.size();
     const uint32_t S = (uint32_t)slice_size;
+    PassPlan plan;
+    const bool fits_pass = plan_create_pass(n, R, S, c->num_cus, band_cols, &plan) &&
+                           mfma_afrag_bytes(n, R) <= kPassAfragBudget;
+    c->last_pass_fused = variant == kEncodeMfma && fits_pass;
+    if (variant == kEncodeMfma && !fits_pass && n && R) {
+        // More slices than the pass's descriptor table holds (or fragments
+        // above its budget): the batched pass runs the set as one job.  Its
+        // units take the slices in 2048-slice chunks and its pieces split
+        // the rows by a fragment budget, so the encode stays on the LDS-DMA
+        // loop and the hashes stay in the same launches.
+        p2g_batch_job job;
+        job.files = files;
+        job.slice_size = slice_size;
+        job.rec = rec;
+        job.slice_md5 = slice_md5_out;
+        job.slice_crc = slice_crc_out;
+        job.pkt_md5 = pkt_md5_out;
+        job.nfiles = nfiles;
+        job.recovery_count = R;
+        memcpy(job.set_id, set_id, 16);
+        return create_pass_batched(c, job, st);
+    }
     rc = c->scratch_acquire(Ctx::kScratchEnc, st);
     if (rc != P2G_OK) return rc;
     // Partial slices (the last one of each file) are staged into zero-padded
@@ -561,11 +586,7 @@ int create_pass_impl(Ctx *c, const p2g_dev_file *files, uint32_t nfiles, uint64_
     uint4 *tb = static_cast<uint4 *>(c->enc_descs.tb.ptr);
     hipError_t e;
 
-    PassPlan plan;
-    const bool fused = variant == kEncodeMfma &&
-                       plan_create_pass(n, R, S, c->num_cus, band_cols, &plan) &&
-                       mfma_afrag_bytes(n, R) <= kPassAfragBudget;
-    if (!fused) {
+    if (!c->last_pass_fused) {
         // The same results from the separate kernels, in stream order.
         rc = dev_encode(c, dd, n, slice_size, 0, n, R, rec, 0, st, variant);
         if (rc != P2G_OK) return rc;

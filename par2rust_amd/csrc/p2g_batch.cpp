@@ -418,6 +418,28 @@ int wait_plan(BatchPlan *B) {
     return P2G_OK;
 }
 
+int create_pass_batched(Ctx *c, const p2g_batch_job &job, hipStream_t st) {
+    // keep at most two such plans (and their fragment buffers) alive for a
+    // caller that queues many sets before synchronizing
+    while (c->pending_batches.size() >= 2) {
+        BatchPlan *old = c->pending_batches.front();
+        c->pending_batches.erase(c->pending_batches.begin());
+        const int rc = wait_plan(old);
+        free_batch_plan(old);
+        if (rc != P2G_OK) return rc;
+    }
+    BatchPlan *B = nullptr;
+    int rc = make_plan(c, &job, 1, false, &B);
+    if (rc != P2G_OK) return rc;
+    rc = launch_plan(B, st);
+    if (rc != P2G_OK) {
+        delete B;
+        return rc;
+    }
+    c->pending_batches.push_back(B);  // p2g_synchronize waits, checks and frees it
+    return P2G_OK;
+}
+
 }  // namespace p2g
 
 using namespace p2g;

@@ -75,6 +75,7 @@ struct Ctx {
     DevBuf d_partials;             // zero-padded copies of partial slices (fused pass)
     DevBuf d_xs_jumps;             // xorshift64 jump matrices (synthetic input)
     bool pass_pending = false;     // a fused pass may have set error flags not yet read
+    bool last_pass_fused = false;  // the last p2g_dev_create_pass ran as one k_create_pass
     // Shared device buffers and the event recorded after the last kernel that
     // reads them; an entry point that rewrites them first makes its stream
     // wait for that event, so calls on different streams never overlap on
@@ -126,6 +127,10 @@ int build_plan(Ctx *c, const std::vector<PlanItem> &items, bool use_graph, Batch
 int launch_plan(BatchPlan *b, hipStream_t st);
 int wait_plan(BatchPlan *b);  // waits for the last launch, reads its flags and stamps
 void free_batch_plan(BatchPlan *b);
+// One set as a one-job batched plan launched on st (p2g_dev_create_pass for
+// sets the fused pass cannot hold); the plan is freed by p2g_synchronize, or
+// here once more than two such plans are pending.
+int create_pass_batched(Ctx *c, const p2g_batch_job &job, hipStream_t st);
 
 // Appends the descriptors of every slice of `files` (file order, slice order).
 int append_descs(Ctx *c, const p2g_dev_file *files, uint32_t nfiles, uint64_t S,

@@ -1,7 +1,8 @@
 """GPU parity of the fused create pass (p2g_dev_create_pass): recovery blocks,
 slice MD5 / CRC-32 and recovery packet MD5s of one launch, bit-exact against
 the oracle, across band widths, ragged and unaligned slices, several packet
-and slice hashing workgroups, and the fallback to separate kernels."""
+and slice hashing workgroups, sets above one descriptor table (a one-job
+batched launch) and the bit-sliced variant's separate kernels."""
 import os
 import struct
 
@@ -101,12 +102,55 @@ def test_create_pass_many_bands(band_ctx, band_cols, lens, S, R):
     assert run_pass(c, files, S, R, sid) == expected(files, S, R, sid)
 
 
-def test_create_pass_fallback_beyond_one_launch(ctx):
-    # 3000 inputs exceed one fused launch: the entry point runs the separate kernels
-    S, R = 4, 6
-    files = [data(3000 * S, 90)]
+@pytest.mark.parametrize("lens,S,R", [
+    ([3000 * 4], 4, 6),                          # 3000 slices of 4 bytes
+    ([2500 * 64 + 10, 700 * 64 + 3], 64, 40),    # two files, ragged tails, 3202 slices
+    ([5000 * 100 + 52], 100, 17),                # S % 16 != 0, 5001 slices
+    ([9000 * 16], 16, 300),                      # 9000 slices: 5 chunks of 2048 per unit
+])
+def test_create_pass_beyond_one_descriptor_table(ctx, lens, S, R):
+    # more than 2048 slices: the entry point runs the set as a one-job batched launch
+    files = [data(n, 90 + i) for i, n in enumerate(lens)]
     sid = data(16, 91)
     assert run_pass(ctx, files, S, R, sid) == expected(files, S, R, sid)
+    assert ctx.pass_timing()["encode"] == (0.0, 0.0)  # no fused pass ran
+
+
+def test_create_pass_large_sets_queued(ctx):
+    # three large sets queued before one synchronize: at most two plans stay
+    # pending, the oldest is retired by the next call; every output is whole
+    from par2rust_amd.hipmem import DeviceBuffer
+
+    S, R = 32, 9
+    sets = [[data(2100 * S + 8 * k, 600 + k)] for k in range(3)]
+    sid = data(16, 95)
+    outs = []
+    for files in sets:
+        dev = [DeviceBuffer.from_bytes(f) for f in files]
+        n = sum((len(f) + S - 1) // S for f in files)
+        o = (DeviceBuffer(R * S), DeviceBuffer(16 * n), DeviceBuffer(4 * n), DeviceBuffer(16 * R))
+        ctx.dev_create_pass([(d.addr, len(f)) for d, f in zip(dev, files)], S, R, sid, *o)
+        outs.append((dev, n, o))
+    ctx.synchronize()
+    for files, (_, n, (rec, md5, crc, pkt)) in zip(sets, outs):
+        got = (rec.download(R * S), md5.download(16 * n),
+               list(struct.unpack(f"<{n}I", crc.download(4 * n))), pkt.download(16 * R))
+        assert got == expected(files, S, R, sid)
+
+
+def test_create_pass_bitslice_beyond_one_launch(monkeypatch):
+    # the bit-sliced variant runs the separate kernels at any size
+    import par2rust_amd as p2
+
+    monkeypatch.setenv("P2G_ENCODE", "bitslice")
+    c = p2.Context(0)
+    try:
+        S, R = 4, 6
+        files = [data(3000 * S, 96)]
+        sid = data(16, 97)
+        assert run_pass(c, files, S, R, sid) == expected(files, S, R, sid)
+    finally:
+        c.close()
 
 
 def test_create_pass_twice_reuses_state(ctx):
