@@ -82,7 +82,7 @@ def _cpu_model() -> str:
     return "unknown"
 
 
-def run_batch_mix(ctx, n_jobs: int, reps: int) -> dict:
+def run_batch_mix(ctx, n_jobs: int, reps: int, tune: bool = True) -> dict:
     """The batched-create comparison: the job mix of synth.batch_mix, every
     file generated on the device, run (1) one job at a time through the plain
     device path (p2g_dev_create_pass, then p2g_synchronize, per job) and (2)
@@ -90,7 +90,13 @@ def run_batch_mix(ctx, n_jobs: int, reps: int) -> dict:
     (3) as replays of a planned batch (HIP graph).  Jobs/s = jobs / wall time
     of the whole run; latency of a job = its completion time from the start
     of the run, all jobs being submitted together at the start.  Every job's
-    outputs of (2) are compared with those of (1)."""
+    outputs of (2) are compared with those of (1).
+
+    With `tune`, every (N, R, S) bucket of the mix is first tuned on
+    synthetic input of its largest job's shape (p2g_autotune); both runs then
+    take their choices from that cache: the plain path its variant and band
+    width, the batched planner its band widths (the batched kernel is
+    MFMA-only)."""
     import numpy as np
 
     import par2rust_amd as p2
@@ -120,6 +126,25 @@ def run_batch_mix(ctx, n_jobs: int, reps: int) -> dict:
     ctx.synchronize()
     log(f"bench: batch mix of {n_jobs} jobs, {src_bytes / 1e9:.2f} GB, generated on the device "
         f"in {time.perf_counter() - t:.1f}s")
+
+    tuned = None
+    if tune:
+        t = time.perf_counter()
+        shapes = {}
+        for j in jobs:
+            b = p2.tune_bucket(j["n"], j["R"], j["S"])
+            shapes[b] = max(shapes.get(b, (0, 0, 0)), (j["n"], j["R"], j["S"]))
+        picks = [ctx.autotune(*shape) for shape in shapes.values()]
+        t_tune = time.perf_counter() - t
+        bands = {}
+        for c in picks:
+            if c["variant"] == p2.VARIANT_MFMA:
+                bands[str(c["band_cols"])] = bands.get(str(c["band_cols"]), 0) + 1
+        tuned = {"buckets": len(picks), "seconds": round(t_tune, 2),
+                 "mfma_pass": sum(1 for c in picks if c["variant"] == p2.VARIANT_MFMA),
+                 "bitslice": sum(1 for c in picks if c["variant"] == p2.VARIANT_BITSLICE),
+                 "mfma_band_cols": bands}
+        log(f"bench: autotuned {len(picks)} buckets in {t_tune:.1f}s: {tuned}")
 
     def plain_run():
         lat = []
@@ -171,7 +196,7 @@ def run_batch_mix(ctx, n_jobs: int, reps: int) -> dict:
 
     pct = lambda v, q: round(float(np.percentile(np.array(v) * 1e3, q)), 2)
     return {
-        "jobs": n_jobs, "source_bytes": src_bytes,
+        "jobs": n_jobs, "source_bytes": src_bytes, "autotune": tuned,
         "slice_sizes": {str(S): sum(1 for j in jobs if j["S"] == S) for S in synth.BATCH_SLICE_SIZES},
         "plain_jobs_per_s": round(n_jobs / plain_wall, 2),
         "batched_jobs_per_s": round(n_jobs / batch_wall, 2),
@@ -404,6 +429,8 @@ def main() -> int:
     ap.add_argument("--batch-jobs", type=int, default=256,
                     help="jobs of the batched-create comparison (0 skips it)")
     ap.add_argument("--batch-reps", type=int, default=3)
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="run the batch mix without tuning its buckets first")
     ap.add_argument("--serve-seconds", type=float, default=10.0,
                     help="arrival window of the serving run (0 skips it)")
     ap.add_argument("--serve-load", type=float, default=0.5,
@@ -566,7 +593,8 @@ def main() -> int:
     if not args.no_many_files and args.workload == "headline":
         out["many_files"] = run_many_files(ctx)
     if args.batch_jobs > 0:
-        out["batched"] = run_batch_mix(ctx, args.batch_jobs, args.batch_reps)
+        out["batched"] = run_batch_mix(ctx, args.batch_jobs, args.batch_reps,
+                                       tune=not args.no_autotune)
     if args.serve_seconds > 0:
         cap = out["batched"]["batched_jobs_per_s"] if "batched" in out else 400.0
         out["serving"] = run_serving(ctx, args.serve_seconds, args.serve_load * cap,

@@ -75,6 +75,7 @@ int tune_and_run(Ctx *c, uint64_t n, uint32_t R, uint64_t S, hipStream_t st,
             t_best = std::min(
                 t_best,
                 std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+            if (t_best > 2 * best) break;  // clearly slower: no second measurement
         }
         if (t_best < best) {
             best = t_best;
