@@ -317,13 +317,12 @@ __device__ __forceinline__ void mf_b_words(uint32_t w, uint32_t out[4]) {
 }
 
 // A dword d of matrix row m (bit b = coefficient of symbol bit b), in the
-// same nibble order.
-__device__ __forceinline__ uint32_t mf_a_word(uint32_t m, int d) {
-    uint32_t out = 0;
-#pragma unroll
-    for (int n = 0; n < 8; n++)
-        out |= ((m >> (4 * (n & 3) + (n >> 2) + 2 * d)) & 1u) << (4 * n + 1);
-    return out;
+// same nibble order: nibble n holds bit 4 (n % 4) + n / 4 + 2 d of m at its
+// bit 1, so dword 0 takes bits 4q (nibbles 0-3) and 4q + 1 (nibbles 4-7),
+// dword 1 bits 4q + 2 and 4q + 3.
+__host__ __device__ __forceinline__ uint32_t mf_a_word(uint32_t m, int d) {
+    return d == 0 ? ((m & 0x1111u) << 1) | ((m & 0x2222u) << 16)
+                  : ((m & 0x4444u) >> 1) | ((m & 0x8888u) << 14);
 }
 
 // E8M0 block scale of 1.0 (biased exponent 127) for both operands; a scale
@@ -336,24 +335,73 @@ constexpr int kMfThreads = 256;  // 4 waves, each 64 symbols (two 32-column tile
 // A fragments: frag (t, g) is 64 lanes x 16 bytes at afrag[(t * nq + g) * 64 + lane].
 // Entry idx = (t * nq + g) * 64 + lane.  Row r of the fragments is recovery
 // exponent first_exp + r; rows 2t + rr >= R and inputs >= n are zero.
-__device__ __forceinline__ uint4 mfma_afrag_entry(const uint16_t *__restrict__ antilog,
+// Row j of the GF(2) matrix of multiplication by x^k, k, j < 16: bit b is
+// bit j of x^(k + b) mod the field polynomial.  The matrix of multiplication
+// by v is the sum of those of the x^k for the bits k of v, so a row of it is
+// the XOR of 16 such rows selected by v's bits.
+struct MulXRows {
+    uint16_t row[16][16];  // [j][k]
+};
+constexpr MulXRows make_mulx_rows() {
+    MulXRows t{};
+    uint32_t pw[31] = {};
+    uint32_t v = 1;
+    for (int e = 0; e < 31; e++) {
+        pw[e] = v;
+        v = (v << 1) ^ ((v & 0x8000u) ? kFieldGenerator : 0u);
+    }
+    for (int j = 0; j < 16; j++)
+        for (int k = 0; k < 16; k++) {
+            uint32_t m = 0;
+            for (int b = 0; b < 16; b++) m |= ((pw[k + b] >> j) & 1u) << b;
+            t.row[j][k] = (uint16_t)m;
+        }
+    return t;
+}
+__constant__ MulXRows kMulXRows = make_mulx_rows();
+
+// x mod 65535 for x < 2^32 (65536 = 1 mod 65535): two folds of the halves.
+__host__ __device__ __forceinline__ uint32_t mod_field_limit(uint32_t x) {
+    x = (x & 0xFFFFu) + (x >> 16);
+    x = (x & 0xFFFFu) + (x >> 16);
+    return x == kFieldLimit ? 0u : x;
+}
+
+// The rows of the x^k matrices a fragment lane uses: lane l holds matrix row
+// j = l % 16 of its coefficients in every fragment.
+struct AfragLane {
+    uint32_t rows[16];
+    __device__ __forceinline__ void load(uint32_t lane) {
+        const uint32_t j = lane & 15;
+#pragma unroll
+        for (int k = 0; k < 16; k++) rows[k] = kMulXRows.row[j][k];
+    }
+};
+
+// Lane `lane` of fragment (t, g): matrix row j = lane % 16 of the two
+// coefficients of recovery row r = 2 t + lane / 16 % 2 and inputs
+// 4 g + 2 h, 4 g + 2 h + 1 (h = lane / 32).
+__device__ __forceinline__ uint4 mfma_afrag_entry(const AfragLane &L,
+                                                  const uint16_t *__restrict__ antilog,
                                                   const uint16_t *__restrict__ logbase, uint32_t n,
-                                                  uint32_t R, uint32_t first_exp, uint32_t nq,
-                                                  uint64_t idx) {
-    const uint32_t lane = (uint32_t)(idx & 63);
-    const uint64_t tg = idx >> 6;
-    const uint32_t g = (uint32_t)(tg % nq), t = (uint32_t)(tg / nq);
+                                                  uint32_t R, uint32_t first_exp, uint32_t t,
+                                                  uint32_t g, uint32_t lane) {
     const uint32_t row = lane & 31, h = lane >> 5;
-    const uint32_t r = 2 * t + (row >> 4), j = row & 15;
+    const uint32_t r = 2 * t + (row >> 4);
     uint32_t d[4] = {0, 0, 0, 0};
+#pragma unroll
     for (uint32_t e = 0; e < 2; e++) {
         const uint32_t i = 4 * g + 2 * h + e;
         if (r >= R || i >= n) continue;
-        uint32_t v = antilog[(uint32_t)(((uint64_t)logbase[i] * (first_exp + r)) % kFieldLimit)];
+        // exponents and logs are below 65535, so the product fits 32 bits
+        const uint32_t v = antilog[mod_field_limit((uint32_t)logbase[i] * (first_exp + r))];
         uint32_t m = 0;  // row j of the matrix of multiplication by v
-        for (int b = 0; b < 16; b++) {
-            m |= ((v >> j) & 1u) << b;
-            v = ((v << 1) & 0xFFFFu) ^ ((v & 0x8000u) ? (kFieldGenerator & 0xFFFFu) : 0u);
+#pragma unroll
+        for (int k = 0; k < 16; k++) {
+            // all ones when bit k of v is set; then m ^= rows[k] & mask as one
+            // v_bitop3 (truth-table bit (m << 2 | row << 1 | mask): 0x78)
+            const uint32_t mask = (uint32_t)__builtin_amdgcn_sbfe((int)v, k, 1);
+            m = __builtin_amdgcn_bitop3_b32(m, L.rows[k], mask, 0x78);
         }
         d[2 * e] = mf_a_word(m, 0);
         d[2 * e + 1] = mf_a_word(m, 1);
@@ -361,14 +409,37 @@ __device__ __forceinline__ uint4 mfma_afrag_entry(const uint16_t *__restrict__ a
     return make_uint4(d[0], d[1], d[2], d[3]);
 }
 
-// Thread per fragment entry.
+// Every fragment (t, g) of tiles x nq, entry (t * nq + g) * 64 + lane: a
+// thread keeps its lane and walks fragments frag0, frag0 + F, ... (F = the
+// grid's waves), stepping (t, g) without a division.
+__device__ __forceinline__ void afrag_fill(const AfragLane &L, const uint16_t *__restrict__ antilog,
+                                           const uint16_t *__restrict__ logbase, uint32_t n,
+                                           uint32_t R, uint32_t first_exp, uint32_t nq,
+                                           uint32_t tiles, uint4 *__restrict__ out) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t frag0 = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t F = (gridDim.x * blockDim.x) >> 6;
+    const uint32_t dt = F / nq, dg = F % nq;
+    uint32_t t = frag0 / nq, g = frag0 % nq;
+    while (t < tiles) {
+        out[((size_t)t * nq + g) * 64 + lane] = mfma_afrag_entry(L, antilog, logbase, n, R, first_exp, t, g, lane);
+        t += dt;
+        g += dg;
+        if (g >= nq) {
+            g -= nq;
+            t++;
+        }
+    }
+}
+
+// Grid-stride over the fragments; the block size is a multiple of 64.
 __global__ __launch_bounds__(256) void k_mfma_afrag(const uint16_t *__restrict__ antilog,
                                                     const uint16_t *__restrict__ logbase,
                                                     uint32_t n, uint32_t R, uint32_t nq,
                                                     uint32_t tiles, uint4 *__restrict__ afrag) {
-    const uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (uint64_t)tiles * nq * 64) return;
-    afrag[idx] = mfma_afrag_entry(antilog, logbase, n, R, 0, nq, idx);
+    AfragLane L;
+    L.load(threadIdx.x & 63);
+    afrag_fill(L, antilog, logbase, n, R, 0, nq, tiles, afrag);
 }
 
 hipError_t launch_mfma_afrag(const uint16_t *d_antilog, const uint16_t *d_logbase, uint32_t n,
@@ -377,8 +448,9 @@ hipError_t launch_mfma_afrag(const uint16_t *d_antilog, const uint16_t *d_logbas
     const uint32_t tiles = mfma_row_tiles(R);
     const uint64_t total = (uint64_t)tiles * nq * 64;
     if (total == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_mfma_afrag, dim3((uint32_t)((total + 255) / 256)), dim3(256), 0, stream,
-                       d_antilog, d_logbase, n, R, nq, tiles, d_afrag);
+    const uint32_t blocks = (uint32_t)std::min<uint64_t>((total + 255) / 256, 16384);
+    hipLaunchKernelGGL(k_mfma_afrag, dim3(blocks), dim3(256), 0, stream, d_antilog, d_logbase, n,
+                       R, nq, tiles, d_afrag);
     return hipGetLastError();
 }
 
@@ -1944,14 +2016,12 @@ __global__ __launch_bounds__(256) void k_batch_afrag(const uint16_t *__restrict_
                                                      const uint16_t *__restrict__ logbase,
                                                      const BatchPiece *__restrict__ pieces,
                                                      uint32_t n_pieces) {
-    const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const uint64_t gsz = (uint64_t)gridDim.x * blockDim.x;
+    AfragLane L;
+    L.load(threadIdx.x & 63);
     for (uint32_t p = 0; p < n_pieces; p++) {
         const BatchPiece &P = pieces[p];
-        const uint64_t total = (uint64_t)mfma_row_tiles(P.R) * P.nq * 64;
-        uint4 *out = static_cast<uint4 *>(const_cast<void *>(P.afrag));
-        for (uint64_t idx = gid; idx < total; idx += gsz)
-            out[idx] = mfma_afrag_entry(antilog, logbase, P.n, P.R, P.first_exp, P.nq, idx);
+        afrag_fill(L, antilog, logbase, P.n, P.R, P.first_exp, P.nq, mfma_row_tiles(P.R),
+                   static_cast<uint4 *>(const_cast<void *>(P.afrag)));
     }
 }
 
