@@ -4,7 +4,8 @@
 
 Besides the headline line it reports, in the same JSON object: the whole
 create's wall time on the headline file (end_to_end_create), the device pass
-over the many-files workload (64 files, S = 64 KiB, R = 1000: many_files),
+over the many-files workload (64 files, S = 64 KiB, R = 1000: many_files,
+with its whole create from files),
 the batched job mix against the same jobs run one at a time (batched), and a
 serving run with Poisson arrivals and a 32 GiB job (serving); flags skip
 each.
@@ -365,6 +366,46 @@ def run_many_files(ctx, reps: int = 3) -> dict:
             "timing": "host wall time per call, synchronized, inputs resident in HBM"}
 
 
+def run_many_files_end_to_end(ctx) -> dict:
+    """Wall time of the whole create (p2g_run_create, every file read once
+    into HBM) of the many-files workload (the 60 non-empty files of
+    synth.config2_files, S = 64 KiB, R = 1000, uniform split into 15
+    volumes) from the page cache of a temporary directory.  The bytes are
+    checked against the committed manifest by tests/test_gpu_fullsize.py."""
+    import shutil
+    import tempfile
+
+    import par2rust_amd as p2
+    from par2rust_amd import synth
+
+    tmp = tempfile.mkdtemp(prefix="p2g_e2e2_")
+    try:
+        inputs, total = [], 0
+        for e in synth.config2_files():
+            if e[2] == "empty":
+                continue
+            path = os.path.join(tmp, e[0])
+            with open(path, "wb") as f:
+                f.write(synth.config2_bytes(e))
+            inputs.append((path, e[0].encode()))
+            total += e[1]
+        os.makedirs(os.path.join(tmp, "out"))
+        opts = p2.CreateOptions(os.path.join(tmp, "out", "recovery.par2"), 65536, 1000,
+                                p2.VolumeScheme.uniform(15))
+        best = 1e9
+        for _ in range(2):
+            t = time.perf_counter()
+            ctx.run_create(opts, inputs)
+            best = min(best, time.perf_counter() - t)
+        out = {"files": len(inputs), "file_bytes": total, "slice_size": 65536, "recovery": 1000,
+               "gpu_resident_s": round(best, 3), "gpu_resident_GBps": round(total / best / 1e9, 3),
+               "source": "page-cached files in a temporary directory"}
+        log(f"bench: many files end to end {out}")
+        return out
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+
+
 def run_end_to_end(ctx, host: bytes, S: int, R: int, cpu: bool) -> dict:
     """Wall time of the whole create (SourceFile::scan of the file, then
     run_create writing the index and the volumes) on the headline file, read
@@ -592,6 +633,8 @@ def main() -> int:
         out["end_to_end_create"] = run_end_to_end(ctx, host, S, R, not args.no_cpu_baseline)
     if not args.no_many_files and args.workload == "headline":
         out["many_files"] = run_many_files(ctx)
+        if not args.no_end_to_end:
+            out["many_files"]["end_to_end_create"] = run_many_files_end_to_end(ctx)
     if args.batch_jobs > 0:
         out["batched"] = run_batch_mix(ctx, args.batch_jobs, args.batch_reps,
                                        tune=not args.no_autotune)

@@ -562,21 +562,26 @@ int create_resident(p2g_ctx *ctx, const CreateOptions &opts, const std::vector<s
     uint32_t *d_crc = reinterpret_cast<uint32_t *>(d_md5 + 16 * n);
     uint8_t *d_pkt = reinterpret_cast<uint8_t *>(d_crc + n);
 
-    // read once: pinned ring -> HBM, and the whole-file hashes on a thread
-    constexpr int kSlots = 4;
-    Pinned slot[kSlots];
-    hipEvent_t copied[kSlots] = {};
+    // Read once: a ring of pinned buffers -> HBM, and the whole-file hashes
+    // on host threads.  A file's chunks are hashed in order by one thread
+    // (file f by thread f % H), so different files hash in parallel; a
+    // buffer is refilled once its copy and its hashing are done.
+    const int H = (int)std::min<size_t>(nf, 6);  // hashing threads
+    const int kSlots = H > 1 ? H + 2 : 4;        // pinned 64 MiB buffers
+    std::vector<Pinned> slot(kSlots);
+    std::vector<hipEvent_t> copied(kSlots, nullptr);
+    std::vector<char> busy(kSlots, 0);  // filled and not yet hashed
     hipStream_t copy = nullptr;
     hipError_t he = hipStreamCreateWithFlags(&copy, hipStreamNonBlocking);
     for (int k = 0; k < kSlots && he == hipSuccess; k++)
         he = hipEventCreateWithFlags(&copied[k], hipEventDisableTiming);
     struct Cleanup {
         hipStream_t &s;
-        hipEvent_t *e;
+        std::vector<hipEvent_t> &e;
         ~Cleanup() {
             if (s) hipStreamSynchronize(s);
-            for (int k = 0; k < kSlots; k++)
-                if (e[k]) hipEventDestroy(e[k]);
+            for (hipEvent_t ev : e)
+                if (ev) hipEventDestroy(ev);
             if (s) hipStreamDestroy(s);
         }
     } cleanup{copy, copied};
@@ -589,53 +594,62 @@ int create_resident(p2g_ctx *ctx, const CreateOptions &opts, const std::vector<s
     std::vector<HostMd5> full(nf), first16k(nf);
     std::mutex mu;
     std::condition_variable cv;
-    std::deque<Item> queue;
-    uint64_t hashed = 0;  // items hashed so far
+    std::vector<std::deque<Item>> queue(H);
     bool stop = false;
-    std::thread hasher([&]() {
-        for (;;) {
-            Item it;
-            {
-                std::unique_lock<std::mutex> lk(mu);
-                cv.wait(lk, [&] { return stop || !queue.empty(); });
-                if (queue.empty()) return;
-                it = queue.front();
-                queue.pop_front();
+    std::vector<std::thread> hashers;
+    for (int t = 0; t < H; t++)
+        hashers.emplace_back([&, t]() {
+            for (;;) {
+                Item it;
+                {
+                    std::unique_lock<std::mutex> lk(mu);
+                    cv.wait(lk, [&] { return stop || !queue[t].empty(); });
+                    if (queue[t].empty()) return;
+                    it = queue[t].front();
+                    queue[t].pop_front();
+                }
+                const uint8_t *p = slot[it.slot].p;
+                full[it.f].update(p, it.got);
+                if (it.at < 16384)
+                    first16k[it.f].update(p, (size_t)std::min<uint64_t>(it.got, 16384 - it.at));
+                {
+                    std::lock_guard<std::mutex> lk(mu);
+                    busy[it.slot] = 0;
+                }
+                cv.notify_all();
             }
-            const uint8_t *p = slot[it.slot].p;
-            full[it.f].update(p, it.got);
-            if (it.at < 16384)
-                first16k[it.f].update(p, (size_t)std::min<uint64_t>(it.got, 16384 - it.at));
-            {
-                std::lock_guard<std::mutex> lk(mu);
-                hashed++;
-            }
-            cv.notify_all();
-        }
-    });
-    auto finish_hasher = [&]() {
+        });
+    auto finish_hashers = [&]() {
         {
             std::lock_guard<std::mutex> lk(mu);
             stop = true;
         }
         cv.notify_all();
-        hasher.join();
+        for (auto &th : hashers) th.join();
     };
     const uint64_t chunk = kChunkBytes;
-    uint64_t items = 0;
+    int next = 0;  // slot to try first
     for (size_t f = 0; f < nf && rc == P2G_OK; f++) {
         int fd = ::open(paths[f].c_str(), O_RDONLY);
         if (fd < 0) {
             rc = set_err(err, P2G_ERR_IO, "cannot open " + paths[f] + ": " + strerror(errno));
             break;
         }
-        for (uint64_t at = 0; at < sources[f].length; at += chunk, items++) {
-            const int k = (int)(items % kSlots);
+        for (uint64_t at = 0; at < sources[f].length; at += chunk) {
+            int k = -1;
             {
-                // the slot's previous chunk must be hashed and copied
+                // a buffer whose previous chunk is hashed (its copy is waited below)
                 std::unique_lock<std::mutex> lk(mu);
-                cv.wait(lk, [&] { return hashed + kSlots > items; });
+                cv.wait(lk, [&] {
+                    for (int j = 0; j < kSlots; j++)
+                        if (!busy[(next + j) % kSlots]) {
+                            k = (next + j) % kSlots;
+                            return true;
+                        }
+                    return false;
+                });
             }
+            next = (k + 1) % kSlots;
             if (hipEventSynchronize(copied[k]) != hipSuccess) {
                 rc = set_err(err, P2G_ERR_DEVICE, "chunk copy");
                 break;
@@ -655,13 +669,14 @@ int create_resident(p2g_ctx *ctx, const CreateOptions &opts, const std::vector<s
             }
             {
                 std::lock_guard<std::mutex> lk(mu);
-                queue.push_back({k, f, got, at});
+                busy[k] = 1;
+                queue[f % H].push_back({k, f, got, at});
             }
             cv.notify_all();
         }
         ::close(fd);
     }
-    finish_hasher();
+    finish_hashers();
     if (rc != P2G_OK) return rc;
     if (hipStreamSynchronize(copy) != hipSuccess) return set_err(err, P2G_ERR_DEVICE, "chunk copies");
     for (size_t f = 0; f < nf; f++) {
