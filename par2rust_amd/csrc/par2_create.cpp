@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cerrno>
 #include <condition_variable>
 #include <cstdio>
@@ -390,11 +391,21 @@ int write_volumes(const CreateOptions &opts, const std::vector<uint32_t> &sizes,
                   const uint8_t *rec, const uint8_t *pkt_md5, std::vector<std::string> &written,
                   Error &err) {
     const uint64_t S = opts.slice_size;
-    uint32_t first = 0;
-    for (uint32_t count : sizes) {
+    // Each volume is its own file (written, fsynced and renamed by
+    // write_atomic): up to 8 are written at once.  The result is that of
+    // writing them in order: the written names up to the first volume that
+    // failed, and that volume's error.
+    const size_t nv = sizes.size();
+    std::vector<uint32_t> first(nv);
+    for (size_t v = 0, e = 0; v < nv; e += sizes[v], v++) first[v] = (uint32_t)e;
+    std::vector<std::string> names(nv);
+    std::vector<Error> errs(nv);
+    std::vector<int> rcs(nv, P2G_OK);
+    auto write_one = [&](size_t v) {
+        const uint32_t count = sizes[v], first_exp = first[v];
         std::vector<std::vector<uint8_t>> hdrs(count);
         for (uint32_t k = 0; k < count; k++) {
-            const uint32_t exp = first + k;
+            const uint32_t exp = first_exp + k;
             auto &h = hdrs[k];
             h.insert(h.end(), kMagic, kMagic + 8);
             put_le64(h, 64 + 4 + S);
@@ -406,14 +417,38 @@ int write_volumes(const CreateOptions &opts, const std::vector<uint32_t> &sizes,
         std::vector<std::pair<const uint8_t *, size_t>> parts;
         for (uint32_t k = 0; k < count; k++) {
             parts.push_back({hdrs[k].data(), hdrs[k].size()});
-            parts.push_back({rec + (size_t)(first + k) * S, (size_t)S});
+            parts.push_back({rec + (size_t)(first_exp + k) * S, (size_t)S});
         }
         parts.push_back({critical.data(), critical.size()});
-        const std::string vname = volume_name(opts.output, first, count);
-        int rc = write_atomic(vname, {&critical}, parts, err);
-        if (rc != P2G_OK) return rc;
-        written.push_back(vname);
-        first += count;
+        names[v] = volume_name(opts.output, first_exp, count);
+        rcs[v] = write_atomic(names[v], {&critical}, parts, errs[v]);
+    };
+    const size_t nthreads = std::min<size_t>(nv, 8);
+    std::vector<std::thread> pool;
+    std::mutex mu;
+    size_t next = 0;
+    for (size_t t = 0; t < nthreads; t++)
+        pool.emplace_back([&]() {
+            for (;;) {
+                size_t v;
+                {
+                    std::lock_guard<std::mutex> lk(mu);
+                    if (next == nv) return;
+                    v = next++;
+                }
+                write_one(v);
+            }
+        });
+    for (auto &th : pool) th.join();
+    for (size_t v = 0; v < nv; v++) {
+        if (rcs[v] != P2G_OK) {
+            // as if written in order: no volume after the failed one exists
+            for (size_t w = v + 1; w < nv; w++)
+                if (rcs[w] == P2G_OK) ::unlink(names[w].c_str());
+            err = errs[v];
+            return rcs[v];
+        }
+        written.push_back(names[v]);
     }
     return P2G_OK;
 }
@@ -505,6 +540,13 @@ int create_resident(p2g_ctx *ctx, const CreateOptions &opts, const std::vector<s
                     Error &err, bool *declined) {
     written.clear();
     *declined = false;
+    // P2G_CREATE_VERBOSE: phase times on stderr (read + hash, device pass,
+    // read-back, index and volume writes)
+    const bool verbose = getenv("P2G_CREATE_VERBOSE") != nullptr;
+    const auto t_start = std::chrono::steady_clock::now();
+    auto ms_since = [](std::chrono::steady_clock::time_point t) {
+        return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t).count();
+    };
     const uint64_t S = opts.slice_size;
     const size_t nf = paths.size();
     std::vector<SourceFile> sources(nf);
@@ -564,8 +606,9 @@ int create_resident(p2g_ctx *ctx, const CreateOptions &opts, const std::vector<s
 
     // Read once: a ring of pinned buffers -> HBM, and the whole-file hashes
     // on host threads.  A file's chunks are hashed in order by one thread
-    // (file f by thread f % H), so different files hash in parallel; a
-    // buffer is refilled once its copy and its hashing are done.
+    // (the one with the fewest bytes pending when the file's first chunk is
+    // read), so different files hash in parallel; a buffer is refilled once
+    // its copy and its hashing are done.
     const int H = (int)std::min<size_t>(nf, 6);  // hashing threads
     const int kSlots = H > 1 ? H + 2 : 4;        // pinned 64 MiB buffers
     std::vector<Pinned> slot(kSlots);
@@ -595,6 +638,7 @@ int create_resident(p2g_ctx *ctx, const CreateOptions &opts, const std::vector<s
     std::mutex mu;
     std::condition_variable cv;
     std::vector<std::deque<Item>> queue(H);
+    std::vector<uint64_t> pending(H, 0);  // bytes queued or being hashed per thread
     bool stop = false;
     std::vector<std::thread> hashers;
     for (int t = 0; t < H; t++)
@@ -615,6 +659,7 @@ int create_resident(p2g_ctx *ctx, const CreateOptions &opts, const std::vector<s
                 {
                     std::lock_guard<std::mutex> lk(mu);
                     busy[it.slot] = 0;
+                    pending[t] -= it.got;
                 }
                 cv.notify_all();
             }
@@ -634,6 +679,12 @@ int create_resident(p2g_ctx *ctx, const CreateOptions &opts, const std::vector<s
         if (fd < 0) {
             rc = set_err(err, P2G_ERR_IO, "cannot open " + paths[f] + ": " + strerror(errno));
             break;
+        }
+        int owner = 0;  // the hashing thread of this file
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            for (int t = 1; t < H; t++)
+                if (pending[t] < pending[owner]) owner = t;
         }
         for (uint64_t at = 0; at < sources[f].length; at += chunk) {
             int k = -1;
@@ -670,7 +721,8 @@ int create_resident(p2g_ctx *ctx, const CreateOptions &opts, const std::vector<s
             {
                 std::lock_guard<std::mutex> lk(mu);
                 busy[k] = 1;
-                queue[f % H].push_back({k, f, got, at});
+                pending[owner] += got;
+                queue[owner].push_back({k, f, got, at});
             }
             cv.notify_all();
         }
@@ -678,6 +730,7 @@ int create_resident(p2g_ctx *ctx, const CreateOptions &opts, const std::vector<s
     }
     finish_hashers();
     if (rc != P2G_OK) return rc;
+    const double ms_read = ms_since(t_start);
     if (hipStreamSynchronize(copy) != hipSuccess) return set_err(err, P2G_ERR_DEVICE, "chunk copies");
     for (size_t f = 0; f < nf; f++) {
         SourceFile &s = sources[f];
@@ -698,6 +751,7 @@ int create_resident(p2g_ctx *ctx, const CreateOptions &opts, const std::vector<s
     // one device pass: slice checksums, recovery blocks, packet hashes
     std::vector<p2g_dev_file> files(nf);
     for (size_t f = 0; f < nf; f++) files[f] = {dev + off[f], sources[f].length};
+    const auto t_pass = std::chrono::steady_clock::now();
     rc = p2g_dev_create_pass(ctx, files.data(), (uint32_t)nf, S, R, set_id, d_rec, d_md5, d_crc,
                              d_pkt, nullptr);
     if (rc == P2G_OK) rc = p2g_synchronize(ctx);
@@ -711,6 +765,8 @@ int create_resident(p2g_ctx *ctx, const CreateOptions &opts, const std::vector<s
     if (he == hipSuccess && R) he = hipMemcpy(pkt_md5.data(), d_pkt, 16ull * R, hipMemcpyDeviceToHost);
     if (he == hipSuccess && R) he = hipMemcpy(rec.p, d_rec, (size_t)R * S, hipMemcpyDeviceToHost);
     if (he != hipSuccess) return set_err(err, P2G_ERR_DEVICE, "read back the create pass");
+    const double ms_pass = ms_since(t_pass);  // pass and read-back
+    const auto t_write = std::chrono::steady_clock::now();
     for (size_t f = 0, sl = 0; f < nf; f++) {
         SourceFile &s = sources[f];
         s.slice_checksums.resize((s.length + S - 1) / S);
@@ -729,7 +785,11 @@ int create_resident(p2g_ctx *ctx, const CreateOptions &opts, const std::vector<s
         err = scheme_err;
         return scheme_rc;
     }
-    return write_volumes(opts, sizes, critical, set_id, rec.p, pkt_md5.data(), written, err);
+    rc = write_volumes(opts, sizes, critical, set_id, rec.p, pkt_md5.data(), written, err);
+    if (verbose)
+        fprintf(stderr, "create: read+hash %.1f ms, pass+read-back %.1f ms, write %.1f ms, total %.1f ms\n",
+                ms_read, ms_pass, ms_since(t_write), ms_since(t_start));
+    return rc;
 }
 
 }  // namespace par2
