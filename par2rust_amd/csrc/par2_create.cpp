@@ -392,9 +392,10 @@ int write_volumes(const CreateOptions &opts, const std::vector<uint32_t> &sizes,
                   Error &err) {
     const uint64_t S = opts.slice_size;
     // Each volume is its own file (written, fsynced and renamed by
-    // write_atomic): up to 8 are written at once.  The result is that of
-    // writing them in order: the written names up to the first volume that
-    // failed, and that volume's error.
+    // write_atomic): up to 8 are written at once, and every volume is
+    // attempted.  As in src/creator.rs:296-316 (a parallel map over the
+    // volumes, then the results in volume order), the first failed volume's
+    // error is returned and the volumes written beside it stay on disk.
     const size_t nv = sizes.size();
     std::vector<uint32_t> first(nv);
     for (size_t v = 0, e = 0; v < nv; e += sizes[v], v++) first[v] = (uint32_t)e;
@@ -442,9 +443,6 @@ int write_volumes(const CreateOptions &opts, const std::vector<uint32_t> &sizes,
     for (auto &th : pool) th.join();
     for (size_t v = 0; v < nv; v++) {
         if (rcs[v] != P2G_OK) {
-            // as if written in order: no volume after the failed one exists
-            for (size_t w = v + 1; w < nv; w++)
-                if (rcs[w] == P2G_OK) ::unlink(names[w].c_str());
             err = errs[v];
             return rcs[v];
         }

@@ -143,3 +143,22 @@ def test_tiny_against_golden_bytes(tmp_path, ctx):
     files = ctx.run_create(p2.CreateOptions(str(tmp_path / "recovery.par2"), 4, 2,
                                             p2.VolumeScheme.single()), [(str(src), b"a.bin")])
     assert [open(p, "rb").read().hex() for p in files] == [f["hex"] for f in t["files"]]
+
+
+def test_failed_volume_error_and_the_other_volumes(tmp_path, ctx, create_path):
+    # src/creator.rs:296-316 writes the volumes as a parallel map and returns
+    # the first failure in volume order: when one volume cannot be written
+    # (here its name is taken by a directory, so its rename fails) the create
+    # reports an I/O error, and the index and every other volume are on disk
+    src = write(tmp_path / "input.bin", O.xorshift64_bytes(4 << 20, 1))
+    out = tmp_path / "out"
+    out.mkdir()
+    (out / "recovery.vol2+2.par2").mkdir()
+    with pytest.raises(p2.Par2Error) as e:
+        ctx.run_create(p2.CreateOptions(str(out / "recovery.par2"), 4096, 10,
+                                        p2.VolumeScheme.exponential()), [(str(src), b"input.bin")])
+    assert e.value.kind == "Io"
+    names = set(os.listdir(out))
+    assert {"recovery.par2", "recovery.vol0+1.par2", "recovery.vol1+1.par2",
+            "recovery.vol4+4.par2", "recovery.vol8+2.par2"} <= names
+    assert (out / "recovery.vol2+2.par2").is_dir()
