@@ -68,6 +68,30 @@ def measured_traffic(kernel: str):
     return None
 
 
+def measured_counters(kernel: str):
+    """MFMA-pipe busy and VALU-issue fractions of `kernel` from the committed
+    counter summary (SQ_VALU_MFMA_BUSY_CYCLES and SQ_INSTS_VALU against
+    GRBM_GUI_ACTIVE), or None when the summary has none."""
+    import re
+
+    path = os.path.join(ROOT, "profiles", f"{kernel}_headline_traffic.md")
+    out = {}
+    try:
+        for line in open(path):
+            m = re.search(r"MFMA pipe busy ([0-9.]+)%", line)
+            if m:
+                out["mfma_pipe_busy"] = round(float(m.group(1)) / 100, 3)
+            m = re.search(r"VALU issue ([0-9.]+)% of cycles", line)
+            if m:
+                out["valu_issue"] = round(float(m.group(1)) / 100, 3)
+    except OSError:
+        return None
+    if not out:
+        return None
+    out["source"] = f"profiles/{kernel}_headline_traffic.md (rocprofv3 --pmc, one dispatch)"
+    return out
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -587,6 +611,7 @@ def main() -> int:
                 "peak": FP4_DENSE_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(mfma_ops / (k1_ms / 1e3) / 1e12 / FP4_DENSE_PEAK_TFLOPS, 4),
                 "traffic": measured_traffic(traffic_kernel),
+                "counters": measured_counters(traffic_kernel),
                 "alg_ops_per_launch": mfma_ops, "alg_bytes_per_launch": alg_bytes,
                 "hbm_frac": round(alg_bytes / (k1_ms / 1e3) / 1e9 / HBM_PEAK_GBPS, 5)}
     else:
