@@ -653,6 +653,16 @@ def main() -> int:
                        "k3_packet_md5": round(k_ms["k3"], 4)}),
         "roofline": roof,
     }
+    if args.mode == "pass" and "mfma" in roof.get("bound", ""):
+        # the encode role alone: from the pass's first stamp to the last encode
+        # unit's (235 of 256 CUs encode; the hash roles end after it)
+        enc = out["kernel_ms"]["pass_roles_start_end_ms"].get("encode", (0.0, 0.0))
+        if enc[1] > 0:
+            roof["encode_role"] = {
+                "ms": round(enc[1], 4),
+                "achieved": round(mfma_ops / (enc[1] / 1e3) / 1e12, 2),
+                "frac": round(mfma_ops / (enc[1] / 1e3) / 1e12 / FP4_DENSE_PEAK_TFLOPS, 4),
+                "note": "fp4 operations over the encode role's end in the last timed pass"}
 
     if not args.no_end_to_end:
         out["end_to_end_create"] = run_end_to_end(ctx, host, S, R, not args.no_cpu_baseline)
