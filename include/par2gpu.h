@@ -6,6 +6,8 @@
  * internal call sites, cited as file:line of the reference tree.  Plain
  * pointers and sizes only; every call returns a p2g status (0 = success);
  * p2g_last_error() gives the message of the last failure on a context.
+ * No C++ exception crosses this ABI: a host allocation that fails inside a
+ * call returns P2G_ERR_OUT_OF_MEMORY, any other host failure P2G_ERR_IO.
  *
  * Threading: a context is used by one host thread at a time.  All device
  * work of a context runs on that context's HIP streams; the host-buffer

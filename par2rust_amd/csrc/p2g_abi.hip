@@ -298,18 +298,20 @@ extern "C" {
 const char *p2g_version(void) { return "par2gpu 0.1.0 gfx950"; }
 
 int p2g_open(int device, p2g_ctx **out) {
-    if (!out) return P2G_ERR_INVALID_ARGUMENT;
-    *out = nullptr;
-    Ctx *c = new (std::nothrow) Ctx();
-    if (!c) return P2G_ERR_OUT_OF_MEMORY;
-    int rc = c->open(device);
-    if (rc != P2G_OK) {
-        fprintf(stderr, "p2g_open: %s\n", c->err.c_str());
-        delete c;
-        return rc;
-    }
-    *out = reinterpret_cast<p2g_ctx *>(c);
-    return P2G_OK;
+    return guarded(nullptr, [&]() -> int {
+        if (!out) return P2G_ERR_INVALID_ARGUMENT;
+        *out = nullptr;
+        Ctx *c = new (std::nothrow) Ctx();
+        if (!c) return P2G_ERR_OUT_OF_MEMORY;
+        int rc = c->open(device);
+        if (rc != P2G_OK) {
+            fprintf(stderr, "p2g_open: %s\n", c->err.c_str());
+            delete c;
+            return rc;
+        }
+        *out = reinterpret_cast<p2g_ctx *>(c);
+        return P2G_OK;
+    });
 }
 
 void p2g_close(p2g_ctx *ctx) { delete reinterpret_cast<Ctx *>(ctx); }
@@ -328,177 +330,189 @@ static uint64_t xs_apply_host(const uint64_t *m, uint64_t v) {
 
 int p2g_dev_fill_xorshift64(p2g_ctx *ctx, uint8_t *dst, uint64_t nbytes, uint64_t seed,
                             void *stream) {
-    Ctx *c = reinterpret_cast<Ctx *>(ctx);
-    if (!c) return P2G_ERR_INVALID_ARGUMENT;
-    if (nbytes == 0) return P2G_OK;
-    if (!dst || reinterpret_cast<uintptr_t>(dst) % 16 != 0)
-        return c->fail(P2G_ERR_INVALID_ARGUMENT, "destination must be 16-byte aligned");
-    hipSetDevice(c->device);
-    if (!c->d_xs_jumps.ptr) {
-        // step^(W 2^b) for b = 0..31, W = kXorshiftLaneWords
-        std::vector<uint64_t> step(64), m(64), t(64), jumps(64 * 32);
-        for (int b = 0; b < 64; b++) {
-            uint64_t x = 1ull << b;
-            x ^= x << 13;
-            x ^= x >> 7;
-            x ^= x << 17;
-            step[b] = x;
+    return guarded(reinterpret_cast<Ctx *>(ctx), [&]() -> int {
+        Ctx *c = reinterpret_cast<Ctx *>(ctx);
+        if (!c) return P2G_ERR_INVALID_ARGUMENT;
+        if (nbytes == 0) return P2G_OK;
+        if (!dst || reinterpret_cast<uintptr_t>(dst) % 16 != 0)
+            return c->fail(P2G_ERR_INVALID_ARGUMENT, "destination must be 16-byte aligned");
+        hipSetDevice(c->device);
+        if (!c->d_xs_jumps.ptr) {
+            // step^(W 2^b) for b = 0..31, W = kXorshiftLaneWords
+            std::vector<uint64_t> step(64), m(64), t(64), jumps(64 * 32);
+            for (int b = 0; b < 64; b++) {
+                uint64_t x = 1ull << b;
+                x ^= x << 13;
+                x ^= x >> 7;
+                x ^= x << 17;
+                step[b] = x;
+            }
+            m = step;  // step^(2^k), squared until 2^k = W
+            for (uint32_t w = 1; w < kXorshiftLaneWords; w *= 2) {
+                for (int b = 0; b < 64; b++) t[b] = xs_apply_host(m.data(), m[b]);
+                m = t;
+            }
+            for (int k = 0; k < 32; k++) {
+                std::copy(m.begin(), m.end(), jumps.begin() + 64 * k);
+                for (int b = 0; b < 64; b++) t[b] = xs_apply_host(m.data(), m[b]);
+                m = t;
+            }
+            if (c->d_xs_jumps.reserve(jumps.size() * 8) != P2G_OK)
+                return c->fail(P2G_ERR_OUT_OF_MEMORY, "xorshift jump table");
+            hipError_t e = hipMemcpy(c->d_xs_jumps.ptr, jumps.data(), jumps.size() * 8,
+                                     hipMemcpyHostToDevice);
+            if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "upload xorshift jumps", e);
         }
-        m = step;  // step^(2^k), squared until 2^k = W
-        for (uint32_t w = 1; w < kXorshiftLaneWords; w *= 2) {
-            for (int b = 0; b < 64; b++) t[b] = xs_apply_host(m.data(), m[b]);
-            m = t;
-        }
-        for (int k = 0; k < 32; k++) {
-            std::copy(m.begin(), m.end(), jumps.begin() + 64 * k);
-            for (int b = 0; b < 64; b++) t[b] = xs_apply_host(m.data(), m[b]);
-            m = t;
-        }
-        if (c->d_xs_jumps.reserve(jumps.size() * 8) != P2G_OK)
-            return c->fail(P2G_ERR_OUT_OF_MEMORY, "xorshift jump table");
-        hipError_t e = hipMemcpy(c->d_xs_jumps.ptr, jumps.data(), jumps.size() * 8,
-                                 hipMemcpyHostToDevice);
-        if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "upload xorshift jumps", e);
-    }
-    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c->stream;
-    hipError_t e = launch_xorshift_fill(dst, nbytes, seed,
-                                        static_cast<const uint64_t *>(c->d_xs_jumps.ptr), st);
-    if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "xorshift fill kernel", e);
-    return P2G_OK;
+        hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c->stream;
+        hipError_t e = launch_xorshift_fill(dst, nbytes, seed,
+                                            static_cast<const uint64_t *>(c->d_xs_jumps.ptr), st);
+        if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "xorshift fill kernel", e);
+        return P2G_OK;
+    });
 }
 
 int p2g_synchronize(p2g_ctx *ctx) {
-    Ctx *c = reinterpret_cast<Ctx *>(ctx);
-    if (!c) return P2G_ERR_INVALID_ARGUMENT;
-    hipSetDevice(c->device);
-    hipError_t e = hipStreamSynchronize(c->stream);
-    if (e == hipSuccess && c->side) e = hipStreamSynchronize(c->side);
-    for (hipEvent_t ev : c->scratch_done)  // work queued on callers' streams
-        if (e == hipSuccess) e = hipEventSynchronize(ev);
-    if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "synchronize", e);
-    int brc = P2G_OK;
-    for (BatchPlan *b : c->pending_batches) {
-        const int rc = wait_plan(b);
-        if (brc == P2G_OK) brc = rc;
-        free_batch_plan(b);
-    }
-    c->pending_batches.clear();
-    if (brc != P2G_OK) return brc;
-    if (c->pass_pending) {
-        c->pass_pending = false;
-        uint32_t flags = 0;
-        e = hipMemcpy(&flags, static_cast<uint32_t *>(c->d_sync.ptr) + kPassFlags, 4,
-                      hipMemcpyDeviceToHost);
-        if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "read create-pass flags", e);
-        if (flags & kPassErrWait)
-            return c->fail(P2G_ERR_DEVICE,
-                           "create pass: a packet hash stopped waiting for the encode (no progress for 2 s)");
-    }
-    return P2G_OK;
+    return guarded(reinterpret_cast<Ctx *>(ctx), [&]() -> int {
+        Ctx *c = reinterpret_cast<Ctx *>(ctx);
+        if (!c) return P2G_ERR_INVALID_ARGUMENT;
+        hipSetDevice(c->device);
+        hipError_t e = hipStreamSynchronize(c->stream);
+        if (e == hipSuccess && c->side) e = hipStreamSynchronize(c->side);
+        for (hipEvent_t ev : c->scratch_done)  // work queued on callers' streams
+            if (e == hipSuccess) e = hipEventSynchronize(ev);
+        if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "synchronize", e);
+        int brc = P2G_OK;
+        for (BatchPlan *b : c->pending_batches) {
+            const int rc = wait_plan(b);
+            if (brc == P2G_OK) brc = rc;
+            free_batch_plan(b);
+        }
+        c->pending_batches.clear();
+        if (brc != P2G_OK) return brc;
+        if (c->pass_pending) {
+            c->pass_pending = false;
+            uint32_t flags = 0;
+            e = hipMemcpy(&flags, static_cast<uint32_t *>(c->d_sync.ptr) + kPassFlags, 4,
+                          hipMemcpyDeviceToHost);
+            if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "read create-pass flags", e);
+            if (flags & kPassErrWait)
+                return c->fail(P2G_ERR_DEVICE,
+                               "create pass: a packet hash stopped waiting for the encode (no progress for 2 s)");
+        }
+        return P2G_OK;
+    });
 }
 
 int p2g_dev_encode(p2g_ctx *ctx, const p2g_dev_file *files, uint32_t nfiles, uint64_t slice_size,
                    uint32_t first_input_index, uint32_t total_inputs, uint32_t recovery_count,
                    uint8_t *rec, int accumulate, void *stream) {
-    Ctx *c = reinterpret_cast<Ctx *>(ctx);
-    if (!c) return P2G_ERR_INVALID_ARGUMENT;
-    if ((nfiles && !files) || (recovery_count && !rec))
-        return c->fail(P2G_ERR_INVALID_ARGUMENT, "null argument");
-    if (reinterpret_cast<uintptr_t>(rec) % 4 != 0)
-        return c->fail(P2G_ERR_INVALID_ARGUMENT, "recovery buffer must be 4-byte aligned");
-    int rc = check_slice_size(c, slice_size);
-    if (rc != P2G_OK) return rc;
-    hipSetDevice(c->device);
-    std::vector<SliceDesc> descs;
-    rc = append_descs(c, files, nfiles, slice_size, descs);
-    if (rc != P2G_OK) return rc;
-    if (descs.size() > kMaxInputs)
-        return c->fail(P2G_ERR_TOO_MANY_INPUT_BLOCKS, "more than 32768 input slices");
-    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c->stream;
-    rc = c->scratch_acquire(Ctx::kScratchEnc, st);
-    if (rc == P2G_OK) rc = upload_descs(c, c->enc_descs, descs, st);
-    if (rc == P2G_OK)
-        rc = dev_encode(c, static_cast<const SliceDesc *>(c->enc_descs.d.ptr), (uint32_t)descs.size(),
-                        slice_size, first_input_index, total_inputs, recovery_count, rec,
-                        accumulate, st);
-    if (rc != P2G_OK) return rc;
-    return c->scratch_release(Ctx::kScratchEnc, st);
+    return guarded(reinterpret_cast<Ctx *>(ctx), [&]() -> int {
+        Ctx *c = reinterpret_cast<Ctx *>(ctx);
+        if (!c) return P2G_ERR_INVALID_ARGUMENT;
+        if ((nfiles && !files) || (recovery_count && !rec))
+            return c->fail(P2G_ERR_INVALID_ARGUMENT, "null argument");
+        if (reinterpret_cast<uintptr_t>(rec) % 4 != 0)
+            return c->fail(P2G_ERR_INVALID_ARGUMENT, "recovery buffer must be 4-byte aligned");
+        int rc = check_slice_size(c, slice_size);
+        if (rc != P2G_OK) return rc;
+        hipSetDevice(c->device);
+        std::vector<SliceDesc> descs;
+        rc = append_descs(c, files, nfiles, slice_size, descs);
+        if (rc != P2G_OK) return rc;
+        if (descs.size() > kMaxInputs)
+            return c->fail(P2G_ERR_TOO_MANY_INPUT_BLOCKS, "more than 32768 input slices");
+        hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c->stream;
+        rc = c->scratch_acquire(Ctx::kScratchEnc, st);
+        if (rc == P2G_OK) rc = upload_descs(c, c->enc_descs, descs, st);
+        if (rc == P2G_OK)
+            rc = dev_encode(c, static_cast<const SliceDesc *>(c->enc_descs.d.ptr), (uint32_t)descs.size(),
+                            slice_size, first_input_index, total_inputs, recovery_count, rec,
+                            accumulate, st);
+        if (rc != P2G_OK) return rc;
+        return c->scratch_release(Ctx::kScratchEnc, st);
+    });
 }
 
 int p2g_dev_slice_checksums(p2g_ctx *ctx, const p2g_dev_file *files, uint32_t nfiles,
                             uint64_t slice_size, uint8_t *md5_out, uint32_t *crc_out,
                             void *stream) {
-    Ctx *c = reinterpret_cast<Ctx *>(ctx);
-    if (!c) return P2G_ERR_INVALID_ARGUMENT;
-    if ((nfiles && !files) || !md5_out || !crc_out)
-        return c->fail(P2G_ERR_INVALID_ARGUMENT, "null argument");
-    if (reinterpret_cast<uintptr_t>(md5_out) % 4 != 0 || reinterpret_cast<uintptr_t>(crc_out) % 4 != 0)
-        return c->fail(P2G_ERR_INVALID_ARGUMENT, "checksum outputs must be 4-byte aligned");
-    int rc = check_slice_size(c, slice_size);
-    if (rc != P2G_OK) return rc;
-    hipSetDevice(c->device);
-    std::vector<SliceDesc> descs;
-    rc = append_descs(c, files, nfiles, slice_size, descs);
-    if (rc != P2G_OK) return rc;
-    if (descs.empty()) return P2G_OK;
-    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c->stream;
-    rc = c->scratch_acquire(Ctx::kScratchSum, st);
-    if (rc == P2G_OK) rc = upload_descs(c, c->sum_descs, descs, st);
-    if (rc != P2G_OK) return rc;
-    SliceDesc *dd = static_cast<SliceDesc *>(c->sum_descs.d.ptr);
-    const uint32_t n = (uint32_t)descs.size();
-    if (c->sum_descs.tb.reserve((size_t)n * 64) != P2G_OK)
-        return c->fail(P2G_ERR_OUT_OF_MEMORY, "slice tail blocks");
-    uint4 *tb = static_cast<uint4 *>(c->sum_descs.tb.ptr);
-    hipError_t e = launch_desc_tails(dd, n, tb, st);
-    if (e == hipSuccess) e = launch_slice_md5(dd, tb, n, (uint32_t)slice_size, md5_out, st);
-    if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "slice md5 kernel", e);
-    e = launch_slice_crc32(dd, n, (uint32_t)slice_size, crc_out, st);
-    if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "slice crc kernel", e);
-    return c->scratch_release(Ctx::kScratchSum, st);
+    return guarded(reinterpret_cast<Ctx *>(ctx), [&]() -> int {
+        Ctx *c = reinterpret_cast<Ctx *>(ctx);
+        if (!c) return P2G_ERR_INVALID_ARGUMENT;
+        if ((nfiles && !files) || !md5_out || !crc_out)
+            return c->fail(P2G_ERR_INVALID_ARGUMENT, "null argument");
+        if (reinterpret_cast<uintptr_t>(md5_out) % 4 != 0 || reinterpret_cast<uintptr_t>(crc_out) % 4 != 0)
+            return c->fail(P2G_ERR_INVALID_ARGUMENT, "checksum outputs must be 4-byte aligned");
+        int rc = check_slice_size(c, slice_size);
+        if (rc != P2G_OK) return rc;
+        hipSetDevice(c->device);
+        std::vector<SliceDesc> descs;
+        rc = append_descs(c, files, nfiles, slice_size, descs);
+        if (rc != P2G_OK) return rc;
+        if (descs.empty()) return P2G_OK;
+        hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c->stream;
+        rc = c->scratch_acquire(Ctx::kScratchSum, st);
+        if (rc == P2G_OK) rc = upload_descs(c, c->sum_descs, descs, st);
+        if (rc != P2G_OK) return rc;
+        SliceDesc *dd = static_cast<SliceDesc *>(c->sum_descs.d.ptr);
+        const uint32_t n = (uint32_t)descs.size();
+        if (c->sum_descs.tb.reserve((size_t)n * 64) != P2G_OK)
+            return c->fail(P2G_ERR_OUT_OF_MEMORY, "slice tail blocks");
+        uint4 *tb = static_cast<uint4 *>(c->sum_descs.tb.ptr);
+        hipError_t e = launch_desc_tails(dd, n, tb, st);
+        if (e == hipSuccess) e = launch_slice_md5(dd, tb, n, (uint32_t)slice_size, md5_out, st);
+        if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "slice md5 kernel", e);
+        e = launch_slice_crc32(dd, n, (uint32_t)slice_size, crc_out, st);
+        if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "slice crc kernel", e);
+        return c->scratch_release(Ctx::kScratchSum, st);
+    });
 }
 
 int p2g_dev_recovery_packet_md5(p2g_ctx *ctx, const uint8_t *rec, uint32_t count,
                                 uint64_t slice_size, uint32_t first_exponent,
                                 const uint8_t set_id[16], uint8_t *pkt_md5_out, void *stream) {
-    Ctx *c = reinterpret_cast<Ctx *>(ctx);
-    if (!c) return P2G_ERR_INVALID_ARGUMENT;
-    if (count && (!rec || !set_id || !pkt_md5_out))
-        return c->fail(P2G_ERR_INVALID_ARGUMENT, "null argument");
-    if (reinterpret_cast<uintptr_t>(rec) % 4 != 0 || reinterpret_cast<uintptr_t>(pkt_md5_out) % 4 != 0)
-        return c->fail(P2G_ERR_INVALID_ARGUMENT, "recovery blocks and digests must be 4-byte aligned");
-    int rc = check_slice_size(c, slice_size);
-    if (rc != P2G_OK) return rc;
-    if ((uint64_t)first_exponent + count > 65536)
-        return c->fail(P2G_ERR_TOO_MANY_RECOVERY_BLOCKS, "recovery exponent above 65535");
-    hipSetDevice(c->device);
-    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c->stream;
-    hipError_t e = launch_recovery_md5(rec, count, (uint32_t)slice_size, first_exponent, set_id,
-                                       pkt_md5_out, st);
-    if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "recovery md5 kernel", e);
-    return P2G_OK;
+    return guarded(reinterpret_cast<Ctx *>(ctx), [&]() -> int {
+        Ctx *c = reinterpret_cast<Ctx *>(ctx);
+        if (!c) return P2G_ERR_INVALID_ARGUMENT;
+        if (count && (!rec || !set_id || !pkt_md5_out))
+            return c->fail(P2G_ERR_INVALID_ARGUMENT, "null argument");
+        if (reinterpret_cast<uintptr_t>(rec) % 4 != 0 || reinterpret_cast<uintptr_t>(pkt_md5_out) % 4 != 0)
+            return c->fail(P2G_ERR_INVALID_ARGUMENT, "recovery blocks and digests must be 4-byte aligned");
+        int rc = check_slice_size(c, slice_size);
+        if (rc != P2G_OK) return rc;
+        if ((uint64_t)first_exponent + count > 65536)
+            return c->fail(P2G_ERR_TOO_MANY_RECOVERY_BLOCKS, "recovery exponent above 65535");
+        hipSetDevice(c->device);
+        hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c->stream;
+        hipError_t e = launch_recovery_md5(rec, count, (uint32_t)slice_size, first_exponent, set_id,
+                                           pkt_md5_out, st);
+        if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "recovery md5 kernel", e);
+        return P2G_OK;
+    });
 }
 
 int p2g_pass_timing(p2g_ctx *ctx, double *ms_out) {
-    Ctx *c = reinterpret_cast<Ctx *>(ctx);
-    if (!c || !ms_out) return P2G_ERR_INVALID_ARGUMENT;
-    if (!c->last_pass_fused || !c->d_sync.ptr) {  // separate kernels, batched, or none yet
-        for (uint32_t k = 0; k < 2 * kPassRoles; k++) ms_out[k] = 0.0;
+    return guarded(reinterpret_cast<Ctx *>(ctx), [&]() -> int {
+        Ctx *c = reinterpret_cast<Ctx *>(ctx);
+        if (!c || !ms_out) return P2G_ERR_INVALID_ARGUMENT;
+        if (!c->last_pass_fused || !c->d_sync.ptr) {  // separate kernels, batched, or none yet
+            for (uint32_t k = 0; k < 2 * kPassRoles; k++) ms_out[k] = 0.0;
+            return P2G_OK;
+        }
+        hipSetDevice(c->device);
+        hipError_t e = hipEventSynchronize(c->scratch_done[Ctx::kScratchEnc]);
+        unsigned long long t[2 * kPassRoles];
+        if (e == hipSuccess)
+            e = hipMemcpy(t, static_cast<uint32_t *>(c->d_sync.ptr) + kPassTimes, sizeof t,
+                          hipMemcpyDeviceToHost);
+        if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "read create-pass stamps", e);
+        unsigned long long t0 = ~0ull;
+        for (uint32_t r = 0; r < kPassRoles; r++)
+            if (t[2 * r + 1]) t0 = std::min(t0, t[2 * r]);
+        for (uint32_t k = 0; k < 2 * kPassRoles; k++)  // 100 MHz counter: 1e5 ticks per ms
+            ms_out[k] = t[k | 1] && t[k] >= t0 ? (double)(t[k] - t0) / 1e5 : 0.0;
         return P2G_OK;
-    }
-    hipSetDevice(c->device);
-    hipError_t e = hipEventSynchronize(c->scratch_done[Ctx::kScratchEnc]);
-    unsigned long long t[2 * kPassRoles];
-    if (e == hipSuccess)
-        e = hipMemcpy(t, static_cast<uint32_t *>(c->d_sync.ptr) + kPassTimes, sizeof t,
-                      hipMemcpyDeviceToHost);
-    if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "read create-pass stamps", e);
-    unsigned long long t0 = ~0ull;
-    for (uint32_t r = 0; r < kPassRoles; r++)
-        if (t[2 * r + 1]) t0 = std::min(t0, t[2 * r]);
-    for (uint32_t k = 0; k < 2 * kPassRoles; k++)  // 100 MHz counter: 1e5 ticks per ms
-        ms_out[k] = t[k | 1] && t[k] >= t0 ? (double)(t[k] - t0) / 1e5 : 0.0;
-    return P2G_OK;
+    });
 }
 
 }  // extern "C"
@@ -625,201 +639,213 @@ int p2g_dev_create_pass(p2g_ctx *ctx, const p2g_dev_file *files, uint32_t nfiles
                         uint64_t slice_size, uint32_t recovery_count, const uint8_t set_id[16],
                         uint8_t *rec, uint8_t *slice_md5_out, uint32_t *slice_crc_out,
                         uint8_t *pkt_md5_out, void *stream) {
-    Ctx *c = reinterpret_cast<Ctx *>(ctx);
-    if (!c) return P2G_ERR_INVALID_ARGUMENT;
-    if ((nfiles && !files) || slice_size == 0)
-        return create_pass_impl(c, files, nfiles, slice_size, recovery_count, set_id, rec,
-                                slice_md5_out, slice_crc_out, pkt_md5_out, c->stream,
-                                c->encode_variant, c->band_cols);  // reports the argument error
-    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c->stream;
-    uint64_t n = 0;
-    for (uint32_t f = 0; f < nfiles; f++) n += (files[f].len + slice_size - 1) / slice_size;
-    auto run = [&](int variant, uint32_t band_cols) {
-        return create_pass_impl(c, files, nfiles, slice_size, recovery_count, set_id, rec,
-                                slice_md5_out, slice_crc_out, pkt_md5_out, st, variant, band_cols);
-    };
-    TuneChoice ch;
-    if (tune_lookup(c, n, recovery_count, slice_size, &ch)) return run(ch.variant, ch.band_cols);
-    if (c->autotune && n && recovery_count)
-        return tune_and_run(c, n, recovery_count, slice_size, st, run, nullptr);
-    return run(c->encode_variant, c->band_cols);
+    return guarded(reinterpret_cast<Ctx *>(ctx), [&]() -> int {
+        Ctx *c = reinterpret_cast<Ctx *>(ctx);
+        if (!c) return P2G_ERR_INVALID_ARGUMENT;
+        if ((nfiles && !files) || slice_size == 0)
+            return create_pass_impl(c, files, nfiles, slice_size, recovery_count, set_id, rec,
+                                    slice_md5_out, slice_crc_out, pkt_md5_out, c->stream,
+                                    c->encode_variant, c->band_cols);  // reports the argument error
+        hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c->stream;
+        uint64_t n = 0;
+        for (uint32_t f = 0; f < nfiles; f++) n += (files[f].len + slice_size - 1) / slice_size;
+        auto run = [&](int variant, uint32_t band_cols) {
+            return create_pass_impl(c, files, nfiles, slice_size, recovery_count, set_id, rec,
+                                    slice_md5_out, slice_crc_out, pkt_md5_out, st, variant, band_cols);
+        };
+        TuneChoice ch;
+        if (tune_lookup(c, n, recovery_count, slice_size, &ch)) return run(ch.variant, ch.band_cols);
+        if (c->autotune && n && recovery_count)
+            return tune_and_run(c, n, recovery_count, slice_size, st, run, nullptr);
+        return run(c->encode_variant, c->band_cols);
+    });
 }
 
 // ---- host-buffer entry points ----------------------------------------
 
 int p2g_slice_checksums(p2g_ctx *ctx, const uint8_t *data, uint64_t len, uint64_t slice_size,
                         uint8_t *md5_out, uint32_t *crc_out) {
-    Ctx *c = reinterpret_cast<Ctx *>(ctx);
-    if (!c) return P2G_ERR_INVALID_ARGUMENT;
-    if (len && (!data || !md5_out || !crc_out))
-        return c->fail(P2G_ERR_INVALID_ARGUMENT, "null argument");
-    int rc = check_slice_size(c, slice_size);
-    if (rc != P2G_OK) return rc;
-    if (len == 0) return P2G_OK;
-    hipSetDevice(c->device);
-    const uint64_t n = (len + slice_size - 1) / slice_size;
-    if (c->d_stage.reserve(len) != P2G_OK || c->d_md5.reserve(n * 16) != P2G_OK ||
-        c->d_crc.reserve(n * 4) != P2G_OK)
-        return c->fail(P2G_ERR_OUT_OF_MEMORY, "checksum buffers");
-    hipError_t e = hipMemcpyAsync(c->d_stage.ptr, data, len, hipMemcpyHostToDevice, c->stream);
-    if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "upload file bytes", e);
-    p2g_dev_file f{static_cast<const uint8_t *>(c->d_stage.ptr), len};
-    rc = p2g_dev_slice_checksums(ctx, &f, 1, slice_size, static_cast<uint8_t *>(c->d_md5.ptr),
-                                 static_cast<uint32_t *>(c->d_crc.ptr), c->stream);
-    if (rc != P2G_OK) return rc;
-    e = hipMemcpyAsync(md5_out, c->d_md5.ptr, n * 16, hipMemcpyDeviceToHost, c->stream);
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(crc_out, c->d_crc.ptr, n * 4, hipMemcpyDeviceToHost, c->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "slice checksums", e);
-    return P2G_OK;
+    return guarded(reinterpret_cast<Ctx *>(ctx), [&]() -> int {
+        Ctx *c = reinterpret_cast<Ctx *>(ctx);
+        if (!c) return P2G_ERR_INVALID_ARGUMENT;
+        if (len && (!data || !md5_out || !crc_out))
+            return c->fail(P2G_ERR_INVALID_ARGUMENT, "null argument");
+        int rc = check_slice_size(c, slice_size);
+        if (rc != P2G_OK) return rc;
+        if (len == 0) return P2G_OK;
+        hipSetDevice(c->device);
+        const uint64_t n = (len + slice_size - 1) / slice_size;
+        if (c->d_stage.reserve(len) != P2G_OK || c->d_md5.reserve(n * 16) != P2G_OK ||
+            c->d_crc.reserve(n * 4) != P2G_OK)
+            return c->fail(P2G_ERR_OUT_OF_MEMORY, "checksum buffers");
+        hipError_t e = hipMemcpyAsync(c->d_stage.ptr, data, len, hipMemcpyHostToDevice, c->stream);
+        if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "upload file bytes", e);
+        p2g_dev_file f{static_cast<const uint8_t *>(c->d_stage.ptr), len};
+        rc = p2g_dev_slice_checksums(ctx, &f, 1, slice_size, static_cast<uint8_t *>(c->d_md5.ptr),
+                                     static_cast<uint32_t *>(c->d_crc.ptr), c->stream);
+        if (rc != P2G_OK) return rc;
+        e = hipMemcpyAsync(md5_out, c->d_md5.ptr, n * 16, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(crc_out, c->d_crc.ptr, n * 4, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "slice checksums", e);
+        return P2G_OK;
+    });
 }
 
 int p2g_recovery_packet_md5(p2g_ctx *ctx, const uint8_t *rec, uint32_t count, uint64_t slice_size,
                             uint32_t first_exponent, const uint8_t set_id[16],
                             uint8_t *pkt_md5_out) {
-    Ctx *c = reinterpret_cast<Ctx *>(ctx);
-    if (!c) return P2G_ERR_INVALID_ARGUMENT;
-    if (count && (!rec || !set_id || !pkt_md5_out))
-        return c->fail(P2G_ERR_INVALID_ARGUMENT, "null argument");
-    int rc = check_slice_size(c, slice_size);
-    if (rc != P2G_OK) return rc;
-    if (count == 0) return P2G_OK;
-    hipSetDevice(c->device);
-    const size_t bytes = (size_t)count * slice_size;
-    if (c->d_stage.reserve(bytes) != P2G_OK || c->d_md5.reserve((size_t)count * 16) != P2G_OK)
-        return c->fail(P2G_ERR_OUT_OF_MEMORY, "packet md5 buffers");
-    hipError_t e = hipMemcpyAsync(c->d_stage.ptr, rec, bytes, hipMemcpyHostToDevice, c->stream);
-    if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "upload recovery blocks", e);
-    rc = p2g_dev_recovery_packet_md5(ctx, static_cast<const uint8_t *>(c->d_stage.ptr), count,
-                                     slice_size, first_exponent, set_id,
-                                     static_cast<uint8_t *>(c->d_md5.ptr), c->stream);
-    if (rc != P2G_OK) return rc;
-    e = hipMemcpyAsync(pkt_md5_out, c->d_md5.ptr, (size_t)count * 16, hipMemcpyDeviceToHost,
-                       c->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "recovery packet md5", e);
-    return P2G_OK;
+    return guarded(reinterpret_cast<Ctx *>(ctx), [&]() -> int {
+        Ctx *c = reinterpret_cast<Ctx *>(ctx);
+        if (!c) return P2G_ERR_INVALID_ARGUMENT;
+        if (count && (!rec || !set_id || !pkt_md5_out))
+            return c->fail(P2G_ERR_INVALID_ARGUMENT, "null argument");
+        int rc = check_slice_size(c, slice_size);
+        if (rc != P2G_OK) return rc;
+        if (count == 0) return P2G_OK;
+        hipSetDevice(c->device);
+        const size_t bytes = (size_t)count * slice_size;
+        if (c->d_stage.reserve(bytes) != P2G_OK || c->d_md5.reserve((size_t)count * 16) != P2G_OK)
+            return c->fail(P2G_ERR_OUT_OF_MEMORY, "packet md5 buffers");
+        hipError_t e = hipMemcpyAsync(c->d_stage.ptr, rec, bytes, hipMemcpyHostToDevice, c->stream);
+        if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "upload recovery blocks", e);
+        rc = p2g_dev_recovery_packet_md5(ctx, static_cast<const uint8_t *>(c->d_stage.ptr), count,
+                                         slice_size, first_exponent, set_id,
+                                         static_cast<uint8_t *>(c->d_md5.ptr), c->stream);
+        if (rc != P2G_OK) return rc;
+        e = hipMemcpyAsync(pkt_md5_out, c->d_md5.ptr, (size_t)count * 16, hipMemcpyDeviceToHost,
+                           c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "recovery packet md5", e);
+        return P2G_OK;
+    });
 }
 
 // ---- streaming encode job ----------------------------------------------
 
 int p2g_encode_begin(p2g_ctx *ctx, uint64_t slice_size, uint32_t total_inputs,
                      uint32_t recovery_count, p2g_job **out) {
-    Ctx *c = reinterpret_cast<Ctx *>(ctx);
-    if (!c || !out) return P2G_ERR_INVALID_ARGUMENT;
-    *out = nullptr;
-    int rc = check_slice_size(c, slice_size);
-    if (rc != P2G_OK) return rc;
-    if (recovery_count > P2G_MAX_RECOVERY_BLOCKS)
-        return c->fail(P2G_ERR_TOO_MANY_RECOVERY_BLOCKS, "recovery count above 65535");
-    if (total_inputs > kMaxInputs)
-        return c->fail(P2G_ERR_TOO_MANY_INPUT_BLOCKS, "more than 32768 input slices");
-    hipSetDevice(c->device);
-    Job *j = new (std::nothrow) Job();
-    if (!j) return P2G_ERR_OUT_OF_MEMORY;
-    j->ctx = c;
-    j->S = slice_size;
-    j->N = total_inputs;
-    j->R = recovery_count;
-    const size_t rec_bytes = (size_t)recovery_count * slice_size;
-    const size_t desc_bytes = (size_t)std::max<uint32_t>(total_inputs, 1) * sizeof(SliceDesc);
-    if ((rec_bytes && j->d_rec.reserve(rec_bytes) != P2G_OK) ||
-        j->d_descs.reserve(desc_bytes) != P2G_OK || j->h_descs.reserve(desc_bytes) != P2G_OK) {
-        p2g_encode_free(reinterpret_cast<p2g_job *>(j));
-        return c->fail(P2G_ERR_OUT_OF_MEMORY, "encode job buffers");
-    }
-    if (rec_bytes) {
-        hipError_t e = hipMemsetAsync(j->d_rec.ptr, 0, rec_bytes, c->stream);
-        if (e != hipSuccess) {
+    return guarded(reinterpret_cast<Ctx *>(ctx), [&]() -> int {
+        Ctx *c = reinterpret_cast<Ctx *>(ctx);
+        if (!c || !out) return P2G_ERR_INVALID_ARGUMENT;
+        *out = nullptr;
+        int rc = check_slice_size(c, slice_size);
+        if (rc != P2G_OK) return rc;
+        if (recovery_count > P2G_MAX_RECOVERY_BLOCKS)
+            return c->fail(P2G_ERR_TOO_MANY_RECOVERY_BLOCKS, "recovery count above 65535");
+        if (total_inputs > kMaxInputs)
+            return c->fail(P2G_ERR_TOO_MANY_INPUT_BLOCKS, "more than 32768 input slices");
+        hipSetDevice(c->device);
+        Job *j = new (std::nothrow) Job();
+        if (!j) return P2G_ERR_OUT_OF_MEMORY;
+        j->ctx = c;
+        j->S = slice_size;
+        j->N = total_inputs;
+        j->R = recovery_count;
+        const size_t rec_bytes = (size_t)recovery_count * slice_size;
+        const size_t desc_bytes = (size_t)std::max<uint32_t>(total_inputs, 1) * sizeof(SliceDesc);
+        if ((rec_bytes && j->d_rec.reserve(rec_bytes) != P2G_OK) ||
+            j->d_descs.reserve(desc_bytes) != P2G_OK || j->h_descs.reserve(desc_bytes) != P2G_OK) {
             p2g_encode_free(reinterpret_cast<p2g_job *>(j));
-            return c->fail(P2G_ERR_DEVICE, "clear accumulator", e);
+            return c->fail(P2G_ERR_OUT_OF_MEMORY, "encode job buffers");
         }
-    }
-    *out = reinterpret_cast<p2g_job *>(j);
-    return P2G_OK;
+        if (rec_bytes) {
+            hipError_t e = hipMemsetAsync(j->d_rec.ptr, 0, rec_bytes, c->stream);
+            if (e != hipSuccess) {
+                p2g_encode_free(reinterpret_cast<p2g_job *>(j));
+                return c->fail(P2G_ERR_DEVICE, "clear accumulator", e);
+            }
+        }
+        *out = reinterpret_cast<p2g_job *>(j);
+        return P2G_OK;
+    });
 }
 
 int p2g_encode_push(p2g_job *job, uint32_t first_input_index, const uint8_t *data, uint64_t len) {
-    Job *j = reinterpret_cast<Job *>(job);
-    if (!j) return P2G_ERR_INVALID_ARGUMENT;
-    Ctx *c = j->ctx;
-    if (len && !data) return c->fail(P2G_ERR_INVALID_ARGUMENT, "null argument");
-    if (len == 0 || j->R == 0) return P2G_OK;
-    hipSetDevice(c->device);
-    const uint64_t n = (len + j->S - 1) / j->S;
-    if ((uint64_t)first_input_index + n > j->N)
-        return c->fail(P2G_ERR_INVALID_ARGUMENT, "pushed slices exceed total_inputs");
-    // Double-buffered staging: this push's copy runs on the side stream while
-    // the previous push's encode runs on the main stream.  A staging buffer
-    // is reused only after the encode that read it has finished, and the
-    // call returns once `data` has been copied, so the caller may reuse it.
-    Job::Stage &sg = j->stage[j->next_stage];
-    j->next_stage ^= 1;
-    hipError_t e;
-    if (sg.done) {
-        e = hipEventSynchronize(sg.done);
-        if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "staging wait", e);
-    } else {
-        e = hipEventCreateWithFlags(&sg.done, hipEventDisableTiming);
-        if (e == hipSuccess) e = hipEventCreateWithFlags(&sg.copied, hipEventDisableTiming);
-        if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "event create", e);
-    }
-    if (sg.buf.reserve(len) != P2G_OK) return c->fail(P2G_ERR_OUT_OF_MEMORY, "staging buffer");
-    hipStream_t side;
-    int rc = c->side_stream(&side);
-    if (rc != P2G_OK) return rc;
-    e = hipMemcpyAsync(sg.buf.ptr, data, len, hipMemcpyHostToDevice, side);
-    if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "upload slices", e);
-    // Descriptors of this push go to their own range of the job's tables,
-    // which no other push touches.
-    SliceDesc *hd = static_cast<SliceDesc *>(j->h_descs.ptr) + first_input_index;
-    const uint8_t *base = static_cast<const uint8_t *>(sg.buf.ptr);
-    for (uint64_t s = 0; s < n; s++) {
-        const uint64_t have = len - s * j->S;
-        hd[s] = {base + s * j->S, (uint32_t)(have < j->S ? have : j->S), 0};
-    }
-    SliceDesc *dd = static_cast<SliceDesc *>(j->d_descs.ptr) + first_input_index;
-    e = hipMemcpyAsync(dd, hd, n * sizeof(SliceDesc), hipMemcpyHostToDevice, side);
-    if (e == hipSuccess) e = hipEventRecord(sg.copied, side);
-    if (e == hipSuccess) e = hipStreamWaitEvent(c->stream, sg.copied, 0);
-    if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "upload slice descriptors", e);
-    rc = c->scratch_acquire(Ctx::kScratchEnc, c->stream);
-    if (rc == P2G_OK)
-        rc = dev_encode(c, dd, (uint32_t)n, j->S, first_input_index, j->N, j->R,
-                        static_cast<uint8_t *>(j->d_rec.ptr), 1, c->stream);
-    if (rc == P2G_OK) rc = c->scratch_release(Ctx::kScratchEnc, c->stream);
-    if (rc != P2G_OK) return rc;
-    e = hipEventRecord(sg.done, c->stream);
-    if (e == hipSuccess) e = hipEventSynchronize(sg.copied);
-    if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "event record", e);
-    return P2G_OK;
+    return guarded(job ? reinterpret_cast<Job *>(job)->ctx : nullptr, [&]() -> int {
+        Job *j = reinterpret_cast<Job *>(job);
+        if (!j) return P2G_ERR_INVALID_ARGUMENT;
+        Ctx *c = j->ctx;
+        if (len && !data) return c->fail(P2G_ERR_INVALID_ARGUMENT, "null argument");
+        if (len == 0 || j->R == 0) return P2G_OK;
+        hipSetDevice(c->device);
+        const uint64_t n = (len + j->S - 1) / j->S;
+        if ((uint64_t)first_input_index + n > j->N)
+            return c->fail(P2G_ERR_INVALID_ARGUMENT, "pushed slices exceed total_inputs");
+        // Double-buffered staging: this push's copy runs on the side stream while
+        // the previous push's encode runs on the main stream.  A staging buffer
+        // is reused only after the encode that read it has finished, and the
+        // call returns once `data` has been copied, so the caller may reuse it.
+        Job::Stage &sg = j->stage[j->next_stage];
+        j->next_stage ^= 1;
+        hipError_t e;
+        if (sg.done) {
+            e = hipEventSynchronize(sg.done);
+            if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "staging wait", e);
+        } else {
+            e = hipEventCreateWithFlags(&sg.done, hipEventDisableTiming);
+            if (e == hipSuccess) e = hipEventCreateWithFlags(&sg.copied, hipEventDisableTiming);
+            if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "event create", e);
+        }
+        if (sg.buf.reserve(len) != P2G_OK) return c->fail(P2G_ERR_OUT_OF_MEMORY, "staging buffer");
+        hipStream_t side;
+        int rc = c->side_stream(&side);
+        if (rc != P2G_OK) return rc;
+        e = hipMemcpyAsync(sg.buf.ptr, data, len, hipMemcpyHostToDevice, side);
+        if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "upload slices", e);
+        // Descriptors of this push go to their own range of the job's tables,
+        // which no other push touches.
+        SliceDesc *hd = static_cast<SliceDesc *>(j->h_descs.ptr) + first_input_index;
+        const uint8_t *base = static_cast<const uint8_t *>(sg.buf.ptr);
+        for (uint64_t s = 0; s < n; s++) {
+            const uint64_t have = len - s * j->S;
+            hd[s] = {base + s * j->S, (uint32_t)(have < j->S ? have : j->S), 0};
+        }
+        SliceDesc *dd = static_cast<SliceDesc *>(j->d_descs.ptr) + first_input_index;
+        e = hipMemcpyAsync(dd, hd, n * sizeof(SliceDesc), hipMemcpyHostToDevice, side);
+        if (e == hipSuccess) e = hipEventRecord(sg.copied, side);
+        if (e == hipSuccess) e = hipStreamWaitEvent(c->stream, sg.copied, 0);
+        if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "upload slice descriptors", e);
+        rc = c->scratch_acquire(Ctx::kScratchEnc, c->stream);
+        if (rc == P2G_OK)
+            rc = dev_encode(c, dd, (uint32_t)n, j->S, first_input_index, j->N, j->R,
+                            static_cast<uint8_t *>(j->d_rec.ptr), 1, c->stream);
+        if (rc == P2G_OK) rc = c->scratch_release(Ctx::kScratchEnc, c->stream);
+        if (rc != P2G_OK) return rc;
+        e = hipEventRecord(sg.done, c->stream);
+        if (e == hipSuccess) e = hipEventSynchronize(sg.copied);
+        if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "event record", e);
+        return P2G_OK;
+    });
 }
 
 int p2g_encode_finish(p2g_job *job, uint8_t *rec_out, const uint8_t *set_id, uint8_t *pkt_md5_out) {
-    Job *j = reinterpret_cast<Job *>(job);
-    if (!j) return P2G_ERR_INVALID_ARGUMENT;
-    Ctx *c = j->ctx;
-    if (j->R == 0) return P2G_OK;
-    if (!rec_out || (set_id && !pkt_md5_out))
-        return c->fail(P2G_ERR_INVALID_ARGUMENT, "null argument");
-    hipSetDevice(c->device);
-    const size_t bytes = (size_t)j->R * j->S;
-    hipError_t e;
-    if (set_id) {
-        if (c->d_md5.reserve((size_t)j->R * 16) != P2G_OK)
-            return c->fail(P2G_ERR_OUT_OF_MEMORY, "packet md5 buffer");
-        e = launch_recovery_md5(static_cast<const uint8_t *>(j->d_rec.ptr), j->R, (uint32_t)j->S,
-                                0, set_id, static_cast<uint8_t *>(c->d_md5.ptr), c->stream);
-        if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "recovery md5 kernel", e);
-    }
-    e = hipMemcpyAsync(rec_out, j->d_rec.ptr, bytes, hipMemcpyDeviceToHost, c->stream);
-    if (e == hipSuccess && set_id)
-        e = hipMemcpyAsync(pkt_md5_out, c->d_md5.ptr, (size_t)j->R * 16, hipMemcpyDeviceToHost,
-                           c->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "encode finish", e);
-    return P2G_OK;
+    return guarded(job ? reinterpret_cast<Job *>(job)->ctx : nullptr, [&]() -> int {
+        Job *j = reinterpret_cast<Job *>(job);
+        if (!j) return P2G_ERR_INVALID_ARGUMENT;
+        Ctx *c = j->ctx;
+        if (j->R == 0) return P2G_OK;
+        if (!rec_out || (set_id && !pkt_md5_out))
+            return c->fail(P2G_ERR_INVALID_ARGUMENT, "null argument");
+        hipSetDevice(c->device);
+        const size_t bytes = (size_t)j->R * j->S;
+        hipError_t e;
+        if (set_id) {
+            if (c->d_md5.reserve((size_t)j->R * 16) != P2G_OK)
+                return c->fail(P2G_ERR_OUT_OF_MEMORY, "packet md5 buffer");
+            e = launch_recovery_md5(static_cast<const uint8_t *>(j->d_rec.ptr), j->R, (uint32_t)j->S,
+                                    0, set_id, static_cast<uint8_t *>(c->d_md5.ptr), c->stream);
+            if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "recovery md5 kernel", e);
+        }
+        e = hipMemcpyAsync(rec_out, j->d_rec.ptr, bytes, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess && set_id)
+            e = hipMemcpyAsync(pkt_md5_out, c->d_md5.ptr, (size_t)j->R * 16, hipMemcpyDeviceToHost,
+                               c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "encode finish", e);
+        return P2G_OK;
+    });
 }
 
 void p2g_encode_free(p2g_job *job) {

@@ -448,53 +448,63 @@ extern "C" {
 
 int p2g_batch_plan(p2g_ctx *ctx, const p2g_batch_job *jobs, uint32_t njobs, int use_graph,
                    p2g_batch **out) {
-    Ctx *c = reinterpret_cast<Ctx *>(ctx);
-    if (!c || !out) return P2G_ERR_INVALID_ARGUMENT;
-    if (const char *v = getenv("P2G_BATCH_GRAPH")) use_graph = atoi(v);
-    BatchPlan *B = nullptr;
-    int rc = make_plan(c, jobs, njobs, use_graph != 0, &B);
-    *out = reinterpret_cast<p2g_batch *>(B);
-    return rc;
+    return guarded(reinterpret_cast<Ctx *>(ctx), [&]() -> int {
+        Ctx *c = reinterpret_cast<Ctx *>(ctx);
+        if (!c || !out) return P2G_ERR_INVALID_ARGUMENT;
+        if (const char *v = getenv("P2G_BATCH_GRAPH")) use_graph = atoi(v);
+        BatchPlan *B = nullptr;
+        int rc = make_plan(c, jobs, njobs, use_graph != 0, &B);
+        *out = reinterpret_cast<p2g_batch *>(B);
+        return rc;
+    });
 }
 
 int p2g_batch_launch(p2g_batch *b, void *stream) {
-    BatchPlan *B = reinterpret_cast<BatchPlan *>(b);
-    if (!B) return P2G_ERR_INVALID_ARGUMENT;
-    return launch_plan(B, stream ? static_cast<hipStream_t>(stream) : B->c->stream);
+    return guarded(b ? reinterpret_cast<BatchPlan *>(b)->c : nullptr, [&]() -> int {
+        BatchPlan *B = reinterpret_cast<BatchPlan *>(b);
+        if (!B) return P2G_ERR_INVALID_ARGUMENT;
+        return launch_plan(B, stream ? static_cast<hipStream_t>(stream) : B->c->stream);
+    });
 }
 
 int p2g_batch_wait(p2g_batch *b) {
-    BatchPlan *B = reinterpret_cast<BatchPlan *>(b);
-    if (!B) return P2G_ERR_INVALID_ARGUMENT;
-    return wait_plan(B);
+    return guarded(b ? reinterpret_cast<BatchPlan *>(b)->c : nullptr, [&]() -> int {
+        BatchPlan *B = reinterpret_cast<BatchPlan *>(b);
+        if (!B) return P2G_ERR_INVALID_ARGUMENT;
+        return wait_plan(B);
+    });
 }
 
 int p2g_batch_job_ms(p2g_batch *b, double *ms_out) {
-    BatchPlan *B = reinterpret_cast<BatchPlan *>(b);
-    if (!B || !ms_out) return P2G_ERR_INVALID_ARGUMENT;
-    if (B->h_done.size() != B->njobs + 1)
-        return B->c->fail(P2G_ERR_INVALID_ARGUMENT, "no finished batch launch to report");
-    const unsigned long long t0 = B->h_done[B->njobs];
-    for (uint32_t j = 0; j < B->njobs; j++)
-        ms_out[j] = B->h_done[j] >= t0 ? (double)(B->h_done[j] - t0) * 1e-5 : 0.0;
-    return P2G_OK;
+    return guarded(b ? reinterpret_cast<BatchPlan *>(b)->c : nullptr, [&]() -> int {
+        BatchPlan *B = reinterpret_cast<BatchPlan *>(b);
+        if (!B || !ms_out) return P2G_ERR_INVALID_ARGUMENT;
+        if (B->h_done.size() != B->njobs + 1)
+            return B->c->fail(P2G_ERR_INVALID_ARGUMENT, "no finished batch launch to report");
+        const unsigned long long t0 = B->h_done[B->njobs];
+        for (uint32_t j = 0; j < B->njobs; j++)
+            ms_out[j] = B->h_done[j] >= t0 ? (double)(B->h_done[j] - t0) * 1e-5 : 0.0;
+        return P2G_OK;
+    });
 }
 
 void p2g_batch_free(p2g_batch *b) { delete reinterpret_cast<BatchPlan *>(b); }
 
 int p2g_dev_create_batch(p2g_ctx *ctx, const p2g_batch_job *jobs, uint32_t njobs, void *stream) {
-    Ctx *c = reinterpret_cast<Ctx *>(ctx);
-    if (!c) return P2G_ERR_INVALID_ARGUMENT;
-    BatchPlan *B = nullptr;
-    int rc = make_plan(c, jobs, njobs, false, &B);
-    if (rc != P2G_OK) return rc;
-    rc = launch_plan(B, stream ? static_cast<hipStream_t>(stream) : c->stream);
-    if (rc != P2G_OK) {
-        delete B;
-        return rc;
-    }
-    c->pending_batches.push_back(B);  // p2g_synchronize waits, checks and frees it
-    return P2G_OK;
+    return guarded(reinterpret_cast<Ctx *>(ctx), [&]() -> int {
+        Ctx *c = reinterpret_cast<Ctx *>(ctx);
+        if (!c) return P2G_ERR_INVALID_ARGUMENT;
+        BatchPlan *B = nullptr;
+        int rc = make_plan(c, jobs, njobs, false, &B);
+        if (rc != P2G_OK) return rc;
+        rc = launch_plan(B, stream ? static_cast<hipStream_t>(stream) : c->stream);
+        if (rc != P2G_OK) {
+            delete B;
+            return rc;
+        }
+        c->pending_batches.push_back(B);  // p2g_synchronize waits, checks and frees it
+        return P2G_OK;
+    });
 }
 
 }  // extern "C"

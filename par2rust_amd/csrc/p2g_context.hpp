@@ -4,8 +4,10 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <exception>
 #include <functional>
 #include <map>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -164,5 +166,34 @@ int tune_and_run(Ctx *c, uint64_t n, uint32_t R, uint64_t S, hipStream_t st,
                  const std::function<int(int, uint32_t)> &run, TuneChoice *out);
 int tune_load(Ctx *c, const char *path);
 int tune_save(Ctx *c, const char *path);
+
+// Runs the body of a C entry point so that no C++ exception crosses the C
+// ABI: a failed host allocation (std::bad_alloc) returns
+// P2G_ERR_OUT_OF_MEMORY, any other exception (a thread that could not start,
+// a failed system call reported by the standard library) P2G_ERR_IO, as the
+// reference reports OS failures as Par2Error::Io (src/error.rs).  The text
+// goes to c's last error when there is a context.
+template <typename F>
+int guarded(Ctx *c, F &&body) noexcept {
+    const char *what = nullptr;
+    int rc = P2G_ERR_IO;
+    try {
+        return body();
+    } catch (const std::bad_alloc &) {
+        what = "host memory allocation failed";
+        rc = P2G_ERR_OUT_OF_MEMORY;
+    } catch (const std::exception &e) {
+        what = e.what();
+    } catch (...) {
+        what = "unexpected host failure";
+    }
+    if (c) {
+        try {
+            c->err = what;
+        } catch (...) {
+        }
+    }
+    return rc;
+}
 
 }  // namespace p2g

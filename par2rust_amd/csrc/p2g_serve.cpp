@@ -233,143 +233,155 @@ using namespace p2g;
 extern "C" {
 
 int p2g_server_open(p2g_ctx *ctx, uint64_t hbm_budget, double step_ms, p2g_server **out) {
-    Ctx *c = reinterpret_cast<Ctx *>(ctx);
-    if (!c || !out || hbm_budget == 0) return P2G_ERR_INVALID_ARGUMENT;
-    *out = nullptr;
-    hipSetDevice(c->device);
-    Server *s = new (std::nothrow) Server();
-    if (!s) return c->fail(P2G_ERR_OUT_OF_MEMORY, "server");
-    s->c = c;
-    if (step_ms > 0) s->step_ms = step_ms;
-    s->pool.size = a256(hbm_budget);
-    void *p = nullptr;
-    hipError_t e = hipMalloc(&p, s->pool.size);
-    if (e != hipSuccess) {
-        delete s;
-        return c->fail(P2G_ERR_OUT_OF_MEMORY, "server HBM pool", e);
-    }
-    s->pool.base = static_cast<uint8_t *>(p);
-    s->pool.free_[0] = s->pool.size;
-    e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
-    if (e != hipSuccess) {
-        delete s;
-        return c->fail(P2G_ERR_DEVICE, "server stream", e);
-    }
-    for (Step &st : s->steps) st.plan.afrag = &s->afrag;
-    *out = reinterpret_cast<p2g_server *>(s);
-    return P2G_OK;
+    return guarded(reinterpret_cast<Ctx *>(ctx), [&]() -> int {
+        Ctx *c = reinterpret_cast<Ctx *>(ctx);
+        if (!c || !out || hbm_budget == 0) return P2G_ERR_INVALID_ARGUMENT;
+        *out = nullptr;
+        hipSetDevice(c->device);
+        Server *s = new (std::nothrow) Server();
+        if (!s) return c->fail(P2G_ERR_OUT_OF_MEMORY, "server");
+        s->c = c;
+        if (step_ms > 0) s->step_ms = step_ms;
+        s->pool.size = a256(hbm_budget);
+        void *p = nullptr;
+        hipError_t e = hipMalloc(&p, s->pool.size);
+        if (e != hipSuccess) {
+            delete s;
+            return c->fail(P2G_ERR_OUT_OF_MEMORY, "server HBM pool", e);
+        }
+        s->pool.base = static_cast<uint8_t *>(p);
+        s->pool.free_[0] = s->pool.size;
+        e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            delete s;
+            return c->fail(P2G_ERR_DEVICE, "server stream", e);
+        }
+        for (Step &st : s->steps) st.plan.afrag = &s->afrag;
+        *out = reinterpret_cast<p2g_server *>(s);
+        return P2G_OK;
+    });
 }
 
 int p2g_server_submit(p2g_server *srv, const p2g_request *r, uint64_t *id_out) {
-    Server *s = reinterpret_cast<Server *>(srv);
-    if (!s || !r || !id_out) return P2G_ERR_INVALID_ARGUMENT;
-    Ctx *c = s->c;
-    if (r->nfiles && !r->file_lens) return c->fail(P2G_ERR_INVALID_ARGUMENT, "null file lengths");
-    int rc = check_slice_size(c, r->slice_size);
-    if (rc != P2G_OK) return rc;
-    if (r->recovery_count > P2G_MAX_RECOVERY_BLOCKS)
-        return c->fail(P2G_ERR_TOO_MANY_RECOVERY_BLOCKS, "recovery count above 65535");
-    auto q = std::make_unique<Request>();
-    q->S = r->slice_size;
-    q->R = r->recovery_count;
-    uint64_t n = 0, bytes = 0;
-    for (uint32_t f = 0; f < r->nfiles; f++) {
-        q->lens.push_back(r->file_lens[f]);
-        n += (r->file_lens[f] + q->S - 1) / q->S;
-        bytes += a256(r->file_lens[f]);
-        if (r->host_data) q->host.push_back(r->host_data[f]);
-        q->seeds.push_back(r->fill_seeds ? r->fill_seeds[f] : 0);
-    }
-    if (n == 0) return c->fail(P2G_ERR_INVALID_ARGUMENT, "a request has no slices");
-    if (n > kMaxInputs) return c->fail(P2G_ERR_TOO_MANY_INPUT_BLOCKS, "more than 32768 input slices");
-    q->n = (uint32_t)n;
-    memcpy(q->set_id, r->set_id, 16);
-    q->footprint = bytes + a256((uint64_t)q->R * q->S) + a256(16 * n) + a256(4 * n) + a256(16ull * q->R);
-    if (q->footprint > s->pool.size)
-        return c->fail(P2G_ERR_OUT_OF_MEMORY, "request larger than the server's HBM budget");
-    q->ms_per_row = 256.0 * (double)n * (double)q->S / kServeOpsPerMs;
-    q->ms_per_slice = (double)((q->S + 8) / 64 + 1) * kServeMd5BlockMs;
-    q->id = s->next_id++;
-    q->t_submit = s->now_ms();
-    *id_out = q->id;
-    s->queued.push_back(q.get());
-    s->all[q->id] = std::move(q);
-    return P2G_OK;
+    return guarded(srv ? reinterpret_cast<Server *>(srv)->c : nullptr, [&]() -> int {
+        Server *s = reinterpret_cast<Server *>(srv);
+        if (!s || !r || !id_out) return P2G_ERR_INVALID_ARGUMENT;
+        Ctx *c = s->c;
+        if (r->nfiles && !r->file_lens) return c->fail(P2G_ERR_INVALID_ARGUMENT, "null file lengths");
+        int rc = check_slice_size(c, r->slice_size);
+        if (rc != P2G_OK) return rc;
+        if (r->recovery_count > P2G_MAX_RECOVERY_BLOCKS)
+            return c->fail(P2G_ERR_TOO_MANY_RECOVERY_BLOCKS, "recovery count above 65535");
+        auto q = std::make_unique<Request>();
+        q->S = r->slice_size;
+        q->R = r->recovery_count;
+        uint64_t n = 0, bytes = 0;
+        for (uint32_t f = 0; f < r->nfiles; f++) {
+            q->lens.push_back(r->file_lens[f]);
+            n += (r->file_lens[f] + q->S - 1) / q->S;
+            bytes += a256(r->file_lens[f]);
+            if (r->host_data) q->host.push_back(r->host_data[f]);
+            q->seeds.push_back(r->fill_seeds ? r->fill_seeds[f] : 0);
+        }
+        if (n == 0) return c->fail(P2G_ERR_INVALID_ARGUMENT, "a request has no slices");
+        if (n > kMaxInputs) return c->fail(P2G_ERR_TOO_MANY_INPUT_BLOCKS, "more than 32768 input slices");
+        q->n = (uint32_t)n;
+        memcpy(q->set_id, r->set_id, 16);
+        q->footprint = bytes + a256((uint64_t)q->R * q->S) + a256(16 * n) + a256(4 * n) + a256(16ull * q->R);
+        if (q->footprint > s->pool.size)
+            return c->fail(P2G_ERR_OUT_OF_MEMORY, "request larger than the server's HBM budget");
+        q->ms_per_row = 256.0 * (double)n * (double)q->S / kServeOpsPerMs;
+        q->ms_per_slice = (double)((q->S + 8) / 64 + 1) * kServeMd5BlockMs;
+        q->id = s->next_id++;
+        q->t_submit = s->now_ms();
+        *id_out = q->id;
+        s->queued.push_back(q.get());
+        s->all[q->id] = std::move(q);
+        return P2G_OK;
+    });
 }
 
 int p2g_server_poll(p2g_server *srv, double wait_ms, uint64_t *done_ids, uint32_t cap,
                     uint32_t *n_done) {
-    Server *s = reinterpret_cast<Server *>(srv);
-    if (!s || (cap && !done_ids)) return P2G_ERR_INVALID_ARGUMENT;
-    hipSetDevice(s->c->device);
-    const double until = s->now_ms() + std::max(0.0, wait_ms);
-    int rc = P2G_OK;
-    for (;;) {
-        // retire finished steps, in launch order
-        while (!s->inflight.empty()) {
-            const int k = s->inflight.front();
-            const hipError_t q = hipEventQuery(s->steps[k].plan.done);
-            if (q == hipErrorNotReady) break;
-            s->inflight.pop_front();
-            const int r = s->retire(k);
-            if (rc == P2G_OK) rc = r;
+    return guarded(srv ? reinterpret_cast<Server *>(srv)->c : nullptr, [&]() -> int {
+        Server *s = reinterpret_cast<Server *>(srv);
+        if (!s || (cap && !done_ids)) return P2G_ERR_INVALID_ARGUMENT;
+        hipSetDevice(s->c->device);
+        const double until = s->now_ms() + std::max(0.0, wait_ms);
+        int rc = P2G_OK;
+        for (;;) {
+            // retire finished steps, in launch order
+            while (!s->inflight.empty()) {
+                const int k = s->inflight.front();
+                const hipError_t q = hipEventQuery(s->steps[k].plan.done);
+                if (q == hipErrorNotReady) break;
+                s->inflight.pop_front();
+                const int r = s->retire(k);
+                if (rc == P2G_OK) rc = r;
+            }
+            if (rc == P2G_OK) rc = s->admit();
+            // keep one step running and the next one queued behind it
+            while (rc == P2G_OK && s->inflight.size() < 2) {
+                int k = !s->steps[0].busy ? 0 : !s->steps[1].busy ? 1 : -1;
+                if (k < 0) break;
+                const size_t before = s->inflight.size();
+                rc = s->launch_step(k);
+                if (s->inflight.size() == before) break;  // nothing to run
+            }
+            if (rc != P2G_OK || !s->finished.empty() || s->now_ms() >= until) break;
+            std::this_thread::sleep_for(std::chrono::microseconds(50));
         }
-        if (rc == P2G_OK) rc = s->admit();
-        // keep one step running and the next one queued behind it
-        while (rc == P2G_OK && s->inflight.size() < 2) {
-            int k = !s->steps[0].busy ? 0 : !s->steps[1].busy ? 1 : -1;
-            if (k < 0) break;
-            const size_t before = s->inflight.size();
-            rc = s->launch_step(k);
-            if (s->inflight.size() == before) break;  // nothing to run
+        uint32_t k = 0;
+        while (k < cap && !s->finished.empty()) {
+            done_ids[k++] = s->finished.front();
+            s->finished.pop_front();
         }
-        if (rc != P2G_OK || !s->finished.empty() || s->now_ms() >= until) break;
-        std::this_thread::sleep_for(std::chrono::microseconds(50));
-    }
-    uint32_t k = 0;
-    while (k < cap && !s->finished.empty()) {
-        done_ids[k++] = s->finished.front();
-        s->finished.pop_front();
-    }
-    if (n_done) *n_done = k;
-    return rc;
+        if (n_done) *n_done = k;
+        return rc;
+    });
 }
 
 int p2g_server_result(p2g_server *srv, uint64_t id, p2g_batch_job *out, double times_ms[3]) {
-    Server *s = reinterpret_cast<Server *>(srv);
-    if (!s) return P2G_ERR_INVALID_ARGUMENT;
-    auto it = s->all.find(id);
-    if (it == s->all.end()) return s->c->fail(P2G_ERR_INVALID_ARGUMENT, "unknown request id");
-    Request *q = it->second.get();
-    if (q->state != Request::kDone) return s->c->fail(P2G_ERR_INVALID_ARGUMENT, "request not done");
-    if (out) *out = q->job;
-    if (times_ms) {
-        times_ms[0] = q->t_submit;
-        times_ms[1] = q->t_admit;
-        times_ms[2] = q->t_done;
-    }
-    return P2G_OK;
+    return guarded(srv ? reinterpret_cast<Server *>(srv)->c : nullptr, [&]() -> int {
+        Server *s = reinterpret_cast<Server *>(srv);
+        if (!s) return P2G_ERR_INVALID_ARGUMENT;
+        auto it = s->all.find(id);
+        if (it == s->all.end()) return s->c->fail(P2G_ERR_INVALID_ARGUMENT, "unknown request id");
+        Request *q = it->second.get();
+        if (q->state != Request::kDone) return s->c->fail(P2G_ERR_INVALID_ARGUMENT, "request not done");
+        if (out) *out = q->job;
+        if (times_ms) {
+            times_ms[0] = q->t_submit;
+            times_ms[1] = q->t_admit;
+            times_ms[2] = q->t_done;
+        }
+        return P2G_OK;
+    });
 }
 
 int p2g_server_release(p2g_server *srv, uint64_t id) {
-    Server *s = reinterpret_cast<Server *>(srv);
-    if (!s) return P2G_ERR_INVALID_ARGUMENT;
-    auto it = s->all.find(id);
-    if (it == s->all.end()) return s->c->fail(P2G_ERR_INVALID_ARGUMENT, "unknown request id");
-    Request *q = it->second.get();
-    if (q->state != Request::kDone) return s->c->fail(P2G_ERR_INVALID_ARGUMENT, "request not done");
-    s->pool.release(q->off, q->footprint);
-    s->all.erase(it);
-    return P2G_OK;
+    return guarded(srv ? reinterpret_cast<Server *>(srv)->c : nullptr, [&]() -> int {
+        Server *s = reinterpret_cast<Server *>(srv);
+        if (!s) return P2G_ERR_INVALID_ARGUMENT;
+        auto it = s->all.find(id);
+        if (it == s->all.end()) return s->c->fail(P2G_ERR_INVALID_ARGUMENT, "unknown request id");
+        Request *q = it->second.get();
+        if (q->state != Request::kDone) return s->c->fail(P2G_ERR_INVALID_ARGUMENT, "request not done");
+        s->pool.release(q->off, q->footprint);
+        s->all.erase(it);
+        return P2G_OK;
+    });
 }
 
 int p2g_server_stats(p2g_server *srv, uint32_t *queued, uint32_t *active, uint64_t *hbm_used) {
-    Server *s = reinterpret_cast<Server *>(srv);
-    if (!s) return P2G_ERR_INVALID_ARGUMENT;
-    if (queued) *queued = (uint32_t)s->queued.size();
-    if (active) *active = (uint32_t)s->active.size();
-    if (hbm_used) *hbm_used = s->pool.used;
-    return P2G_OK;
+    return guarded(srv ? reinterpret_cast<Server *>(srv)->c : nullptr, [&]() -> int {
+        Server *s = reinterpret_cast<Server *>(srv);
+        if (!s) return P2G_ERR_INVALID_ARGUMENT;
+        if (queued) *queued = (uint32_t)s->queued.size();
+        if (active) *active = (uint32_t)s->active.size();
+        if (hbm_used) *hbm_used = s->pool.used;
+        return P2G_OK;
+    });
 }
 
 void p2g_server_close(p2g_server *srv) { delete reinterpret_cast<Server *>(srv); }

@@ -136,73 +136,81 @@ uint32_t p2g_tune_bucket(uint64_t slices, uint32_t recovery_count, uint64_t slic
 
 int p2g_autotune(p2g_ctx *ctx, uint32_t slices, uint32_t recovery_count, uint64_t slice_size,
                  int *variant_out, uint32_t *band_cols_out, double *ms_out) {
-    Ctx *c = reinterpret_cast<Ctx *>(ctx);
-    if (!c) return P2G_ERR_INVALID_ARGUMENT;
-    if (slices == 0 || slices > kMaxInputs || recovery_count == 0 ||
-        recovery_count > P2G_MAX_RECOVERY_BLOCKS)
-        return c->fail(P2G_ERR_INVALID_ARGUMENT, "autotune shape out of range");
-    int rc = check_slice_size(c, slice_size);
-    if (rc != P2G_OK) return rc;
-    hipSetDevice(c->device);
-    // synthetic input of the shape: one file of slices * slice_size bytes
-    const uint64_t bytes = (uint64_t)slices * slice_size;
-    const uint64_t rec_bytes = (uint64_t)recovery_count * slice_size;
-    DevBuf src, out;
-    if (src.reserve(bytes) != P2G_OK ||
-        out.reserve(rec_bytes + 20ull * slices + 16ull * recovery_count + 1024) != P2G_OK) {
+    return guarded(reinterpret_cast<Ctx *>(ctx), [&]() -> int {
+        Ctx *c = reinterpret_cast<Ctx *>(ctx);
+        if (!c) return P2G_ERR_INVALID_ARGUMENT;
+        if (slices == 0 || slices > kMaxInputs || recovery_count == 0 ||
+            recovery_count > P2G_MAX_RECOVERY_BLOCKS)
+            return c->fail(P2G_ERR_INVALID_ARGUMENT, "autotune shape out of range");
+        int rc = check_slice_size(c, slice_size);
+        if (rc != P2G_OK) return rc;
+        hipSetDevice(c->device);
+        // synthetic input of the shape: one file of slices * slice_size bytes
+        const uint64_t bytes = (uint64_t)slices * slice_size;
+        const uint64_t rec_bytes = (uint64_t)recovery_count * slice_size;
+        DevBuf src, out;
+        if (src.reserve(bytes) != P2G_OK ||
+            out.reserve(rec_bytes + 20ull * slices + 16ull * recovery_count + 1024) != P2G_OK) {
+            src.release();
+            out.release();
+            return c->fail(P2G_ERR_OUT_OF_MEMORY, "autotune buffers");
+        }
+        uint8_t *o = static_cast<uint8_t *>(out.ptr);
+        uint8_t *rec = o, *md5 = o + (rec_bytes + 255) / 256 * 256;
+        uint32_t *crc = reinterpret_cast<uint32_t *>(md5 + (16ull * slices + 255) / 256 * 256);
+        uint8_t *pkt = reinterpret_cast<uint8_t *>(crc) + (4ull * slices + 255) / 256 * 256;
+        static const uint8_t sid[16] = {};
+        rc = p2g_dev_fill_xorshift64(ctx, static_cast<uint8_t *>(src.ptr), bytes, 1, c->stream);
+        p2g_dev_file file{static_cast<const uint8_t *>(src.ptr), bytes};
+        TuneChoice ch;
+        if (rc == P2G_OK)
+            rc = tune_and_run(c, slices, recovery_count, slice_size, c->stream,
+                              [&](int variant, uint32_t band) {
+                                  return create_pass_impl(c, &file, 1, slice_size, recovery_count, sid,
+                                                          rec, md5, crc, pkt, c->stream, variant, band);
+                              },
+                              &ch);
+        if (rc == P2G_OK) rc = p2g_synchronize(ctx);
         src.release();
         out.release();
-        return c->fail(P2G_ERR_OUT_OF_MEMORY, "autotune buffers");
-    }
-    uint8_t *o = static_cast<uint8_t *>(out.ptr);
-    uint8_t *rec = o, *md5 = o + (rec_bytes + 255) / 256 * 256;
-    uint32_t *crc = reinterpret_cast<uint32_t *>(md5 + (16ull * slices + 255) / 256 * 256);
-    uint8_t *pkt = reinterpret_cast<uint8_t *>(crc) + (4ull * slices + 255) / 256 * 256;
-    static const uint8_t sid[16] = {};
-    rc = p2g_dev_fill_xorshift64(ctx, static_cast<uint8_t *>(src.ptr), bytes, 1, c->stream);
-    p2g_dev_file file{static_cast<const uint8_t *>(src.ptr), bytes};
-    TuneChoice ch;
-    if (rc == P2G_OK)
-        rc = tune_and_run(c, slices, recovery_count, slice_size, c->stream,
-                          [&](int variant, uint32_t band) {
-                              return create_pass_impl(c, &file, 1, slice_size, recovery_count, sid,
-                                                      rec, md5, crc, pkt, c->stream, variant, band);
-                          },
-                          &ch);
-    if (rc == P2G_OK) rc = p2g_synchronize(ctx);
-    src.release();
-    out.release();
-    if (rc != P2G_OK) return rc;
-    if (variant_out) *variant_out = ch.variant;
-    if (band_cols_out) *band_cols_out = ch.band_cols;
-    if (ms_out) *ms_out = ch.ms;
-    return P2G_OK;
+        if (rc != P2G_OK) return rc;
+        if (variant_out) *variant_out = ch.variant;
+        if (band_cols_out) *band_cols_out = ch.band_cols;
+        if (ms_out) *ms_out = ch.ms;
+        return P2G_OK;
+    });
 }
 
 int p2g_tune_get(p2g_ctx *ctx, uint32_t slices, uint32_t recovery_count, uint64_t slice_size,
                  int *variant_out, uint32_t *band_cols_out) {
-    Ctx *c = reinterpret_cast<Ctx *>(ctx);
-    if (!c) return P2G_ERR_INVALID_ARGUMENT;
-    TuneChoice ch;
-    if (!tune_lookup(c, slices, recovery_count, slice_size, &ch)) {
-        ch.variant = c->encode_variant;
-        ch.band_cols = c->band_cols;
-    }
-    if (variant_out) *variant_out = ch.variant;
-    if (band_cols_out) *band_cols_out = ch.band_cols;
-    return P2G_OK;
+    return guarded(reinterpret_cast<Ctx *>(ctx), [&]() -> int {
+        Ctx *c = reinterpret_cast<Ctx *>(ctx);
+        if (!c) return P2G_ERR_INVALID_ARGUMENT;
+        TuneChoice ch;
+        if (!tune_lookup(c, slices, recovery_count, slice_size, &ch)) {
+            ch.variant = c->encode_variant;
+            ch.band_cols = c->band_cols;
+        }
+        if (variant_out) *variant_out = ch.variant;
+        if (band_cols_out) *band_cols_out = ch.band_cols;
+        return P2G_OK;
+    });
 }
 
 int p2g_tune_save(p2g_ctx *ctx, const char *path) {
-    Ctx *c = reinterpret_cast<Ctx *>(ctx);
-    if (!c || !path) return P2G_ERR_INVALID_ARGUMENT;
-    return tune_save(c, path);
+    return guarded(reinterpret_cast<Ctx *>(ctx), [&]() -> int {
+        Ctx *c = reinterpret_cast<Ctx *>(ctx);
+        if (!c || !path) return P2G_ERR_INVALID_ARGUMENT;
+        return tune_save(c, path);
+    });
 }
 
 int p2g_tune_load(p2g_ctx *ctx, const char *path) {
-    Ctx *c = reinterpret_cast<Ctx *>(ctx);
-    if (!c || !path) return P2G_ERR_INVALID_ARGUMENT;
-    return tune_load(c, path);
+    return guarded(reinterpret_cast<Ctx *>(ctx), [&]() -> int {
+        Ctx *c = reinterpret_cast<Ctx *>(ctx);
+        if (!c || !path) return P2G_ERR_INVALID_ARGUMENT;
+        return tune_load(c, path);
+    });
 }
 
 }  // extern "C"

@@ -801,45 +801,47 @@ extern "C" int p2g_run_create(p2g_ctx *ctx, const char *output, uint64_t slice_s
                               uint32_t nfiles, const char *const *paths,
                               const uint8_t *const *names, const uint32_t *name_lens,
                               char *names_out, size_t names_cap) {
-    if (!ctx || !output || !scheme || (nfiles && (!paths || !names || !name_lens)))
-        return P2G_ERR_INVALID_ARGUMENT;
-    if (names_out && names_cap) names_out[0] = 0;
-    par2::Error err;
-    par2::CreateOptions opts;
-    opts.output = output;
-    opts.slice_size = slice_size;
-    opts.recovery_block_count = recovery_count;
-    opts.volume_scheme.kind = scheme->kind;
-    opts.volume_scheme.uniform_count = scheme->uniform_count;
-    opts.volume_scheme.limited = scheme->limited != 0;
-    opts.volume_scheme.limit_cap = scheme->limit_cap;
-    if (scheme->explicit_sizes)
-        opts.volume_scheme.explicit_sizes.assign(scheme->explicit_sizes,
-                                                 scheme->explicit_sizes + scheme->n_explicit);
-    std::vector<std::string> written;
-    std::vector<std::string> path_v(paths, paths + nfiles);
-    std::vector<std::vector<uint8_t>> name_v(nfiles);
-    for (uint32_t f = 0; f < nfiles; f++) name_v[f].assign(names[f], names[f] + name_lens[f]);
-    // single read with the set resident in HBM; the streaming scan + encode
-    // when the set does not fit (or P2G_CREATE_STREAM=1)
-    bool declined = true;
-    int rc = P2G_OK;
-    const char *stream_env = getenv("P2G_CREATE_STREAM");
-    if (!(stream_env && atoi(stream_env)))
-        rc = par2::create_resident(ctx, opts, path_v, name_v, written, err, &declined);
-    if (declined) {
-        std::vector<par2::SourceFile> sources(nfiles);
-        for (uint32_t f = 0; f < nfiles && rc == P2G_OK; f++)
-            rc = par2::scan(ctx, paths[f], name_v[f], slice_size, sources[f], err);
-        if (rc == P2G_OK) rc = par2::run_create(ctx, opts, sources, written, err);
-    }
-    size_t used = 0;
-    for (auto &w : written) {
-        if (names_out && used + w.size() + 1 <= names_cap) {
-            memcpy(names_out + used, w.c_str(), w.size() + 1);
-            used += w.size() + 1;
+    return p2g::guarded(reinterpret_cast<p2g::Ctx *>(ctx), [&]() -> int {
+        if (!ctx || !output || !scheme || (nfiles && (!paths || !names || !name_lens)))
+            return P2G_ERR_INVALID_ARGUMENT;
+        if (names_out && names_cap) names_out[0] = 0;
+        par2::Error err;
+        par2::CreateOptions opts;
+        opts.output = output;
+        opts.slice_size = slice_size;
+        opts.recovery_block_count = recovery_count;
+        opts.volume_scheme.kind = scheme->kind;
+        opts.volume_scheme.uniform_count = scheme->uniform_count;
+        opts.volume_scheme.limited = scheme->limited != 0;
+        opts.volume_scheme.limit_cap = scheme->limit_cap;
+        if (scheme->explicit_sizes)
+            opts.volume_scheme.explicit_sizes.assign(scheme->explicit_sizes,
+                                                     scheme->explicit_sizes + scheme->n_explicit);
+        std::vector<std::string> written;
+        std::vector<std::string> path_v(paths, paths + nfiles);
+        std::vector<std::vector<uint8_t>> name_v(nfiles);
+        for (uint32_t f = 0; f < nfiles; f++) name_v[f].assign(names[f], names[f] + name_lens[f]);
+        // single read with the set resident in HBM; the streaming scan + encode
+        // when the set does not fit (or P2G_CREATE_STREAM=1)
+        bool declined = true;
+        int rc = P2G_OK;
+        const char *stream_env = getenv("P2G_CREATE_STREAM");
+        if (!(stream_env && atoi(stream_env)))
+            rc = par2::create_resident(ctx, opts, path_v, name_v, written, err, &declined);
+        if (declined) {
+            std::vector<par2::SourceFile> sources(nfiles);
+            for (uint32_t f = 0; f < nfiles && rc == P2G_OK; f++)
+                rc = par2::scan(ctx, paths[f], name_v[f], slice_size, sources[f], err);
+            if (rc == P2G_OK) rc = par2::run_create(ctx, opts, sources, written, err);
         }
-    }
-    if (rc != P2G_OK) reinterpret_cast<p2g::Ctx *>(ctx)->err = err.message;
-    return rc;
+        size_t used = 0;
+        for (auto &w : written) {
+            if (names_out && used + w.size() + 1 <= names_cap) {
+                memcpy(names_out + used, w.c_str(), w.size() + 1);
+                used += w.size() + 1;
+            }
+        }
+        if (rc != P2G_OK) reinterpret_cast<p2g::Ctx *>(ctx)->err = err.message;
+        return rc;
+    });
 }
