@@ -791,11 +791,20 @@ __device__ __forceinline__ void mf_unit_dma(const uint4 *s_desc, uint4 *A0, uint
                 __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(av, Bc[cb], acc[a][cb], 4, 4, \
                                                                 0, kE8M0One, 0, kE8M0One);    \
         }                                                                                     \
-        /* LDS reads first, then half the MFMAs while they land (the B build */            \
-        /* after them waits for every LDS read), then MFMAs with the VALU */                 \
-        __builtin_amdgcn_sched_group_barrier(0x100, 11, 0);                                   \
-        __builtin_amdgcn_sched_group_barrier(0x008, kMfTiles, 0);                             \
-        _Pragma("unroll") for (int k = 0; k < kMfTiles; k++) {                                \
+        /* Four MFMAs straight after the barrier (their operands are in */                   \
+        /* registers), so the matrix pipe runs while the DMA is issued; */                   \
+        /* then the DMA, the LDS reads two per MFMA, and the B build's VALU */               \
+        /* between the last MFMAs (an MFMA issue holds the wave until the */                 \
+        /* pipe takes it, so only work placed between MFMAs overlaps them). */               \
+        __builtin_amdgcn_sched_group_barrier(0x008, 4, 0);                                    \
+        __builtin_amdgcn_sched_group_barrier(0x010, 4, 0);                                    \
+        _Pragma("unroll") for (int k = 0; k < 5; k++) {                                       \
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);                                \
+            __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);                                \
+        }                                                                                     \
+        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);                                    \
+        __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);                                    \
+        _Pragma("unroll") for (int k = 0; k < 6; k++) {                                       \
             __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);                                \
             __builtin_amdgcn_sched_group_barrier(0x002, 5, 0);                                \
         }                                                                                     \
