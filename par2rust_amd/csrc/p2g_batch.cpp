@@ -298,8 +298,23 @@ int build_plan(Ctx *c, const std::vector<PlanItem> &items, bool use_graph, Batch
         // [slice_lo, slice_hi) may hold the descriptors of items without
         // checksums; the lanes skip them, and only checksummed slices count
         uint32_t hashed = 0;
-        for (uint32_t q = sp.s_lo; q < sp.s_hi; q++) hashed += sjobs[q].slice_end - sjobs[q].slice0;
-        plan_batch_group(c->num_cus, tasks, hashed, units, &g);
+        BatchGroupWork w;
+        for (uint32_t q = sp.s_lo; q < sp.s_hi; q++) {
+            const uint32_t cnt = sjobs[q].slice_end - sjobs[q].slice0;
+            const double blocks = (double)((sjobs[q].S + 8) / 64 + 1);  // MD5 blocks of a slice
+            hashed += cnt;
+            w.slices += cnt;
+            w.slice_blocks += cnt * blocks;
+        }
+        for (uint32_t p = sp.p_lo; p < sp.p_hi; p++) {
+            const BatchPiece &P = pieces[p];
+            const double blocks = (double)((36ull + P.S + 8) / 64 + 1);  // of a packet's hash
+            const uint32_t t = (P.R + 63) / 64;
+            w.enc_steps += (double)P.row_blocks * P.col_groups * P.nq;
+            w.tasks += t;
+            w.task_blocks += t * blocks;
+        }
+        plan_batch_group(c->num_cus, tasks, hashed, units, w, &g);
         B->sync_off.push_back(sync_words_total);
         sync_words_total += (uint32_t)align_up(starts, 64);
     }

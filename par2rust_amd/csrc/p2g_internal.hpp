@@ -175,9 +175,18 @@ hipError_t launch_batch_afrag(const uint16_t *d_antilog, const uint16_t *d_logba
 // Sync reset, band starts, then the persistent launch of one group.
 hipError_t launch_create_batch(const BatchGroupLaunch &g, const SliceDesc *d_descs,
                                const uint4 *d_tail_blocks, hipStream_t stream);
-// Group sizing: MD5 workgroups for n_tasks packet tasks and n_slices slices.
+// The work of one launch group, for sizing its roles: encode unit K steps
+// (units x steps of their job), and the serial MD5 work of the slice lanes
+// and of the packet wave tasks, in 64-byte blocks and in slices and tasks.
+struct BatchGroupWork {
+    double enc_steps = 0;
+    double slice_blocks = 0, slices = 0;
+    double task_blocks = 0, tasks = 0;
+};
+// Group sizing: MD5 workgroups for n_tasks packet tasks and n_slices slices,
+// as few as are estimated to finish within the encode (the rest encode).
 void plan_batch_group(uint32_t num_cus, uint32_t n_tasks, uint32_t n_slices, uint32_t n_units,
-                      BatchGroupLaunch *g);
+                      const BatchGroupWork &w, BatchGroupLaunch *g);
 constexpr uint32_t kBatchProgress = 1;  // sync word: units published so far
 
 // Synthetic input: the xorshift64 stream from `seed` into d_dst.  d_jumps

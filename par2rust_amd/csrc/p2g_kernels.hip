@@ -2064,15 +2064,47 @@ hipError_t launch_batch_afrag(const uint16_t *d_antilog, const uint16_t *d_logba
 }
 
 void plan_batch_group(uint32_t num_cus, uint32_t n_tasks, uint32_t n_slices, uint32_t n_units,
-                      BatchGroupLaunch *g) {
+                      const BatchGroupWork &w, BatchGroupLaunch *g) {
     g->grid = num_cus;
     // four packet waves per packet workgroup, two MD5 + CRC-32 waves (128
     // lanes) per slice workgroup; the rest encode
-    g->h3_blocks = std::min<uint32_t>((n_tasks + 3) / 4, 8);
-    g->h2_blocks = std::min<uint32_t>((n_slices + 127) / 128, 24);
+    const uint32_t h3_cap = std::min<uint32_t>((n_tasks + 3) / 4, 8);
+    const uint32_t h2_cap = std::min<uint32_t>((n_slices + 127) / 128, 24);
     if (n_units == 0) {
+        g->h3_blocks = h3_cap;
+        g->h2_blocks = h2_cap;
         g->grid = std::max<uint32_t>(1, g->h2_blocks + g->h3_blocks);
-    } else if (g->h2_blocks + g->h3_blocks + 1 > num_cus) {
+        return;
+    }
+    // A hash workgroup holds its CU until its role is done, so each role
+    // gets the fewest workgroups whose estimated time stays within a share
+    // of the encode's on the remaining CUs (3/4 for the slice lanes, 1/2 for
+    // the packet tasks, which can only follow the encode band by band).
+    // Costs in µs measured on MI355X at the batched mix: an encode unit's K
+    // step on one workgroup; a 64-byte block of a slice lane (MD5 and
+    // CRC-32) and of a packet lane; the fixed cost of a slice and of a task.
+    constexpr double kStepUs = 0.38, kSliceBlockUs = 1.31, kSliceUs = 36;
+    constexpr double kPacketBlockUs = 0.65, kTaskUs = 40;
+    const double slice_us = w.slices * kSliceUs + w.slice_blocks * kSliceBlockUs;
+    const double task_us = w.tasks * kTaskUs + w.task_blocks * kPacketBlockUs;
+    uint32_t h3 = std::min<uint32_t>(h3_cap, 1), h2 = std::min<uint32_t>(h2_cap, 1);
+    for (;;) {
+        const uint32_t e = num_cus > h2 + h3 + 1 ? num_cus - h2 - h3 : 1;
+        const double t_enc = w.enc_steps * kStepUs / e;
+        bool grew = false;
+        if (h2 < h2_cap && slice_us / (128.0 * h2) > 0.75 * t_enc) {
+            h2++;
+            grew = true;
+        }
+        if (h3 < h3_cap && task_us / (4.0 * h3) > 0.5 * t_enc) {
+            h3++;
+            grew = true;
+        }
+        if (!grew) break;
+    }
+    g->h3_blocks = h3;
+    g->h2_blocks = h2;
+    if (g->h2_blocks + g->h3_blocks + 1 > num_cus) {
         g->h3_blocks = std::min<uint32_t>(g->h3_blocks, 1);
         g->h2_blocks = std::min<uint32_t>(g->h2_blocks, 1);
     }
