@@ -313,6 +313,8 @@ int build_plan(Ctx *c, const std::vector<PlanItem> &items, bool use_graph, Batch
             w.enc_steps += (double)P.row_blocks * P.col_groups * P.nq;
             w.tasks += t;
             w.task_blocks += t * blocks;
+            w.last_tasks = t;
+            w.last_task_blocks = blocks;
         }
         plan_batch_group(c->num_cus, tasks, hashed, units, w, &g);
         B->sync_off.push_back(sync_words_total);

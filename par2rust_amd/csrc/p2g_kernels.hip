@@ -10,6 +10,7 @@
 //   k_recovery_md5   recovery packet hash, one lane per packet
 //                    (src/format.rs:40-44 via src/packet/recovery.rs:11-19)
 #include <algorithm>
+#include <cmath>
 #include <cstddef>
 #include <type_traits>
 
@@ -2079,10 +2080,14 @@ void plan_batch_group(uint32_t num_cus, uint32_t n_tasks, uint32_t n_slices, uin
     // A hash workgroup holds its CU until its role is done, so each role
     // gets the fewest workgroups whose estimated time stays within a share
     // of the encode's on the remaining CUs (3/4 for the slice lanes, 1/2 for
-    // the packet tasks, which can only follow the encode band by band).
-    // Costs in µs measured on MI355X at the batched mix: an encode unit's K
-    // step on one workgroup; a 64-byte block of a slice lane (MD5 and
-    // CRC-32) and of a packet lane; the fixed cost of a slice and of a task.
+    // the packet tasks, which can only follow the encode band by band).  A
+    // task follows its piece's bands to the piece's end, so a wave's next
+    // task starts only then: the tasks of the last piece get enough waves
+    // to run side by side unless the chains they would add after the
+    // encode are under 2% of it.  Costs in µs measured on MI355X at the
+    // batched mix: an encode unit's K step on one workgroup; a 64-byte block
+    // of a slice lane (MD5 and CRC-32) and of a packet lane; the fixed cost
+    // of a slice and of a task.
     constexpr double kStepUs = 0.38, kSliceBlockUs = 1.31, kSliceUs = 36;
     constexpr double kPacketBlockUs = 0.65, kTaskUs = 40;
     const double slice_us = w.slices * kSliceUs + w.slice_blocks * kSliceBlockUs;
@@ -2096,7 +2101,10 @@ void plan_batch_group(uint32_t num_cus, uint32_t n_tasks, uint32_t n_slices, uin
             h2++;
             grew = true;
         }
-        if (h3 < h3_cap && task_us / (4.0 * h3) > 0.5 * t_enc) {
+        const double last_turns = std::ceil(w.last_tasks / (4.0 * h3)) - 1;
+        const double last_chain_us = kTaskUs + w.last_task_blocks * kPacketBlockUs;
+        if (h3 < h3_cap && (task_us / (4.0 * h3) > 0.5 * t_enc ||
+                            last_turns * last_chain_us > 0.02 * t_enc)) {
             h3++;
             grew = true;
         }
