@@ -775,6 +775,15 @@ __device__ __forceinline__ void mf_unit_dma(const uint4 *s_desc, uint4 *A0, uint
 #define MF_STEP_DMA(GG, BUFN, BUFW)                                                           \
     do {                                                                                      \
         __builtin_amdgcn_sched_barrier(0);                                                    \
+        /* the step's first MFMA ahead of its wait and barrier: the pipe */                  \
+        /* runs it while the wave waits for the others */                                    \
+        {                                                                                     \
+            const uint4 A = Ac[0];                                                            \
+            const v8i_t av = v8i_t{(int)A.x, (int)A.y, (int)A.z, (int)A.w, 0, 0, 0, 0};       \
+            acc[0][0] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(av, Bc[0], acc[0][0], \
+                                                   4, 4, 0, kE8M0One, 0, kE8M0One);           \
+        }                                                                                     \
+        __builtin_amdgcn_sched_barrier(0);                                                    \
         __builtin_amdgcn_s_waitcnt(kWaitVmcnt4);                                              \
         asm volatile("" ::: "memory");                                                        \
         __builtin_amdgcn_s_barrier();                                                         \
@@ -788,16 +797,16 @@ __device__ __forceinline__ void mf_unit_dma(const uint4 *s_desc, uint4 *A0, uint
         _Pragma("unroll") for (int a = 0; a < kMfTiles; a++) {                                \
             const uint4 A = Ac[a];                                                            \
             const v8i_t av = v8i_t{(int)A.x, (int)A.y, (int)A.z, (int)A.w, 0, 0, 0, 0};       \
-            _Pragma("unroll") for (int cb = 0; cb < 2; cb++) acc[a][cb] =                     \
+            _Pragma("unroll") for (int cb = 0; cb < 2; cb++) if (a > 0 || cb > 0) acc[a][cb] = \
                 __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(av, Bc[cb], acc[a][cb], 4, 4, \
                                                                 0, kE8M0One, 0, kE8M0One);    \
         }                                                                                     \
-        /* Four MFMAs straight after the barrier (their operands are in */                   \
-        /* registers), so the matrix pipe runs while the DMA is issued; */                   \
+        /* Three more MFMAs straight after the barrier (their operands are */                \
+        /* in registers), so the matrix pipe runs while the DMA is issued; */                \
         /* then the DMA, the LDS reads two per MFMA, and the B build's VALU */               \
         /* between the last MFMAs (an MFMA issue holds the wave until the */                 \
         /* pipe takes it, so only work placed between MFMAs overlaps them). */               \
-        __builtin_amdgcn_sched_group_barrier(0x008, 4, 0);                                    \
+        __builtin_amdgcn_sched_group_barrier(0x008, 3, 0);                                    \
         __builtin_amdgcn_sched_group_barrier(0x010, 4, 0);                                    \
         _Pragma("unroll") for (int k = 0; k < 5; k++) {                                       \
             __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);                                \
