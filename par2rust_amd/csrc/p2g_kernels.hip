@@ -2110,7 +2110,7 @@ void plan_batch_group(uint32_t num_cus, uint32_t n_tasks, uint32_t n_slices, uin
             h2++;
             grew = true;
         }
-        const double last_turns = std::ceil(w.last_tasks / (4.0 * h3)) - 1;
+        const double last_turns = h3 ? std::ceil(w.last_tasks / (4.0 * h3)) - 1 : 0;
         const double last_chain_us = kTaskUs + w.last_task_blocks * kPacketBlockUs;
         if (h3 < h3_cap && (task_us / (4.0 * h3) > 0.5 * t_enc ||
                             last_turns * last_chain_us > 0.02 * t_enc)) {
