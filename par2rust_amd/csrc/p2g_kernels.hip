@@ -2079,7 +2079,7 @@ void plan_batch_group(uint32_t num_cus, uint32_t n_tasks, uint32_t n_slices, uin
     // four packet waves per packet workgroup, two MD5 + CRC-32 waves (128
     // lanes) per slice workgroup; the rest encode
     const uint32_t h3_cap = std::min<uint32_t>((n_tasks + 3) / 4, 8);
-    const uint32_t h2_cap = std::min<uint32_t>((n_slices + 127) / 128, 24);
+    const uint32_t h2_cap = std::min<uint32_t>((n_slices + 127) / 128, 64);
     if (n_units == 0) {
         g->h3_blocks = h3_cap;
         g->h2_blocks = h2_cap;
