@@ -551,7 +551,7 @@ int create_pass_impl(Ctx *c, const p2g_dev_file *files, uint32_t nfiles, uint64_
     if (variant == kEncodeMfma && !fits_pass && n && R) {
         // More slices than the pass's descriptor table holds (or fragments
         // above its budget): the batched pass runs the set as one job.  Its
-        // units take the slices in 2048-slice chunks and its pieces split
+        // units take the slices in 4096-slice chunks and its pieces split
         // the rows by a fragment budget, so the encode stays on the LDS-DMA
         // loop and the hashes stay in the same launches.
         p2g_batch_job job;

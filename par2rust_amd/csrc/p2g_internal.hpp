@@ -58,7 +58,7 @@ inline size_t mfma_afrag_bytes(uint32_t n, uint32_t R) {
 }
 hipError_t launch_mfma_afrag(const uint16_t *d_antilog, const uint16_t *d_logbase, uint32_t n,
                              uint32_t R, uint4 *d_afrag, hipStream_t stream);
-constexpr uint32_t kMfmaChunkInputs = 2048;  // inputs per K chunk of a batched unit
+constexpr uint32_t kMfmaChunkInputs = 4096;  // inputs per K chunk of a batched unit (64 KiB of LDS)
 hipError_t launch_encode_mfma(const SliceDesc *d_inputs, uint32_t n_inputs, const uint4 *d_afrag,
                               uint32_t R, uint32_t S, uint8_t *d_rec, int accumulate,
                               hipStream_t stream);

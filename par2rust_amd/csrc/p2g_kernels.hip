@@ -1891,8 +1891,14 @@ __device__ __forceinline__ void batch_packet_lane(const BatchArgs &a, const Batc
     if (store) store_digest(P.pkt_md5 + (size_t)r * 16, h);
 }
 
+// LDS of the batched kernel beside the A ring: one K chunk's descriptors
+// (the slice role keeps its CRC tables there instead).
+struct BatchLds {
+    uint4 desc[kMfmaChunkInputs];
+};
+
 __global__ __launch_bounds__(kMfThreads, 1) void k_create_batch(BatchArgs a) {
-    __shared__ MfLds lds;
+    __shared__ BatchLds lds;
     __shared__ uint4 s_a0[kMfRing], s_a1[kMfRing], s_a2[kMfRing], s_a3[kMfRing];
     const uint32_t blk = blockIdx.x, tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     if (tid == 0) {
@@ -1918,7 +1924,7 @@ __global__ __launch_bounds__(kMfThreads, 1) void k_create_batch(BatchArgs a) {
         // lds.desc); a lane walks the slice jobs on its own
         if (wave >= 2) return;
         const uint32_t hb = blk - a.h3_blocks;
-        static_assert(sizeof(MfLds::desc) >= 2 * 16 * 256 * 4, "two waves of 16 CRC tables");
+        static_assert(sizeof(BatchLds::desc) >= 2 * 16 * 256 * 4, "two waves of 16 CRC tables");
         uint32_t(*t)[256] = reinterpret_cast<uint32_t(*)[256]>(lds.desc) + 16 * wave;
         crc_tables_init_wave<16>(t);
         const uint32_t n_lanes = a.h2_blocks * 128;

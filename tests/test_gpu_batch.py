@@ -71,7 +71,7 @@ MIX = [
     ([300000], 65536, 70),           # two packet tasks
     ([4 * 600, 8], 4, 5),            # 602 slices, one-byte-ish partial
     ([5000], 64, 0),                 # no recovery blocks: checksums only
-    ([16 * 5000 + 9], 16, 20),       # 5001 inputs: three K chunks
+    ([16 * 5000 + 9], 16, 20),       # 5001 inputs: two K chunks
     ([150, 64, 3], 64, 1),
 ]
 

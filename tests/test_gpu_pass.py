@@ -106,7 +106,7 @@ def test_create_pass_many_bands(band_ctx, band_cols, lens, S, R):
     ([3000 * 4], 4, 6),                          # 3000 slices of 4 bytes
     ([2500 * 64 + 10, 700 * 64 + 3], 64, 40),    # two files, ragged tails, 3202 slices
     ([5000 * 100 + 52], 100, 17),                # S % 16 != 0, 5001 slices
-    ([9000 * 16], 16, 300),                      # 9000 slices: 5 chunks of 2048 per unit
+    ([9000 * 16], 16, 300),                      # 9000 slices: 3 chunks of 4096 per unit
 ])
 def test_create_pass_beyond_one_descriptor_table(ctx, lens, S, R):
     # more than 2048 slices: the entry point runs the set as a one-job batched launch
