@@ -305,6 +305,11 @@ int build_plan(Ctx *c, const std::vector<PlanItem> &items, bool use_graph, Batch
             hashed += cnt;
             w.slices += cnt;
             w.slice_blocks += cnt * blocks;
+            if (cnt && blocks > w.big_blocks) {
+                w.big_blocks = blocks;
+                w.big_slices = 0;
+            }
+            if (cnt && blocks == w.big_blocks) w.big_slices += cnt;
         }
         for (uint32_t p = sp.p_lo; p < sp.p_hi; p++) {
             const BatchPiece &P = pieces[p];

@@ -177,11 +177,13 @@ hipError_t launch_create_batch(const BatchGroupLaunch &g, const SliceDesc *d_des
                                const uint4 *d_tail_blocks, hipStream_t stream);
 // The work of one launch group, for sizing its roles: encode unit K steps
 // (units x steps of their job), the serial MD5 work of the slice lanes and
-// of the packet wave tasks, in 64-byte blocks and in slices and tasks, and
-// the tasks of the group's last piece with the blocks of one of them.
+// of the packet wave tasks, in 64-byte blocks and in slices and tasks, the
+// count of the largest slices, and the tasks of the group's last piece with
+// the blocks of one of them.
 struct BatchGroupWork {
     double enc_steps = 0;
     double slice_blocks = 0, slices = 0;
+    double big_slices = 0, big_blocks = 0;  // the slices of the largest size and their blocks
     double task_blocks = 0, tasks = 0;
     double last_tasks = 0, last_task_blocks = 0;
 };

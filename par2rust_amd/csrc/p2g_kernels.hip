@@ -2099,10 +2099,11 @@ void plan_batch_group(uint32_t num_cus, uint32_t n_tasks, uint32_t n_slices, uin
     // task follows its piece's bands to the piece's end, so a wave's next
     // task starts only then: the tasks of the last piece get enough waves
     // to run side by side unless the chains they would add after the
-    // encode are under 2% of it.  Costs in µs measured on MI355X at the
-    // batched mix: an encode unit's K step on one workgroup; a 64-byte block
-    // of a slice lane (MD5 and CRC-32) and of a packet lane; the fixed cost
-    // of a slice and of a task.
+    // encode are under 2% of it.  The slice lanes' estimate is the larger of
+    // their average work and the rounds of the largest slices.  Costs in µs
+    // measured on MI355X at the batched mix: an encode unit's K step on one
+    // workgroup; a 64-byte block of a slice lane (MD5 and CRC-32) and of a
+    // packet lane; the fixed cost of a slice and of a task.
     constexpr double kStepUs = 0.38, kSliceBlockUs = 1.31, kSliceUs = 36;
     constexpr double kPacketBlockUs = 0.65, kTaskUs = 40;
     const double slice_us = w.slices * kSliceUs + w.slice_blocks * kSliceBlockUs;
@@ -2112,7 +2113,11 @@ void plan_batch_group(uint32_t num_cus, uint32_t n_tasks, uint32_t n_slices, uin
         const uint32_t e = num_cus > h2 + h3 + 1 ? num_cus - h2 - h3 : 1;
         const double t_enc = w.enc_steps * kStepUs / e;
         bool grew = false;
-        if (h2 < h2_cap && slice_us / (128.0 * h2) > 0.75 * t_enc) {
+        // lanes take slices in stride, so the largest slices also need
+        // rounds: ceil(count / lanes) of them back to back on some lane
+        const double big_rounds = std::ceil(w.big_slices / (128.0 * std::max<uint32_t>(h2, 1)));
+        const double big_us = big_rounds > 1 ? big_rounds * (kSliceUs + w.big_blocks * kSliceBlockUs) : 0;
+        if (h2 < h2_cap && std::max(slice_us / (128.0 * h2), big_us) > 0.75 * t_enc) {
             h2++;
             grew = true;
         }
