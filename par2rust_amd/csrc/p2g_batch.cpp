@@ -315,11 +315,11 @@ int build_plan(Ctx *c, const std::vector<PlanItem> &items, bool use_graph, Batch
             const BatchPiece &P = pieces[p];
             const double blocks = (double)((36ull + P.S + 8) / 64 + 1);  // of a packet's hash
             const uint32_t t = (P.R + 63) / 64;
-            w.enc_steps += (double)P.row_blocks * P.col_groups * P.nq;
-            w.tasks += t;
-            w.task_blocks += t * blocks;
-            w.last_tasks = t;
-            w.last_task_blocks = blocks;
+            const double steps = (double)P.row_blocks * P.col_groups * P.nq;
+            w.enc_steps += steps;
+            w.piece_steps.push_back(steps);
+            w.piece_tasks.push_back(t);
+            w.piece_task_blocks.push_back(blocks);
         }
         plan_batch_group(c->num_cus, tasks, hashed, units, w, &g);
         B->sync_off.push_back(sync_words_total);

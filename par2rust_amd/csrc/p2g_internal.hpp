@@ -6,6 +6,7 @@
 
 #include <cstdint>
 #include <string>
+#include <vector>
 
 #include "../../include/par2gpu.h"
 
@@ -176,16 +177,15 @@ hipError_t launch_batch_afrag(const uint16_t *d_antilog, const uint16_t *d_logba
 hipError_t launch_create_batch(const BatchGroupLaunch &g, const SliceDesc *d_descs,
                                const uint4 *d_tail_blocks, hipStream_t stream);
 // The work of one launch group, for sizing its roles: encode unit K steps
-// (units x steps of their job), the serial MD5 work of the slice lanes and
-// of the packet wave tasks, in 64-byte blocks and in slices and tasks, the
-// count of the largest slices, and the tasks of the group's last piece with
-// the blocks of one of them.
+// (units x steps of their job), the serial MD5 work of the slice lanes in
+// 64-byte blocks and in slices, the count of the largest slices, and per
+// piece in launch order its unit-steps, its packet tasks and the blocks of
+// one task's hash.
 struct BatchGroupWork {
     double enc_steps = 0;
     double slice_blocks = 0, slices = 0;
     double big_slices = 0, big_blocks = 0;  // the slices of the largest size and their blocks
-    double task_blocks = 0, tasks = 0;
-    double last_tasks = 0, last_task_blocks = 0;
+    std::vector<double> piece_steps, piece_tasks, piece_task_blocks;
 };
 // Group sizing: MD5 workgroups for n_tasks packet tasks and n_slices slices,
 // as few as are estimated to finish within the encode (the rest encode).

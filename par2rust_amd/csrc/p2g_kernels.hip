@@ -2079,6 +2079,37 @@ hipError_t launch_batch_afrag(const uint16_t *d_antilog, const uint16_t *d_logba
     return hipGetLastError();
 }
 
+// Costs in µs measured on MI355X at the batched mix, for sizing the roles of
+// a group: an encode unit's K step on one workgroup; a 64-byte block of a
+// slice lane (MD5 and CRC-32) and of a packet lane; the fixed cost of a
+// slice and of a packet task.
+constexpr double kStepUs = 0.38, kSliceBlockUs = 1.31, kSliceUs = 36;
+constexpr double kPacketBlockUs = 0.65, kTaskUs = 40;
+
+// Estimated end (µs) of the packet role on 4 h3 waves when the encode takes
+// t_enc: pieces are encoded in order, each over its share of t_enc by its
+// unit-steps; task t runs on wave t mod 4 h3 after the wave's earlier tasks,
+// starts no earlier than its piece's encode and, following its bands, ends
+// no earlier than the piece's encode ends.
+static double packet_role_end(const BatchGroupWork &w, uint32_t h3, double t_enc) {
+    if (h3 == 0 || w.enc_steps <= 0) return 0;
+    std::vector<double> free_at(4 * h3, 0.0);
+    double done = 0, end = 0;
+    size_t t = 0;
+    for (size_t p = 0; p < w.piece_steps.size(); p++) {
+        const double start = t_enc * done / w.enc_steps;
+        done += w.piece_steps[p];
+        const double finish = t_enc * done / w.enc_steps;
+        const double chain = kTaskUs + w.piece_task_blocks[p] * kPacketBlockUs;
+        for (double k = 0; k < w.piece_tasks[p]; k++, t++) {
+            double &f = free_at[t % free_at.size()];
+            f = std::max(std::max(f, start) + chain, finish);
+            end = std::max(end, f);
+        }
+    }
+    return end;
+}
+
 void plan_batch_group(uint32_t num_cus, uint32_t n_tasks, uint32_t n_slices, uint32_t n_units,
                       const BatchGroupWork &w, BatchGroupLaunch *g) {
     g->grid = num_cus;
@@ -2093,40 +2124,36 @@ void plan_batch_group(uint32_t num_cus, uint32_t n_tasks, uint32_t n_slices, uin
         return;
     }
     // A hash workgroup holds its CU until its role is done, so each role
-    // gets the fewest workgroups whose estimated time stays within a share
-    // of the encode's on the remaining CUs (3/4 for the slice lanes, 1/2 for
-    // the packet tasks, which can only follow the encode band by band).  A
-    // task follows its piece's bands to the piece's end, so a wave's next
-    // task starts only then: the tasks of the last piece get enough waves
-    // to run side by side unless the chains they would add after the
-    // encode are under 2% of it.  The slice lanes' estimate is the larger of
-    // their average work and the rounds of the largest slices.  Costs in µs
-    // measured on MI355X at the batched mix: an encode unit's K step on one
-    // workgroup; a 64-byte block of a slice lane (MD5 and CRC-32) and of a
-    // packet lane; the fixed cost of a slice and of a task.
-    constexpr double kStepUs = 0.38, kSliceBlockUs = 1.31, kSliceUs = 36;
-    constexpr double kPacketBlockUs = 0.65, kTaskUs = 40;
+    // gets the fewest workgroups that keep it within the encode on the
+    // remaining CUs.  The slice lanes' estimate is the larger of their
+    // average work and the rounds of the largest slices (lanes take slices
+    // in stride), kept within 3/4 of the encode.  The packet tasks follow
+    // the encode band by band: workgroups are added while the simulated end
+    // of their role lies more than 2% past the encode and more workgroups
+    // would bring it closer.
     const double slice_us = w.slices * kSliceUs + w.slice_blocks * kSliceBlockUs;
-    const double task_us = w.tasks * kTaskUs + w.task_blocks * kPacketBlockUs;
     uint32_t h3 = std::min<uint32_t>(h3_cap, 1), h2 = std::min<uint32_t>(h2_cap, 1);
     for (;;) {
         const uint32_t e = num_cus > h2 + h3 + 1 ? num_cus - h2 - h3 : 1;
         const double t_enc = w.enc_steps * kStepUs / e;
         bool grew = false;
-        // lanes take slices in stride, so the largest slices also need
-        // rounds: ceil(count / lanes) of them back to back on some lane
         const double big_rounds = std::ceil(w.big_slices / (128.0 * std::max<uint32_t>(h2, 1)));
         const double big_us = big_rounds > 1 ? big_rounds * (kSliceUs + w.big_blocks * kSliceBlockUs) : 0;
         if (h2 < h2_cap && std::max(slice_us / (128.0 * h2), big_us) > 0.75 * t_enc) {
             h2++;
             grew = true;
         }
-        const double last_turns = h3 ? std::ceil(w.last_tasks / (4.0 * h3)) - 1 : 0;
-        const double last_chain_us = kTaskUs + w.last_task_blocks * kPacketBlockUs;
-        if (h3 < h3_cap && (task_us / (4.0 * h3) > 0.5 * t_enc ||
-                            last_turns * last_chain_us > 0.02 * t_enc)) {
-            h3++;
-            grew = true;
+        const double end = packet_role_end(w, h3, t_enc);
+        if (h3 < h3_cap && end > 1.02 * t_enc) {
+            // the fewest more workgroups that bring the end closer (waves
+            // that still hold two tasks of one piece gain nothing)
+            for (uint32_t h = h3 + 1; h <= h3_cap; h++) {
+                if (packet_role_end(w, h, t_enc) < end - 0.01 * t_enc) {
+                    h3 = h;
+                    grew = true;
+                    break;
+                }
+            }
         }
         if (!grew) break;
     }
