@@ -326,6 +326,67 @@ def test_scan_slice_count_and_many_slices():
         assert crcs[i] == zlib.crc32(data[16 * i:16 * i + 16])
 
 
+def _slices_padded(data: bytes, S: int):
+    """The zero-padded S-byte slices of data (src/source.rs:168-183)."""
+    for o in range(0, len(data), S):
+        yield data[o:o + S].ljust(S, b"\0")
+
+
+@pytest.mark.parametrize("S", [4, 12, 100, 4096, 65536, 786432, 1048576])
+@pytest.mark.parametrize("threads", [1, 3, 16])
+def test_scan_mt_equals_hashlib_and_zlib(S, threads):
+    # The multi-threaded scan (its own slicing-by-8 CRC-32) against library
+    # MD5 and CRC-32 of each zero-padded slice, src/source.rs:168-183
+    # (md5_impl::digest and crc32fast::hash of the padded slice buffer).
+    import numpy as np
+
+    rng = np.random.default_rng(S * 31 + threads)
+    # whole slices plus a partial tail that is not a multiple of 4, 16 or 64
+    n_whole = max(1, min(40, (3 << 20) // S))
+    tail = max(1, (S * 5) // 7) if S > 4 else 3
+    data = rng.integers(0, 256, n_whole * S + tail, dtype=np.uint8).tobytes()
+    md5s, crcs = O.slice_checksums_mt(data, S, threads)
+    want = list(_slices_padded(data, S))
+    assert len(md5s) == len(crcs) == len(want) == n_whole + 1
+    for i, sl in enumerate(want):
+        assert md5s[i] == hashlib.md5(sl).digest(), (S, threads, i)
+        assert crcs[i] == zlib.crc32(sl), (S, threads, i)
+    # the serial scan agrees too
+    assert O.slice_checksums(data, S) == (md5s, crcs)
+
+
+def test_scan_mt_edge_lengths():
+    # single short slice, exact multiples, one byte over
+    rnd = random.Random(12)
+    for S, L in ((4, 1), (4, 4), (8, 9), (4096, 4096), (4096, 4097), (65536, 3), (786432, 786431)):
+        data = bytes(rnd.randrange(256) for _ in range(L))
+        md5s, crcs = O.slice_checksums_mt(data, S, 3)
+        want = list(_slices_padded(data, S))
+        assert md5s == [hashlib.md5(s).digest() for s in want], (S, L)
+        assert crcs == [zlib.crc32(s) for s in want], (S, L)
+
+
+@pytest.mark.parametrize("S,first", [(4, 0), (100, 65530), (4096, 65533), (65540, 7), (524288, 65400)])
+@pytest.mark.parametrize("threads", [1, 3, 16])
+def test_recovery_md5_mt_equals_hashlib(S, first, threads):
+    # build_packet's hash, src/format.rs:40-44, over the recovery body of
+    # src/packet/recovery.rs:11-19: MD5(set_id || "PAR 2.0\0RecvSlic" ||
+    # le32(exponent) || block).  S % 64 != 0 for 4, 100 and 65540.
+    import numpy as np
+
+    count = min(9, 65536 - first)
+    rng = np.random.default_rng(S + first + threads)
+    rec = rng.integers(0, 256, count * S, dtype=np.uint8).tobytes()
+    sid = rng.integers(0, 256, 16, dtype=np.uint8).tobytes()
+    got = O.recovery_md5_mt(rec, count, S, first, sid, threads)
+    for r in range(count):
+        msg = sid + b"PAR 2.0\0RecvSlic" + struct.pack("<I", first + r) + rec[r * S:(r + 1) * S]
+        assert got[r] == hashlib.md5(msg).digest(), (S, first, threads, r)
+        # and the whole packet the serial oracle builds carries the same hash
+        if S <= 4096:
+            assert O.recovery_packet(sid, first + r, rec[r * S:(r + 1) * S])[16:32] == got[r]
+
+
 def test_file_id_derivation():
     # src/source.rs:414-426
     h16 = b"\x11" * 16
