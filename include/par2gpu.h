@@ -161,6 +161,20 @@ int p2g_dev_create_pass(p2g_ctx *ctx, const p2g_dev_file *files, uint32_t nfiles
  * a batched launch, or when none has run). */
 int p2g_pass_timing(p2g_ctx *ctx, double *ms_out);
 
+/* Diagnostics, no reference counterpart; host only (needs no device).  The
+ * role of every workgroup of the persistent launch the planner makes for one
+ * set of `slices` slices of slice_size bytes and recovery_count blocks, on
+ * a device of num_cus CUs that holds max_resident workgroups of the kernel
+ * at once: kind 0 the fused pass of p2g_dev_create_pass, kind 1 a one-job
+ * batched plan.  roles_out[b] (up to cap entries) = 0 packet hash, 1 slice
+ * hash, 2 encode; grid_out = workgroups of the launch.  The encode
+ * workgroups come first, so that a partly resident grid runs its producers
+ * before the roles that wait on them.  P2G_ERR_INVALID_ARGUMENT for a shape
+ * the planner does not take (kind 0: a set the fused pass does not hold). */
+int p2g_plan_roles(int kind, uint32_t num_cus, uint32_t max_resident, uint32_t slices,
+                   uint32_t recovery_count, uint64_t slice_size, uint32_t *roles_out, uint32_t cap,
+                   uint32_t *grid_out);
+
 /* ---------------------------------------------------------------------
  * Batched create (no reference counterpart: the reference runs one
  * run_create at a time, src/creator.rs:188-321).  Many independent sets,
@@ -298,14 +312,20 @@ int p2g_synchronize(p2g_ctx *ctx);
 enum { P2G_SCHEME_SINGLE = 0, P2G_SCHEME_EXPONENTIAL = 1, P2G_SCHEME_UNIFORM = 2,
        P2G_SCHEME_EXPLICIT = 3 };
 
-/* VolumeScheme, src/creator.rs:110-139.  Limited wraps the inner scheme. */
+/* VolumeScheme, src/creator.rs:110-139.  `kind` is the base scheme; with
+ * limited != 0 it is wrapped in Limited { max_blocks_per_volume = limit_cap,
+ * inner }, and that in n_outer_caps further Limited wrappers whose caps are
+ * outer_caps[0] (the next one out) .. outer_caps[n_outer_caps - 1] (the
+ * outermost), as Limited nests through its boxed inner scheme. */
 typedef struct {
-    int kind;                        /* P2G_SCHEME_* */
+    int kind;                        /* P2G_SCHEME_*: the innermost scheme */
     uint32_t uniform_count;          /* Uniform { count } */
     int limited;                     /* Limited { max_blocks_per_volume, inner } */
     uint32_t limit_cap;              /* max_blocks_per_volume */
     const uint32_t *explicit_sizes;  /* Explicit(sizes) */
     uint32_t n_explicit;
+    const uint32_t *outer_caps;      /* further Limited wrappers, innermost first */
+    uint32_t n_outer_caps;           /* 0: none (needs limited != 0 otherwise) */
 } p2g_volume_scheme;
 
 /* Writes the index file `output` and its volume files.  names_out receives

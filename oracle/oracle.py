@@ -11,7 +11,9 @@ import os
 from typing import List, Sequence, Tuple
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libpar2_oracle.so")
+# PAR2_ORACLE_LIB: another build of the same sources (the sanitizer test
+# loads an AddressSanitizer / UBSan build through it)
+LIB_PATH = os.environ.get("PAR2_ORACLE_LIB") or os.path.join(HERE, "libpar2_oracle.so")
 REF_MD5_PATH = os.path.join(HERE, "_ref", "libmd5ref.so")
 
 SCHEME_SINGLE, SCHEME_EXPONENTIAL, SCHEME_UNIFORM, SCHEME_EXPLICIT = 0, 1, 2, 3
@@ -30,6 +32,8 @@ class _Scheme(ctypes.Structure):
         ("limit_cap", ctypes.c_uint32),
         ("explicit_sizes", ctypes.POINTER(ctypes.c_uint32)),
         ("n_explicit", ctypes.c_uint32),
+        ("outer_caps", ctypes.POINTER(ctypes.c_uint32)),
+        ("n_outer_caps", ctypes.c_uint32),
     ]
 
 
@@ -235,12 +239,14 @@ def creator_packet(set_id: bytes) -> bytes:
 
 
 def _scheme(kind: int, uniform_count: int = 0, limited: bool = False, limit_cap: int = 0,
-            explicit: Sequence[int] = ()):
+            explicit: Sequence[int] = (), outer_caps: Sequence[int] = ()):
     arr = (ctypes.c_uint32 * max(1, len(explicit)))(*explicit)
+    caps = (ctypes.c_uint32 * max(1, len(outer_caps)))(*outer_caps)
+    u32p = ctypes.POINTER(ctypes.c_uint32)
     s = _Scheme(kind, uniform_count, int(limited), limit_cap,
-                ctypes.cast(arr, ctypes.POINTER(ctypes.c_uint32)) if explicit else None,
-                len(explicit))
-    return s, arr
+                ctypes.cast(arr, u32p) if explicit else None, len(explicit),
+                ctypes.cast(caps, u32p) if outer_caps else None, len(outer_caps))
+    return s, (arr, caps)
 
 
 class SchemeError(ValueError):
@@ -248,8 +254,9 @@ class SchemeError(ValueError):
 
 
 def resolve_volume_sizes(total: int, kind: int, uniform_count: int = 0, limited: bool = False,
-                         limit_cap: int = 0, explicit: Sequence[int] = ()) -> List[int]:
-    s, keep = _scheme(kind, uniform_count, limited, limit_cap, explicit)
+                         limit_cap: int = 0, explicit: Sequence[int] = (),
+                         outer_caps: Sequence[int] = ()) -> List[int]:
+    s, keep = _scheme(kind, uniform_count, limited, limit_cap, explicit, outer_caps)
     cap = 70000
     out = (ctypes.c_uint32 * cap)()
     n = L.or_resolve_volume_sizes(ctypes.byref(s), total, out, cap)
@@ -266,8 +273,9 @@ class OracleError(RuntimeError):
 
 def run_create(output: str, S: int, R: int, inputs: Sequence[Tuple[str, bytes]],
                kind: int = SCHEME_SINGLE, uniform_count: int = 0, limited: bool = False,
-               limit_cap: int = 0, explicit: Sequence[int] = ()) -> List[str]:
-    s, keep = _scheme(kind, uniform_count, limited, limit_cap, explicit)
+               limit_cap: int = 0, explicit: Sequence[int] = (),
+               outer_caps: Sequence[int] = ()) -> List[str]:
+    s, keep = _scheme(kind, uniform_count, limited, limit_cap, explicit, outer_caps)
     n = len(inputs)
     paths = (ctypes.c_char_p * max(1, n))(*[os.fsencode(p) for p, _ in inputs])
     names = (ctypes.c_char_p * max(1, n))(*[bytes(nm) for _, nm in inputs])

@@ -466,27 +466,39 @@ static int resolve(const or_scheme *s, uint32_t total, u32vec *out) {
     return OR_OK;
 }
 
-/* resolve_volume_sizes, src/creator.rs:325-362 with Limited (:333-344)
- * wrapping the inner scheme and cap_volume_sizes (:414-431). */
+/* cap_volume_sizes, src/creator.rs:414-431 */
+static void cap_volume_sizes(const u32vec *in, uint32_t cap, u32vec *out) {
+    for (uint32_t i = 0; i < in->n; i++) {
+        uint32_t n = in->v[i];
+        if (n <= cap) { vpush(out, n); continue; }
+        uint32_t rem = n;
+        while (rem > cap) { vpush(out, cap); rem -= cap; }
+        if (rem > 0) vpush(out, rem);
+    }
+}
+
+/* resolve_volume_sizes, src/creator.rs:325-362, for the scheme under `k`
+ * Limited wrappers (wrapper k - 1 is the outermost of them; wrapper 0 has
+ * limit_cap, wrapper j >= 1 outer_caps[j - 1]).  As in the reference's
+ * recursion through `inner: Box<VolumeScheme>` (:330-344), a wrapper checks
+ * its cap before its inner scheme is resolved, and caps the inner sizes. */
+static int resolve_wrapped(const or_scheme *s, uint32_t k, uint32_t total, u32vec *out) {
+    if (k == 0) return resolve(s, total, out);
+    uint32_t cap = k == 1 ? s->limit_cap : s->outer_caps[k - 2];
+    if (cap == 0) return OR_ERR_INVALID_VOLUME_SCHEME;
+    u32vec inner = {0};
+    int rc = resolve_wrapped(s, k - 1, total, &inner);
+    if (rc == OR_OK) cap_volume_sizes(&inner, cap, out);
+    free(inner.v);
+    return rc;
+}
+
 int or_resolve_volume_sizes(const or_scheme *s, uint32_t total, uint32_t *sizes_out,
                             uint32_t cap) {
-    if (s->limited && s->limit_cap == 0) return -OR_ERR_INVALID_VOLUME_SCHEME;
-    u32vec inner = {0}, out = {0};
-    int rc = resolve(s, total, &inner);
-    if (rc != OR_OK) { free(inner.v); return -rc; }
-    if (s->limited) {
-        for (uint32_t i = 0; i < inner.n; i++) {
-            uint32_t n = inner.v[i];
-            if (n <= s->limit_cap) { vpush(&out, n); continue; }
-            uint32_t rem = n;
-            while (rem > s->limit_cap) { vpush(&out, s->limit_cap); rem -= s->limit_cap; }
-            if (rem > 0) vpush(&out, rem);
-        }
-    } else {
-        out = inner;
-        inner.v = NULL;
-    }
-    free(inner.v);
+    if (s->n_outer_caps && !s->limited) return -OR_ERR_INVALID_VOLUME_SCHEME;
+    u32vec out = {0};
+    int rc = resolve_wrapped(s, s->limited ? 1 + s->n_outer_caps : 0, total, &out);
+    if (rc != OR_OK) { free(out.v); return -rc; }
     uint32_t n = out.n;
     for (uint32_t i = 0; i < n && i < cap; i++) sizes_out[i] = out.v[i];
     free(out.v);

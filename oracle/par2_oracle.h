@@ -123,6 +123,8 @@ typedef struct {
     uint32_t limit_cap;
     const uint32_t *explicit_sizes;
     uint32_t n_explicit;
+    const uint32_t *outer_caps; /* further Limited wrappers around that one, innermost first */
+    uint32_t n_outer_caps;
 } or_scheme;
 /* Returns the number of volumes written into sizes_out (capacity cap), or a
  * negative OR_ERR_* code. */

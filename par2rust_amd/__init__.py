@@ -47,6 +47,7 @@ EXPORTED_SYMBOLS = [
     "p2g_server_open", "p2g_server_submit", "p2g_server_poll", "p2g_server_result",
     "p2g_server_release", "p2g_server_stats", "p2g_server_close",
     "p2g_tune_bucket", "p2g_autotune", "p2g_tune_get", "p2g_tune_save", "p2g_tune_load",
+    "p2g_plan_roles",
 ]
 
 VARIANT_MFMA, VARIANT_BITSLICE = 0, 1
@@ -101,6 +102,8 @@ class _Scheme(ctypes.Structure):
         ("limit_cap", ctypes.c_uint32),
         ("explicit_sizes", ctypes.POINTER(ctypes.c_uint32)),
         ("n_explicit", ctypes.c_uint32),
+        ("outer_caps", ctypes.POINTER(ctypes.c_uint32)),
+        ("n_outer_caps", ctypes.c_uint32),
     ]
 
 
@@ -151,6 +154,9 @@ def _load() -> ctypes.CDLL:
                                             ctypes.POINTER(ctypes.c_uint64)]),
         "p2g_server_close": (None, [P]),
         "p2g_tune_bucket": (ctypes.c_uint32, [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint64]),
+        "p2g_plan_roles": (ctypes.c_int, [ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32,
+                                          ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64, u32p,
+                                          ctypes.c_uint32, u32p]),
         "p2g_autotune": (ctypes.c_int, [P, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64,
                                         ctypes.POINTER(ctypes.c_int),
                                         ctypes.POINTER(ctypes.c_uint32),
@@ -178,13 +184,16 @@ lib = _load()
 
 @dataclass
 class VolumeScheme:
-    """VolumeScheme, src/creator.rs:110-139 (Limited wraps the inner kind)."""
+    """VolumeScheme, src/creator.rs:110-139.  Limited wraps the inner kind
+    (limit_cap), and further Limited wrappers nest around it (outer_caps,
+    innermost first)."""
 
     kind: int = SCHEME_SINGLE
     uniform_count: int = 0
     limited: bool = False
     limit_cap: int = 0
     explicit_sizes: List[int] = field(default_factory=list)
+    outer_caps: List[int] = field(default_factory=list)
 
     @staticmethod
     def single() -> "VolumeScheme":
@@ -203,14 +212,21 @@ class VolumeScheme:
         return VolumeScheme(SCHEME_EXPLICIT, explicit_sizes=list(sizes))
 
     def capped(self, cap: int) -> "VolumeScheme":
+        """Limited { max_blocks_per_volume: cap, inner: self }."""
+        if self.limited:
+            return VolumeScheme(self.kind, self.uniform_count, True, self.limit_cap,
+                                list(self.explicit_sizes), self.outer_caps + [cap])
         return VolumeScheme(self.kind, self.uniform_count, True, cap, list(self.explicit_sizes))
 
     def _c(self):
         arr = (ctypes.c_uint32 * max(1, len(self.explicit_sizes)))(*self.explicit_sizes)
+        caps = (ctypes.c_uint32 * max(1, len(self.outer_caps)))(*self.outer_caps)
+        u32p = ctypes.POINTER(ctypes.c_uint32)
         s = _Scheme(self.kind, self.uniform_count, int(self.limited), self.limit_cap,
-                    ctypes.cast(arr, ctypes.POINTER(ctypes.c_uint32)) if self.explicit_sizes
-                    else None, len(self.explicit_sizes))
-        return s, arr
+                    ctypes.cast(arr, u32p) if self.explicit_sizes else None,
+                    len(self.explicit_sizes),
+                    ctypes.cast(caps, u32p) if self.outer_caps else None, len(self.outer_caps))
+        return s, (arr, caps)
 
 
 @dataclass
@@ -606,6 +622,23 @@ class Context:
 def tune_bucket(slices: int, recovery_count: int, slice_size: int) -> int:
     """The autotuner's bucket id of a shape (p2g_tune_bucket)."""
     return lib.p2g_tune_bucket(slices, recovery_count, slice_size)
+
+
+ROLE_PACKET, ROLE_SLICE, ROLE_ENCODE = 0, 1, 2
+
+
+def plan_roles(kind: str, num_cus: int, max_resident: int, slices: int, recovery_count: int,
+               slice_size: int) -> List[int]:
+    """Role of each workgroup of the persistent launch the planner makes
+    (p2g_plan_roles; host only): kind "pass" or "batch"."""
+    cap = 4096
+    roles = (ctypes.c_uint32 * cap)()
+    grid = ctypes.c_uint32()
+    rc = lib.p2g_plan_roles({"pass": 0, "batch": 1}[kind], num_cus, max_resident, slices,
+                            recovery_count, slice_size, roles, cap, ctypes.byref(grid))
+    if rc != P2G_OK:
+        raise Par2Error(rc, "p2g_plan_roles")
+    return list(roles)[:grid.value]
 
 
 def version() -> str:

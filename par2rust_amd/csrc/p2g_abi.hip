@@ -112,6 +112,13 @@ int Ctx::open(int dev) {
     e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     if (e != hipSuccess || cus <= 0) return fail(P2G_ERR_DEVICE, "compute unit count", e);
     num_cus = (uint32_t)cus;
+    {
+        const int po = pass_occupancy(), bo = batch_occupancy();
+        if (po <= 0 || bo <= 0)
+            return fail(P2G_ERR_DEVICE, "occupancy of the persistent kernels could not be computed");
+        pass_resident = (uint32_t)po * num_cus;
+        batch_resident = (uint32_t)bo * num_cus;
+    }
     if (const char *v = getenv("P2G_BAND_COLS")) band_cols = (uint32_t)strtoul(v, nullptr, 10);
     if (const char *v = getenv("P2G_AUTOTUNE")) autotune = atoi(v) != 0;
     if (const char *v = getenv("P2G_TUNE_CACHE")) {
@@ -133,6 +140,9 @@ int Ctx::open(int dev) {
         return fail(P2G_ERR_OUT_OF_MEMORY, "base logs");
     e = hipMemcpy(d_base_logs.ptr, base_logs.data(), base_logs.size() * 2, hipMemcpyHostToDevice);
     if (e != hipSuccess) return fail(P2G_ERR_DEVICE, "upload base logs", e);
+    if (d_err.reserve(256) != P2G_OK) return fail(P2G_ERR_OUT_OF_MEMORY, "error word");
+    e = hipMemset(d_err.ptr, 0, 256);
+    if (e != hipSuccess) return fail(P2G_ERR_DEVICE, "clear error word", e);
     return P2G_OK;
 }
 
@@ -148,6 +158,7 @@ Ctx::~Ctx() {
     d_crc.release();
     d_stage.release();
     d_sync.release();
+    d_err.release();
     d_partials.release();
     d_xs_jumps.release();
     for (hipEvent_t ev : scratch_done)
@@ -390,10 +401,12 @@ int p2g_synchronize(p2g_ctx *ctx) {
         c->pending_batches.clear();
         if (brc != P2G_OK) return brc;
         if (c->pass_pending) {
+            // every pass queued since the last wait ORs its errors into the
+            // sticky word; it is cleared only here, after all of them ended
             c->pass_pending = false;
             uint32_t flags = 0;
-            e = hipMemcpy(&flags, static_cast<uint32_t *>(c->d_sync.ptr) + kPassFlags, 4,
-                          hipMemcpyDeviceToHost);
+            e = hipMemcpy(&flags, c->d_err.ptr, 4, hipMemcpyDeviceToHost);
+            if (e == hipSuccess && flags) e = hipMemset(c->d_err.ptr, 0, 4);
             if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "read create-pass flags", e);
             if (flags & kPassErrWait)
                 return c->fail(P2G_ERR_DEVICE,
@@ -545,7 +558,7 @@ int create_pass_impl(Ctx *c, const p2g_dev_file *files, uint32_t nfiles, uint64_
     const uint32_t n = (uint32_t)descs.size();
     const uint32_t S = (uint32_t)slice_size;
     PassPlan plan;
-    const bool fits_pass = plan_create_pass(n, R, S, c->num_cus, band_cols, &plan) &&
+    const bool fits_pass = plan_create_pass(n, R, S, c->num_cus, c->pass_resident, band_cols, &plan) &&
                            mfma_afrag_bytes(n, R) <= kPassAfragBudget;
     c->last_pass_fused = variant == kEncodeMfma && fits_pass;
     if (variant == kEncodeMfma && !fits_pass && n && R) {
@@ -625,7 +638,8 @@ int create_pass_impl(Ctx *c, const p2g_dev_file *files, uint32_t nfiles, uint64_
                               static_cast<const uint16_t *>(c->d_base_logs.ptr), n, R, afrag, st);
     if (e == hipSuccess)
         e = launch_create_pass(plan, dd, tb, n, afrag, R, S, rec, slice_md5_out, slice_crc_out, set_id,
-                               pkt_md5_out, static_cast<uint32_t *>(c->d_sync.ptr), all_full, st);
+                               pkt_md5_out, static_cast<uint32_t *>(c->d_sync.ptr),
+                               static_cast<uint32_t *>(c->d_err.ptr), all_full, st);
     if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "create pass kernels", e);
     c->pass_pending = true;
     return c->scratch_release(Ctx::kScratchEnc, st);

@@ -45,7 +45,7 @@ BatchPlan::~BatchPlan() {
     if (cap) hipStreamDestroy(cap);
     if (done) hipEventDestroy(done);
     for (DevBuf *b : {&d_descs, &d_tb, &d_staging, &d_parts, &d_pieces, &d_sjobs, &d_sync, &d_done,
-                      &own_afrag})
+                      &d_err, &own_afrag})
         b->release();
 }
 
@@ -109,6 +109,29 @@ int check_batch_job(Ctx *c, const p2g_batch_job &J, std::vector<SliceDesc> &desc
     if (descs.size() > kMaxInputs)
         return c->fail(P2G_ERR_TOO_MANY_INPUT_BLOCKS, "more than 32768 input slices");
     return P2G_OK;
+}
+
+// The sizing work of cnt slices of S bytes (one lane's MD5 + CRC-32 each).
+static void add_slice_work(BatchGroupWork &w, uint32_t cnt, uint32_t S) {
+    const double blocks = (double)((S + 8) / 64 + 1);  // MD5 blocks of a slice
+    w.slices += cnt;
+    w.slice_blocks += cnt * blocks;
+    if (cnt && blocks > w.big_blocks) {
+        w.big_blocks = blocks;
+        w.big_slices = 0;
+    }
+    if (cnt && blocks == w.big_blocks) w.big_slices += cnt;
+}
+
+// The sizing work of one piece: its encode unit-steps and packet tasks.
+static void add_piece_work(BatchGroupWork &w, const BatchPiece &P) {
+    const double blocks = (double)((36ull + P.S + 8) / 64 + 1);  // of a packet's hash
+    const uint32_t t = (P.R + 63) / 64;
+    const double steps = (double)P.row_blocks * P.col_groups * P.nq;
+    w.enc_steps += steps;
+    w.piece_steps.push_back(steps);
+    w.piece_tasks.push_back(t);
+    w.piece_task_blocks.push_back(blocks);
 }
 
 // Plans the items into B (its buffers are reused and grown; the previous
@@ -301,27 +324,11 @@ int build_plan(Ctx *c, const std::vector<PlanItem> &items, bool use_graph, Batch
         BatchGroupWork w;
         for (uint32_t q = sp.s_lo; q < sp.s_hi; q++) {
             const uint32_t cnt = sjobs[q].slice_end - sjobs[q].slice0;
-            const double blocks = (double)((sjobs[q].S + 8) / 64 + 1);  // MD5 blocks of a slice
             hashed += cnt;
-            w.slices += cnt;
-            w.slice_blocks += cnt * blocks;
-            if (cnt && blocks > w.big_blocks) {
-                w.big_blocks = blocks;
-                w.big_slices = 0;
-            }
-            if (cnt && blocks == w.big_blocks) w.big_slices += cnt;
+            add_slice_work(w, cnt, sjobs[q].S);
         }
-        for (uint32_t p = sp.p_lo; p < sp.p_hi; p++) {
-            const BatchPiece &P = pieces[p];
-            const double blocks = (double)((36ull + P.S + 8) / 64 + 1);  // of a packet's hash
-            const uint32_t t = (P.R + 63) / 64;
-            const double steps = (double)P.row_blocks * P.col_groups * P.nq;
-            w.enc_steps += steps;
-            w.piece_steps.push_back(steps);
-            w.piece_tasks.push_back(t);
-            w.piece_task_blocks.push_back(blocks);
-        }
-        plan_batch_group(c->num_cus, tasks, hashed, units, w, &g);
+        for (uint32_t p = sp.p_lo; p < sp.p_hi; p++) add_piece_work(w, pieces[p]);
+        plan_batch_group(c->num_cus, c->batch_resident, tasks, hashed, units, w, &g);
         B->sync_off.push_back(sync_words_total);
         sync_words_total += (uint32_t)align_up(starts, 64);
     }
@@ -330,13 +337,19 @@ int build_plan(Ctx *c, const std::vector<PlanItem> &items, bool use_graph, Batch
     if (rc == P2G_OK) rc = upload(c, B->d_pieces, pieces, "batch pieces");
     if (rc == P2G_OK) rc = upload(c, B->d_sjobs, sjobs, "batch slice jobs");
     if (rc != P2G_OK) return rc;
+    const bool new_err = B->d_err.ptr == nullptr;
     if (B->d_sync.reserve((size_t)sync_words_total * 4) != P2G_OK ||
-        B->d_done.reserve((size_t)(nitems + 1) * 8) != P2G_OK)
+        B->d_done.reserve((size_t)(nitems + 1) * 8) != P2G_OK || B->d_err.reserve(256) != P2G_OK)
         return c->fail(P2G_ERR_OUT_OF_MEMORY, "batch sync words");
+    if (new_err) {
+        hipError_t ez = hipMemset(B->d_err.ptr, 0, 256);
+        if (ez != hipSuccess) return c->fail(P2G_ERR_DEVICE, "clear batch error word", ez);
+    }
     for (size_t k = 0; k < gl.size(); k++) {
         gl[k].pieces = static_cast<const BatchPiece *>(B->d_pieces.ptr) + spans[k].p_lo;
         gl[k].sjobs = static_cast<const BatchSlices *>(B->d_sjobs.ptr) + spans[k].s_lo;
         gl[k].sync = static_cast<uint32_t *>(B->d_sync.ptr) + B->sync_off[k];
+        gl[k].err = static_cast<uint32_t *>(B->d_err.ptr);
         gl[k].job_done = static_cast<unsigned long long *>(B->d_done.ptr);
     }
     B->groups = gl;
@@ -401,14 +414,11 @@ int wait_plan(BatchPlan *B) {
     hipError_t e = hipEventSynchronize(B->done);
     if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "batch wait", e);
     B->launched = false;
+    // every launch since the last wait ORs its errors into the sticky word
     uint32_t bad = 0;
-    for (size_t k = 0; k < B->groups.size(); k++) {
-        uint32_t flags = 0;
-        e = hipMemcpy(&flags, static_cast<uint32_t *>(B->d_sync.ptr) + B->sync_off[k] + kPassFlags,
-                      4, hipMemcpyDeviceToHost);
-        if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "read batch flags", e);
-        bad |= flags;
-    }
+    e = hipMemcpy(&bad, B->d_err.ptr, 4, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && bad) e = hipMemset(B->d_err.ptr, 0, 4);
+    if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "read batch flags", e);
     if (getenv("P2G_BATCH_VERBOSE")) {
         // per group: first start and last end of each role, ms from the batch start
         std::vector<unsigned long long> t0(1);
@@ -511,6 +521,48 @@ int p2g_batch_job_ms(p2g_batch *b, double *ms_out) {
 }
 
 void p2g_batch_free(p2g_batch *b) { delete reinterpret_cast<BatchPlan *>(b); }
+
+int p2g_plan_roles(int kind, uint32_t num_cus, uint32_t max_resident, uint32_t slices,
+                   uint32_t recovery_count, uint64_t slice_size, uint32_t *roles_out, uint32_t cap,
+                   uint32_t *grid_out) {
+    return guarded(nullptr, [&]() -> int {
+        if (!grid_out || (cap && !roles_out) || slices == 0 || slices > kMaxInputs ||
+            recovery_count == 0 || recovery_count > P2G_MAX_RECOVERY_BLOCKS || slice_size == 0 ||
+            slice_size % 4 || slice_size > kMaxSliceBytes || num_cus == 0 || max_resident == 0)
+            return P2G_ERR_INVALID_ARGUMENT;
+        const uint32_t S = (uint32_t)slice_size, R = recovery_count, n = slices;
+        uint32_t grid, h2, h3;
+        if (kind == 0) {
+            PassPlan p;
+            if (!plan_create_pass(n, R, S, num_cus, max_resident, 0, &p)) return P2G_ERR_INVALID_ARGUMENT;
+            grid = p.grid;
+            h2 = p.h2_blocks;
+            h3 = p.h3_blocks;
+        } else if (kind == 1) {
+            // a one-job plan with one piece, as build_plan sizes it
+            BatchPiece P{};
+            P.n = n;
+            P.nq = mfma_steps(n);
+            P.R = R;
+            P.S = S;
+            P.row_blocks = (R + 15) / 16;
+            P.col_groups = (S + 511) / 512;
+            BatchGroupWork w;
+            add_slice_work(w, n, S);
+            add_piece_work(w, P);
+            BatchGroupLaunch g{};
+            plan_batch_group(num_cus, max_resident, (R + 63) / 64, n, P.row_blocks * P.col_groups, w, &g);
+            grid = g.grid;
+            h2 = g.h2_blocks;
+            h3 = g.h3_blocks;
+        } else {
+            return P2G_ERR_INVALID_ARGUMENT;
+        }
+        *grid_out = grid;
+        for (uint32_t b = 0; b < grid && b < cap; b++) roles_out[b] = pass_role(b, grid, h2, h3).role;
+        return P2G_OK;
+    });
+}
 
 int p2g_dev_create_batch(p2g_ctx *ctx, const p2g_batch_job *jobs, uint32_t njobs, void *stream) {
     return guarded(reinterpret_cast<Ctx *>(ctx), [&]() -> int {

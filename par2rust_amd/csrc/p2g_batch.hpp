@@ -20,6 +20,7 @@ struct BatchPlan {
     uint32_t njobs = 0;  // items: completion stamps are per item
     uint32_t n_slices = 0, n_parts = 0, max_part_S = 0;
     DevBuf d_descs, d_tb, d_staging, d_parts, d_pieces, d_sjobs, d_sync, d_done;
+    DevBuf d_err;  // sticky error word of every launch of the plan, cleared by wait_plan
     DevBuf own_afrag;
     DevBuf *afrag = &own_afrag;  // or a buffer shared by plans run in one stream order
     std::vector<BatchGroupLaunch> groups;

@@ -63,6 +63,9 @@ struct Ctx {
     int device = 0;
     int encode_variant = kEncodeMfma;
     uint32_t num_cus = 0;          // compute units of the device
+    // workgroups of each persistent kernel the device holds at once
+    // (occupancy x CUs): a persistent launch is never larger
+    uint32_t pass_resident = 0, batch_resident = 0;
     uint32_t band_cols = 0;        // fused-pass band width in 512-byte column groups (0: default)
     hipStream_t stream = nullptr;  // kernels and copies of this context
     hipStream_t side = nullptr;    // copies overlapped with `stream`; see side_stream()
@@ -73,7 +76,8 @@ struct Ctx {
 
     DevBuf d_antilog, d_base_logs;
     DevBuf d_mats, d_md5, d_crc, d_stage;
-    DevBuf d_sync;                 // fused-pass error flags, stamps and band counts
+    DevBuf d_sync;                 // fused-pass stamps and band counts
+    DevBuf d_err;                  // fused-pass sticky error word (read and cleared by p2g_synchronize)
     DevBuf d_partials;             // zero-padded copies of partial slices (fused pass)
     DevBuf d_xs_jumps;             // xorshift64 jump matrices (synthetic input)
     bool pass_pending = false;     // a fused pass may have set error flags not yet read

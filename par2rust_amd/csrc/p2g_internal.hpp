@@ -47,6 +47,14 @@ hipError_t launch_encode(const SliceDesc *d_inputs, uint32_t n_inputs, const uin
 hipError_t launch_desc_tails(SliceDesc *d_descs, uint32_t n, uint4 *d_tail_blocks,
                              hipStream_t stream);
 
+// Encode unit geometry of the matrix-core kernels: a workgroup of
+// kMfThreads threads (4 waves) computes kMfTiles 32-row MFMA tiles (16
+// recovery rows) over 512 bytes of every slice (256 symbols); the fused
+// pass holds at most kMfMaxInputs slice descriptors in LDS.
+constexpr int kMfTiles = 8;
+constexpr int kMfThreads = 256;
+constexpr int kMfMaxInputs = 2048;
+
 // Matrix-core encode.  A fragments for `n` inputs (first input's L at
 // d_logbase[0]) and R recovery rows: mfma_row_tiles(R) * mfma_steps(n) * 1 KiB.
 // A K step covers four inputs; the step count is padded to whole groups of
@@ -65,11 +73,14 @@ hipError_t launch_encode_mfma(const SliceDesc *d_inputs, uint32_t n_inputs, cons
                               hipStream_t stream);
 
 // Fused create pass (encode + slice MD5 + packet MD5, one launch).  The
-// sync words: [kPassFlags] error flags; from kPassTimes, 2 kPassRoles u64
+// sync words (cleared by each launch): [kPassFlags] unused; from kPassTimes, 2 kPassRoles u64
 // real-time counter stamps (100 MHz): the first start and the last end of
 // each role's work (packet MD5, slice MD5, encode, CRC); [kPassBands + b]
 // finished units of column band b; [kPassBands + n_bands + b] the first
-// column group of band b (n_bands + 1 entries).
+// column group of band b (n_bands + 1 entries).  Errors go to a separate
+// sticky word (kPassErrWait bits) that no launch clears: the host reads and
+// clears it when it waits, so an error of an earlier pass queued before the
+// wait is not lost to a later pass's reset.
 constexpr uint32_t kPassFlags = 0, kPassTimes = 2, kPassRoles = 4;
 constexpr uint32_t kPassBands = kPassTimes + 4 * kPassRoles;
 enum { kPassPacket = 0, kPassSlice = 1, kPassEncode = 2, kPassCrc = 3 };
@@ -92,15 +103,40 @@ __host__ __device__ inline uint32_t batch_band_width(uint32_t done, uint32_t rem
     return done == 0 && W >= remaining ? remaining : next_band_width(done, remaining, W);
 }
 
+// Role of workgroup `blk` of a persistent create launch (fused or batched)
+// of `grid` workgroups, h2 of them hashing slices and h3 hashing recovery
+// packets.  The encode workgroups, the producers, take the lowest indices
+// [0, grid - h2 - h3); the slice hashes, which wait for nothing, follow; the
+// packet hashes, which wait for the encode's bands, take the highest.  The
+// dispatcher hands out a grid's workgroups in index order, so when fewer
+// CUs are free than the grid has workgroups (another process on the GPU),
+// every producer is dispatched before any waiter and the waiters never hold
+// the CUs the producers need.  `index` is the workgroup's rank in its role.
+struct PassRole {
+    uint32_t role, index;
+};
+__host__ __device__ inline PassRole pass_role(uint32_t blk, uint32_t grid, uint32_t h2,
+                                              uint32_t h3) {
+    const uint32_t e = grid - h2 - h3;
+    if (blk < e) return {(uint32_t)kPassEncode, blk};
+    if (blk < e + h2) return {(uint32_t)kPassSlice, blk - e};
+    return {(uint32_t)kPassPacket, blk - e - h2};
+}
+
 struct PassPlan {
     uint32_t grid = 0, h2_blocks = 0, h3_blocks = 0;
     uint32_t row_blocks = 0, col_groups = 0, band_cols = 0, n_bands = 0, n_units = 0;
 };
 // False when the fused pass does not apply (no inputs or recovery blocks,
 // more inputs than one encode launch takes, too few CUs).  band_cols = 0
-// picks the default band width.
-bool plan_create_pass(uint32_t n, uint32_t R, uint32_t S, uint32_t num_cus, uint32_t band_cols,
-                      PassPlan *p);
+// picks the default band width.  max_resident: how many workgroups of the
+// pass the device holds at once (occupancy x CUs); the grid is at most that.
+bool plan_create_pass(uint32_t n, uint32_t R, uint32_t S, uint32_t num_cus, uint32_t max_resident,
+                      uint32_t band_cols, PassPlan *p);
+// Workgroups per CU of the persistent kernels as compiled (their LDS and
+// registers allow), from the HIP occupancy calculator; 0 when it fails.
+int pass_occupancy();
+int batch_occupancy();
 uint32_t pass_sync_words(const PassPlan &p);
 // Zeroes the sync words, then runs the pass.  d_inputs must have their tail
 // words (launch_desc_tails) and d_afrag the fragments of all n inputs.
@@ -108,8 +144,8 @@ hipError_t launch_create_pass(const PassPlan &p, const SliceDesc *d_inputs,
                               const uint4 *d_tail_blocks, uint32_t n, const uint4 *d_afrag,
                               uint32_t R, uint32_t S, uint8_t *d_rec,
                               uint8_t *d_slice_md5, uint32_t *d_slice_crc, const uint8_t set_id[16],
-                              uint8_t *d_pkt_md5, uint32_t *d_sync, bool all_full,
-                              hipStream_t stream);
+                              uint8_t *d_pkt_md5, uint32_t *d_sync, uint32_t *d_err,
+                              bool all_full, hipStream_t stream);
 // ---- batched create pass ------------------------------------------------
 //
 // Many create jobs in one persistent launch per group of jobs.  A job's
@@ -160,6 +196,7 @@ struct BatchGroupLaunch {
     uint32_t n_units, n_tasks;
     uint32_t h2_blocks, h3_blocks, grid;
     uint32_t *sync;             // group sync words
+    uint32_t *err;              // the plan's sticky error word (not cleared by launches)
     uint32_t sync_words;        // flags, times, counters (zeroed by the reset)
     uint32_t n_jobs;            // whole batch
     unsigned long long *job_done;  // n_jobs completion stamps, then the batch start
@@ -189,8 +226,8 @@ struct BatchGroupWork {
 };
 // Group sizing: MD5 workgroups for n_tasks packet tasks and n_slices slices,
 // as few as are estimated to finish within the encode (the rest encode).
-void plan_batch_group(uint32_t num_cus, uint32_t n_tasks, uint32_t n_slices, uint32_t n_units,
-                      const BatchGroupWork &w, BatchGroupLaunch *g);
+void plan_batch_group(uint32_t num_cus, uint32_t max_resident, uint32_t n_tasks, uint32_t n_slices,
+                      uint32_t n_units, const BatchGroupWork &w, BatchGroupLaunch *g);
 constexpr uint32_t kBatchProgress = 1;  // sync word: units published so far
 
 // Synthetic input: the xorshift64 stream from `seed` into d_dst.  d_jumps

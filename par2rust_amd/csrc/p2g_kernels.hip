@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstddef>
+#include <cstdlib>
 #include <type_traits>
 
 #include "p2g_internal.hpp"
@@ -330,8 +331,6 @@ __host__ __device__ __forceinline__ uint32_t mf_a_word(uint32_t m, int d) {
 // operand of 0 would mean 2^-127.
 constexpr int kE8M0One = 127;
 
-constexpr int kMfTiles = 8;      // 32-row MFMA tiles per workgroup: 16 recovery rows
-constexpr int kMfThreads = 256;  // 4 waves, each 64 symbols (two 32-column tiles)
 
 // A fragments: frag (t, g) is 64 lanes x 16 bytes at afrag[(t * nq + g) * 64 + lane].
 // Entry idx = (t * nq + g) * 64 + lane.  Row r of the fragments is recovery
@@ -457,7 +456,6 @@ hipError_t launch_mfma_afrag(const uint16_t *d_antilog, const uint16_t *d_logbas
 
 // LDS: input descriptors of the launch (at most kMfMaxInputs) and a double
 // buffer of the workgroup's A fragments for one K step (8 tiles x 1 KiB).
-constexpr int kMfMaxInputs = 2048;
 constexpr int kMfAhead = 4;  // K steps between issuing a load and consuming it
 
 // Branch-free word loads of one K step: whole words come from the slice,
@@ -1478,7 +1476,8 @@ struct PassArgs {
     uint8_t *slice_md5;
     uint32_t *slice_crc;
     uint8_t *pkt_md5;
-    uint32_t *sync;  // [kPassFlags] errors, [kPassBands + b] band counts
+    uint32_t *sync;  // stamps, [kPassBands + b] band counts, band starts
+    uint32_t *err;   // sticky error word: kept across passes until the host reads it
     PacketPrefix pre;
     uint32_t n, nq, R, S;
     uint32_t row_blocks, col_groups, band_cols, n_bands, n_units;
@@ -1527,7 +1526,7 @@ __device__ __forceinline__ void pass_packet_lane(const PassArgs &a, uint32_t r) 
     const uint32_t *bstart = a.sync + kPassBands + a.n_bands;
     for (uint32_t b = 0; b < a.n_bands; b++) {
         const uint32_t c1 = bstart[b + 1], w = c1 - bstart[b];
-        if (!pass_wait(a.sync + kPassBands + b, w * a.row_blocks, a.stall_ticks, a.sync + kPassFlags))
+        if (!pass_wait(a.sync + kPassBands + b, w * a.row_blocks, a.stall_ticks, a.err))
             return;
         // data bytes [0, e) of every block are final; block k needs up to 64 k + 28
         const uint64_t e = min((uint64_t)c1 * (kMfThreads * 2), (uint64_t)a.S);
@@ -1564,15 +1563,16 @@ __global__ __launch_bounds__(kMfThreads, 1) void k_create_pass(PassArgs a) {
     __shared__ MfLds lds;
     // the A ring of mf_unit_dma: four separate objects (see there)
     __shared__ uint4 s_a0[kMfRing], s_a1[kMfRing], s_a2[kMfRing], s_a3[kMfRing];
-    const uint32_t blk = blockIdx.x, tid = threadIdx.x;
+    const uint32_t tid = threadIdx.x;
+    const PassRole role = pass_role(blockIdx.x, gridDim.x, a.h2_blocks, a.h3_blocks);
     // MD5 workgroups run their hash on wave 0 (64 lanes: each lane streams its
     // own message).  In the slice-hash workgroups waves 1-3, on the other
     // SIMDs, take the slice CRCs, one slice per wave, with their tables in the
     // LDS the encode would use; the packet-hash workgroups keep their CU's
     // memory path to themselves (they must keep pace with the encode).
-    if (blk < a.h3_blocks && tid >= 64) return;
-    if (blk < a.h3_blocks + a.h2_blocks && tid >= 64) {
-        const uint32_t cw = (blk - a.h3_blocks) * 3 + (tid >> 6) - 1, n_cw = 3 * a.h2_blocks;
+    if (role.role == kPassPacket && tid >= 64) return;
+    if (role.role == kPassSlice && tid >= 64) {
+        const uint32_t cw = role.index * 3 + (tid >> 6) - 1, n_cw = 3 * a.h2_blocks;
         uint32_t(*t)[256] = reinterpret_cast<uint32_t(*)[256]>(lds.desc) + 4 * ((tid >> 6) - 1);
         crc_tables_init_wave(t);
         if (tid % 64 == 0) pass_stamp_start(a, kPassCrc);
@@ -1581,15 +1581,15 @@ __global__ __launch_bounds__(kMfThreads, 1) void k_create_pass(PassArgs a) {
         if (tid % 64 == 0) pass_stamp_end(a, kPassCrc);
         return;
     }
-    if (blk < a.h3_blocks) {
+    if (role.role == kPassPacket) {
         if (tid == 0) pass_stamp_start(a, kPassPacket);
-        for (uint32_t r = blk * 64 + tid; r < a.R; r += a.h3_blocks * 64) pass_packet_lane(a, r);
+        for (uint32_t r = role.index * 64 + tid; r < a.R; r += a.h3_blocks * 64) pass_packet_lane(a, r);
         if (tid == 0) pass_stamp_end(a, kPassPacket);
         return;
     }
-    if (blk < a.h3_blocks + a.h2_blocks) {
+    if (role.role == kPassSlice) {
         if (tid == 0) pass_stamp_start(a, kPassSlice);
-        const uint32_t hb = blk - a.h3_blocks;
+        const uint32_t hb = role.index;
         for (uint32_t s = hb * 64 + tid; s < a.n; s += a.h2_blocks * 64) {
             // slices start 16-byte aligned when S is a multiple of 16 (file bases are)
             const uint4 *tb = a.tail_blocks + 4 * (size_t)s;
@@ -1608,8 +1608,7 @@ __global__ __launch_bounds__(kMfThreads, 1) void k_create_pass(PassArgs a) {
     // has no divergent exit around its barriers (a unit index fetched by one
     // thread and broadcast through LDS let the compiler loop the other lanes
     // back to the barrier without that thread).
-    const uint32_t first = a.h3_blocks + a.h2_blocks;
-    const uint32_t e_blocks = gridDim.x - first;
+    const uint32_t e_blocks = gridDim.x - a.h3_blocks - a.h2_blocks;
     // Band b holds units [row_blocks * bstart[b], row_blocks * bstart[b + 1]).
     const uint32_t *bstart = a.sync + kPassBands + a.n_bands;
     uint32_t band = 0, b_lo = 0, b_hi = a.row_blocks * bstart[1];
@@ -1619,7 +1618,7 @@ __global__ __launch_bounds__(kMfThreads, 1) void k_create_pass(PassArgs a) {
     // reach L2), a barrier, then one agent-scope release (L2 write-back) and
     // one add by thread 0.
     uint32_t done = 0;  // units of band `band` finished and not yet counted
-    for (uint32_t u = blk - first; u < a.n_units; u += e_blocks) {
+    for (uint32_t u = role.index; u < a.n_units; u += e_blocks) {
         while (u >= b_hi) {
             band++;
             b_lo = b_hi;
@@ -1649,6 +1648,16 @@ __global__ __launch_bounds__(kMfThreads, 1) void k_create_pass(PassArgs a) {
     }
 }
 
+// How long a waiting role keeps waiting without progress before it gives up
+// and flags the launch: 2 s of the 100 MHz real-time counter.
+// P2G_STALL_TICKS overrides it (read at each launch), so that tests can
+// drive the give-up path from the host with a limit that every wait hits
+// while the rest of the launch still runs to its normal end.
+static uint32_t stall_ticks() {
+    if (const char *v = getenv("P2G_STALL_TICKS")) return (uint32_t)strtoul(v, nullptr, 10);
+    return 200000000u;
+}
+
 // Zeroes flags and counts (start stamps, the even u64 slots, get the maximum:
 // they are minima over workgroups), and thread 0 writes the band starts.
 __global__ void k_pass_reset(uint32_t *sync, uint32_t n_bands, uint32_t col_groups,
@@ -1668,32 +1677,14 @@ __global__ void k_pass_reset(uint32_t *sync, uint32_t n_bands, uint32_t col_grou
     }
 }
 
-bool plan_create_pass(uint32_t n, uint32_t R, uint32_t S, uint32_t num_cus, uint32_t band_cols,
-                      PassPlan *p) {
-    if (n == 0 || R == 0 || 4 * mfma_steps(n) > (uint32_t)kMfMaxInputs) return false;
-    p->grid = num_cus;
-    // one 64-lane wave per MD5 workgroup; lanes loop when the caps bind
-    p->h3_blocks = std::min<uint32_t>((R + 63) / 64, 8);
-    p->h2_blocks = std::min<uint32_t>((n + 63) / 64, 32);
-    if (4 * (p->h2_blocks + p->h3_blocks) >= num_cus) return false;
-    p->row_blocks = mfma_row_tiles(R) / kMfTiles;
-    p->col_groups = (S + kMfThreads * 2 - 1) / (kMfThreads * 2);
-    p->band_cols = std::max<uint32_t>(1, std::min(band_cols ? band_cols : 64u, p->col_groups));
-    p->n_bands = 0;
-    for (uint32_t c = 0; c < p->col_groups; p->n_bands++)
-        c += next_band_width(c, p->col_groups - c, p->band_cols);
-    p->n_units = p->row_blocks * p->col_groups;
-    return true;
-}
-
 uint32_t pass_sync_words(const PassPlan &p) { return kPassBands + 2 * p.n_bands + 1; }
 
 hipError_t launch_create_pass(const PassPlan &p, const SliceDesc *d_inputs,
                               const uint4 *d_tail_blocks, uint32_t n,
                               const uint4 *d_afrag, uint32_t R, uint32_t S, uint8_t *d_rec,
                               uint8_t *d_slice_md5, uint32_t *d_slice_crc, const uint8_t set_id[16],
-                              uint8_t *d_pkt_md5, uint32_t *d_sync, bool all_full,
-                              hipStream_t stream) {
+                              uint8_t *d_pkt_md5, uint32_t *d_sync, uint32_t *d_err,
+                              bool all_full, hipStream_t stream) {
     const uint32_t words = kPassBands + p.n_bands;
     hipLaunchKernelGGL(k_pass_reset, dim3(std::min<uint32_t>((words + 255) / 256, 64)), dim3(256),
                        0, stream, d_sync, p.n_bands, p.col_groups, p.band_cols);
@@ -1708,6 +1699,7 @@ hipError_t launch_create_pass(const PassPlan &p, const SliceDesc *d_inputs,
     a.slice_crc = d_slice_crc;
     a.pkt_md5 = d_pkt_md5;
     a.sync = d_sync;
+    a.err = d_err;
     a.pre = make_packet_prefix(set_id);
     a.n = n;
     a.nq = mfma_steps(n);
@@ -1720,7 +1712,7 @@ hipError_t launch_create_pass(const PassPlan &p, const SliceDesc *d_inputs,
     a.n_units = p.n_units;
     a.h3_blocks = p.h3_blocks;
     a.h2_blocks = p.h2_blocks;
-    a.stall_ticks = 200000000u;  // 2 s of the 100 MHz counter without a finished unit
+    a.stall_ticks = stall_ticks();  // without a finished unit
     static const CrcPowers pw = make_crc_powers();
     a.pw = pw;
     if (all_full)
@@ -1800,6 +1792,7 @@ struct BatchArgs {
     const SliceDesc *descs;      // whole batch, indexed by global slice index
     const uint4 *tail_blocks;    // 4 per global slice
     uint32_t *sync;
+    uint32_t *err;  // sticky error word of the plan: kept across launches until read
     unsigned long long *job_done;
     uint32_t n_pieces, n_sjobs, slice_lo, slice_hi;
     uint32_t n_units, n_tasks, h3_blocks, h2_blocks;
@@ -1872,7 +1865,7 @@ __device__ __forceinline__ void batch_packet_lane(const BatchArgs &a, const Batc
     for (uint32_t b = 0; b < P.n_bands; b++) {
         const uint32_t c1 = bstart[b + 1], w = c1 - bstart[b];
         if (!batch_wait(a.sync + P.band0 + b, w * P.row_blocks, a.sync + kBatchProgress,
-                        a.stall_ticks, a.sync + kPassFlags))
+                        a.stall_ticks, a.err))
             return;
         const uint64_t e = min((uint64_t)c1 * (kMfThreads * 2), (uint64_t)S);
         const uint32_t avail = b + 1 == P.n_bands ? nb : e >= 28 ? (uint32_t)((e - 28) / 64) + 1 : 0;
@@ -1901,15 +1894,16 @@ __global__ __launch_bounds__(kMfThreads, 1) void k_create_batch(BatchArgs a) {
     __shared__ BatchLds lds;
     __shared__ uint4 s_a0[kMfRing], s_a1[kMfRing], s_a2[kMfRing], s_a3[kMfRing];
     const uint32_t blk = blockIdx.x, tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const PassRole role = pass_role(blk, gridDim.x, a.h2_blocks, a.h3_blocks);
     if (tid == 0) {
         unsigned long long *t0 = a.job_done + a.n_jobs;
         atomicMin(t0, (unsigned long long)__builtin_amdgcn_s_memrealtime());
     }
-    if (blk < a.h3_blocks) {
+    if (role.role == kPassPacket) {
         // packet tasks: wave task t = 64 packets of one piece
         if (lane == 0) batch_role(a.sync, kPassPacket, false);
         uint32_t pi = 0;
-        for (uint32_t t = blk * 4 + wave; t < a.n_tasks; t += a.h3_blocks * 4) {
+        for (uint32_t t = role.index * 4 + wave; t < a.n_tasks; t += a.h3_blocks * 4) {
             while (t >= a.pieces[pi].task_end) pi++;
             const BatchPiece &P = a.pieces[pi];
             const uint32_t r = (t - P.task0) * 64 + lane;
@@ -1919,11 +1913,11 @@ __global__ __launch_bounds__(kMfThreads, 1) void k_create_batch(BatchArgs a) {
         if (lane == 0) batch_role(a.sync, kPassPacket, true);
         return;
     }
-    if (blk < a.h3_blocks + a.h2_blocks) {
+    if (role.role == kPassSlice) {
         // slice MD5 + CRC-32 lanes on waves 0 and 1 (CRC tables per wave in
         // lds.desc); a lane walks the slice jobs on its own
         if (wave >= 2) return;
-        const uint32_t hb = blk - a.h3_blocks;
+        const uint32_t hb = role.index;
         static_assert(sizeof(BatchLds::desc) >= 2 * 16 * 256 * 4, "two waves of 16 CRC tables");
         uint32_t(*t)[256] = reinterpret_cast<uint32_t(*)[256]>(lds.desc) + 16 * wave;
         crc_tables_init_wave<16>(t);
@@ -1948,16 +1942,15 @@ __global__ __launch_bounds__(kMfThreads, 1) void k_create_batch(BatchArgs a) {
     // Encode.  Every value the loop branches on is computed alike by all
     // threads from the tables (no broadcast), so the barriers inside are
     // reached by the whole workgroup.
-    const uint32_t first = a.h3_blocks + a.h2_blocks;
-    const uint32_t e_blocks = gridDim.x - first;
+    const uint32_t e_blocks = gridDim.x - a.h3_blocks - a.h2_blocks;
     if (tid == 0) batch_role(a.sync, kPassEncode, false);
     // Workgroups are dealt to the 8 XCDs round-robin (XCD = blk % 8).  Each
     // round of e_blocks units is taken in XCD order: this workgroup's rank
-    // is the encode workgroups of lower XCDs, then those of its own XCD
-    // before it, so units adjacent in order (one row block's column groups)
-    // run on one XCD and share its L2.
+    // is the encode workgroups [0, e_blocks) of lower XCDs, then those of
+    // its own XCD before it, so units adjacent in order (one row block's
+    // column groups) run on one XCD and share its L2.
     uint32_t rank = 0;
-    for (uint32_t b = first; b < gridDim.x; b++)
+    for (uint32_t b = 0; b < e_blocks; b++)
         if (b % 8 < blk % 8 || (b % 8 == blk % 8 && b < blk)) rank++;
     uint32_t pi = 0;
     const SliceDesc *loaded = nullptr;  // first descriptor of the chunk in lds.desc
@@ -2079,90 +2072,21 @@ hipError_t launch_batch_afrag(const uint16_t *d_antilog, const uint16_t *d_logba
     return hipGetLastError();
 }
 
-// Costs in µs measured on MI355X at the batched mix, for sizing the roles of
-// a group: an encode unit's K step on one workgroup; a 64-byte block of a
-// slice lane (MD5 and CRC-32) and of a packet lane; the fixed cost of a
-// slice and of a packet task.
-constexpr double kStepUs = 0.38, kSliceBlockUs = 1.31, kSliceUs = 36;
-constexpr double kPacketBlockUs = 0.65, kTaskUs = 40;
-
-// Estimated end (µs) of the packet role on 4 h3 waves when the encode takes
-// t_enc: pieces are encoded in order, each over its share of t_enc by its
-// unit-steps; task t runs on wave t mod 4 h3 after the wave's earlier tasks,
-// starts no earlier than its piece's encode and, following its bands, ends
-// no earlier than the piece's encode ends.
-static double packet_role_end(const BatchGroupWork &w, uint32_t h3, double t_enc) {
-    if (h3 == 0 || w.enc_steps <= 0) return 0;
-    std::vector<double> free_at(4 * h3, 0.0);
-    double done = 0, end = 0;
-    size_t t = 0;
-    for (size_t p = 0; p < w.piece_steps.size(); p++) {
-        const double start = t_enc * done / w.enc_steps;
-        done += w.piece_steps[p];
-        const double finish = t_enc * done / w.enc_steps;
-        const double chain = kTaskUs + w.piece_task_blocks[p] * kPacketBlockUs;
-        for (double k = 0; k < w.piece_tasks[p]; k++, t++) {
-            double &f = free_at[t % free_at.size()];
-            f = std::max(std::max(f, start) + chain, finish);
-            end = std::max(end, f);
-        }
-    }
-    return end;
+int pass_occupancy() {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_create_pass<true>, kMfThreads, 0) != hipSuccess)
+        return 0;
+    int nb2 = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb2, k_create_pass<false>, kMfThreads, 0) != hipSuccess)
+        return 0;
+    return std::min(nb, nb2);
 }
 
-void plan_batch_group(uint32_t num_cus, uint32_t n_tasks, uint32_t n_slices, uint32_t n_units,
-                      const BatchGroupWork &w, BatchGroupLaunch *g) {
-    g->grid = num_cus;
-    // four packet waves per packet workgroup, two MD5 + CRC-32 waves (128
-    // lanes) per slice workgroup; the rest encode
-    const uint32_t h3_cap = std::min<uint32_t>((n_tasks + 3) / 4, 8);
-    const uint32_t h2_cap = std::min<uint32_t>((n_slices + 127) / 128, 64);
-    if (n_units == 0) {
-        g->h3_blocks = h3_cap;
-        g->h2_blocks = h2_cap;
-        g->grid = std::max<uint32_t>(1, g->h2_blocks + g->h3_blocks);
-        return;
-    }
-    // A hash workgroup holds its CU until its role is done, so each role
-    // gets the fewest workgroups that keep it within the encode on the
-    // remaining CUs.  The slice lanes' estimate is the larger of their
-    // average work and the rounds of the largest slices (lanes take slices
-    // in stride), kept within 3/4 of the encode.  The packet tasks follow
-    // the encode band by band: workgroups are added while the simulated end
-    // of their role lies more than 2% past the encode and more workgroups
-    // would bring it closer.
-    const double slice_us = w.slices * kSliceUs + w.slice_blocks * kSliceBlockUs;
-    uint32_t h3 = std::min<uint32_t>(h3_cap, 1), h2 = std::min<uint32_t>(h2_cap, 1);
-    for (;;) {
-        const uint32_t e = num_cus > h2 + h3 + 1 ? num_cus - h2 - h3 : 1;
-        const double t_enc = w.enc_steps * kStepUs / e;
-        bool grew = false;
-        const double big_rounds = std::ceil(w.big_slices / (128.0 * std::max<uint32_t>(h2, 1)));
-        const double big_us = big_rounds > 1 ? big_rounds * (kSliceUs + w.big_blocks * kSliceBlockUs) : 0;
-        if (h2 < h2_cap && std::max(slice_us / (128.0 * h2), big_us) > 0.75 * t_enc) {
-            h2++;
-            grew = true;
-        }
-        const double end = packet_role_end(w, h3, t_enc);
-        if (h3 < h3_cap && end > 1.02 * t_enc) {
-            // the fewest more workgroups that bring the end closer (waves
-            // that still hold two tasks of one piece gain nothing)
-            for (uint32_t h = h3 + 1; h <= h3_cap; h++) {
-                if (packet_role_end(w, h, t_enc) < end - 0.01 * t_enc) {
-                    h3 = h;
-                    grew = true;
-                    break;
-                }
-            }
-        }
-        if (!grew) break;
-    }
-    g->h3_blocks = h3;
-    g->h2_blocks = h2;
-    if (g->h2_blocks + g->h3_blocks + 1 > num_cus) {
-        g->h3_blocks = std::min<uint32_t>(g->h3_blocks, 1);
-        g->h2_blocks = std::min<uint32_t>(g->h2_blocks, 1);
-    }
+int batch_occupancy() {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_create_batch, kMfThreads, 0) != hipSuccess)
+        return 0;
+    return nb;
 }
 
 hipError_t launch_create_batch(const BatchGroupLaunch &g, const SliceDesc *d_descs,
@@ -2177,6 +2101,7 @@ hipError_t launch_create_batch(const BatchGroupLaunch &g, const SliceDesc *d_des
     a.descs = d_descs;
     a.tail_blocks = d_tail_blocks;
     a.sync = g.sync;
+    a.err = g.err;
     a.job_done = g.job_done;
     a.n_pieces = g.n_pieces;
     a.n_sjobs = g.n_sjobs;
@@ -2187,7 +2112,7 @@ hipError_t launch_create_batch(const BatchGroupLaunch &g, const SliceDesc *d_des
     a.h3_blocks = g.h3_blocks;
     a.h2_blocks = g.h2_blocks;
     a.n_jobs = g.n_jobs;
-    a.stall_ticks = 200000000u;  // 2 s of the 100 MHz counter without any unit published
+    a.stall_ticks = stall_ticks();  // without any unit published
     hipLaunchKernelGGL(k_create_batch, dim3(g.grid), dim3(kMfThreads), 0, stream, a);
     return hipGetLastError();
 }

@@ -74,9 +74,12 @@ struct Pool {
 };
 
 struct Request {
-    enum State { kQueued, kActive, kDone };
+    // kFailed: a step holding some of its work could not complete; its
+    // outputs are not valid and p2g_server_result reports the step's error.
+    enum State { kQueued, kActive, kDone, kFailed };
     uint64_t id = 0;
     State state = kQueued;
+    int err = P2G_OK;  // the failed step's status
     uint64_t S = 0;
     uint32_t R = 0, n = 0;
     std::vector<uint64_t> lens, seeds;
@@ -118,6 +121,9 @@ struct Server {
     int admit();
     int retire(int k);
     int launch_step(int k);
+    // Waits for every step still running that holds work of q, so its
+    // memory can go back to the pool.
+    int drain(const Request *q);
     ~Server();
 };
 
@@ -200,7 +206,23 @@ int Server::launch_step(int k) {
     if (items.empty()) return P2G_OK;
     int rc = build_plan(c, items, false, &st.plan);
     if (rc == P2G_OK) rc = launch_plan(&st.plan, stream);
-    if (rc != P2G_OK) return rc;
+    if (rc != P2G_OK) {
+        // whatever part of the step was queued ends before its requests are
+        // reported failed (their memory may then go back to the pool)
+        hipStreamSynchronize(stream);
+        const double t = now_ms();
+        for (auto &pr : st.items) {
+            Request *q = pr.first;
+            if (q->state != Request::kActive) continue;
+            q->state = Request::kFailed;
+            q->err = rc;
+            q->t_done = t;
+            finished.push_back(q->id);
+            active.erase(std::find(active.begin(), active.end(), q));
+        }
+        st.items.clear();
+        return rc;
+    }
     st.busy = true;
     inflight.push_back(k);
     return P2G_OK;
@@ -213,9 +235,21 @@ int Server::retire(int k) {
     const double t = now_ms();
     for (auto &pr : st.items) {
         Request *q = pr.first;
+        if (q->state != Request::kActive) continue;  // failed in an earlier step
+        if (rc != P2G_OK) {
+            // the step's work did not complete (a packet hash gave up, or the
+            // launch failed): none of it counts, and the request is reported
+            // as failed rather than done
+            q->state = Request::kFailed;
+            q->err = rc;
+            q->t_done = t;
+            finished.push_back(q->id);
+            active.erase(std::find(active.begin(), active.end(), q));
+            continue;
+        }
         q->rows_done += pr.second.r1 - pr.second.r0;
         q->slices_done += pr.second.s1 - pr.second.s0;
-        if (q->rows_done == q->R && q->slices_done == q->n && q->state == Request::kActive) {
+        if (q->rows_done == q->R && q->slices_done == q->n) {
             q->state = Request::kDone;
             q->t_done = t;
             finished.push_back(q->id);
@@ -224,6 +258,19 @@ int Server::retire(int k) {
     }
     st.items.clear();
     return rc;
+}
+
+int Server::drain(const Request *q) {
+    for (Step &st : steps) {
+        if (!st.busy) continue;
+        for (auto &pr : st.items)
+            if (pr.first == q) {
+                const hipError_t e = hipEventSynchronize(st.plan.done);
+                if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "wait for a step", e);
+                break;
+            }
+    }
+    return P2G_OK;
 }
 
 }  // namespace p2g
@@ -331,8 +378,10 @@ int p2g_server_poll(p2g_server *srv, double wait_ms, uint64_t *done_ids, uint32_
             if (rc != P2G_OK || !s->finished.empty() || s->now_ms() >= until) break;
             std::this_thread::sleep_for(std::chrono::microseconds(50));
         }
+        // On an error the ids stay queued for the next poll (the requests of a
+        // failed step are among them, reported as failed by result).
         uint32_t k = 0;
-        while (k < cap && !s->finished.empty()) {
+        while (rc == P2G_OK && k < cap && !s->finished.empty()) {
             done_ids[k++] = s->finished.front();
             s->finished.pop_front();
         }
@@ -348,13 +397,16 @@ int p2g_server_result(p2g_server *srv, uint64_t id, p2g_batch_job *out, double t
         auto it = s->all.find(id);
         if (it == s->all.end()) return s->c->fail(P2G_ERR_INVALID_ARGUMENT, "unknown request id");
         Request *q = it->second.get();
-        if (q->state != Request::kDone) return s->c->fail(P2G_ERR_INVALID_ARGUMENT, "request not done");
+        if (q->state != Request::kDone && q->state != Request::kFailed)
+            return s->c->fail(P2G_ERR_INVALID_ARGUMENT, "request not done");
         if (out) *out = q->job;
         if (times_ms) {
             times_ms[0] = q->t_submit;
             times_ms[1] = q->t_admit;
             times_ms[2] = q->t_done;
         }
+        if (q->state == Request::kFailed)
+            return s->c->fail(q->err, "request failed: a step holding its work could not complete");
         return P2G_OK;
     });
 }
@@ -366,7 +418,13 @@ int p2g_server_release(p2g_server *srv, uint64_t id) {
         auto it = s->all.find(id);
         if (it == s->all.end()) return s->c->fail(P2G_ERR_INVALID_ARGUMENT, "unknown request id");
         Request *q = it->second.get();
-        if (q->state != Request::kDone) return s->c->fail(P2G_ERR_INVALID_ARGUMENT, "request not done");
+        if (q->state != Request::kDone && q->state != Request::kFailed)
+            return s->c->fail(P2G_ERR_INVALID_ARGUMENT, "request not done");
+        // a failed request may still have pieces in the step queued after
+        // the failed one: its memory returns to the pool once that ends
+        hipSetDevice(s->c->device);
+        const int rc = s->drain(q);
+        if (rc != P2G_OK) return rc;
         s->pool.release(q->off, q->footprint);
         s->all.erase(it);
         return P2G_OK;

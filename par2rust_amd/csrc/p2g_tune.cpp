@@ -53,7 +53,7 @@ int tune_and_run(Ctx *c, uint64_t n, uint32_t R, uint64_t S, hipStream_t st,
     std::vector<std::pair<int, uint32_t>> cand;
     PassPlan plan;
     const uint32_t col_groups = (uint32_t)((S + 511) / 512);
-    if (n <= kMaxInputs && plan_create_pass((uint32_t)n, R, (uint32_t)S, c->num_cus, 0, &plan)) {
+    if (n <= kMaxInputs && plan_create_pass((uint32_t)n, R, (uint32_t)S, c->num_cus, c->pass_resident, 0, &plan)) {
         for (uint32_t w : {16u, 32u, 64u, 128u})
             if (w <= std::max<uint32_t>(col_groups, 16)) cand.push_back({kEncodeMfma, w});
     } else {

@@ -102,20 +102,6 @@ int write_atomic(const std::string &path, const std::vector<const std::vector<ui
     return P2G_OK;
 }
 
-// derive_volume_filename, src/creator.rs:1046-1057, with Rust's
-// Path::file_stem / extension split (a leading dot does not start an
-// extension).
-std::string volume_name(const std::string &index, uint32_t first, uint32_t count) {
-    const size_t slash = index.rfind('/');
-    const size_t fstart = slash == std::string::npos ? 0 : slash + 1;
-    const std::string dir = index.substr(0, fstart);
-    const std::string fname = index.substr(fstart);
-    const size_t dot = fname.rfind('.');
-    const bool has_ext = dot != std::string::npos && dot != 0;
-    const std::string stem = has_ext ? fname.substr(0, dot) : fname;
-    const std::string ext = has_ext ? fname.substr(dot + 1) : "par2";
-    return dir + stem + ".vol" + std::to_string(first) + "+" + std::to_string(count) + "." + ext;
-}
 
 // Pinned host buffer for file reads.
 struct Pinned {
@@ -158,80 +144,6 @@ uint64_t chunk_bytes_for(uint64_t S) {
 }
 
 }  // namespace
-
-int resolve_volume_sizes(const VolumeScheme &s, uint32_t total, std::vector<uint32_t> &out,
-                         Error &err) {
-    out.clear();
-    std::vector<uint32_t> inner;
-    if (s.limited && s.limit_cap == 0)
-        return set_err(err, P2G_ERR_INVALID_VOLUME_SCHEME,
-                       "Limited scheme requires max_blocks_per_volume > 0");
-    switch (s.kind) {
-    case P2G_SCHEME_SINGLE:
-        inner.push_back(total);
-        break;
-    case P2G_SCHEME_EXPONENTIAL: {  // exponential_split, src/creator.rs:367-385
-        uint32_t remaining = total, next = 1;
-        while (remaining > 0) {
-            const uint32_t take = std::min(next, remaining);
-            inner.push_back(take);
-            remaining -= take;
-            if (inner.size() >= 2) next = next > 0x7FFFFFFFu ? 0xFFFFFFFFu : next * 2;
-        }
-        break;
-    }
-    case P2G_SCHEME_UNIFORM: {  // uniform_split, src/creator.rs:390-409
-        const uint32_t count = s.uniform_count;
-        if (count == 0)
-            return set_err(err, P2G_ERR_INVALID_VOLUME_SCHEME, "Uniform scheme requires count > 0");
-        if (count > total)
-            return set_err(err, P2G_ERR_INVALID_VOLUME_SCHEME,
-                           "Uniform scheme count (" + std::to_string(count) +
-                               ") exceeds recovery block total (" + std::to_string(total) + ")");
-        const uint32_t base = total / count, rem = total - base * count;
-        for (uint32_t i = 0; i + 1 < count; i++) inner.push_back(base);
-        inner.push_back(base + rem);
-        break;
-    }
-    case P2G_SCHEME_EXPLICIT: {  // src/creator.rs:346-360
-        if (s.explicit_sizes.empty())
-            return set_err(err, P2G_ERR_INVALID_VOLUME_SCHEME,
-                           "explicit scheme requires at least one volume");
-        uint64_t sum = 0;
-        for (uint32_t v : s.explicit_sizes) {
-            if (v == 0)
-                return set_err(err, P2G_ERR_INVALID_VOLUME_SCHEME,
-                               "explicit scheme volumes must each have >0 recovery blocks");
-            sum += v;
-        }
-        if (sum != total)
-            return set_err(err, P2G_ERR_INVALID_VOLUME_SCHEME,
-                           "explicit volume sizes sum to " + std::to_string(sum) + ", expected " +
-                               std::to_string(total));
-        inner = s.explicit_sizes;
-        break;
-    }
-    default:
-        return set_err(err, P2G_ERR_INVALID_VOLUME_SCHEME, "unknown volume scheme");
-    }
-    if (!s.limited) {
-        out = inner;
-        return P2G_OK;
-    }
-    for (uint32_t n : inner) {  // cap_volume_sizes, src/creator.rs:414-431
-        if (n <= s.limit_cap) {
-            out.push_back(n);
-            continue;
-        }
-        uint32_t rem = n;
-        while (rem > s.limit_cap) {
-            out.push_back(s.limit_cap);
-            rem -= s.limit_cap;
-        }
-        if (rem > 0) out.push_back(rem);
-    }
-    return P2G_OK;
-}
 
 int scan(p2g_ctx *ctx, const std::string &path, const std::vector<uint8_t> &name,
          uint64_t slice_size, SourceFile &out, Error &err) {
@@ -812,8 +724,12 @@ extern "C" int p2g_run_create(p2g_ctx *ctx, const char *output, uint64_t slice_s
         opts.recovery_block_count = recovery_count;
         opts.volume_scheme.kind = scheme->kind;
         opts.volume_scheme.uniform_count = scheme->uniform_count;
-        opts.volume_scheme.limited = scheme->limited != 0;
-        opts.volume_scheme.limit_cap = scheme->limit_cap;
+        if (scheme->limited) opts.volume_scheme.caps.push_back(scheme->limit_cap);
+        if (scheme->n_outer_caps) {
+            if (!scheme->limited || !scheme->outer_caps) return P2G_ERR_INVALID_ARGUMENT;
+            opts.volume_scheme.caps.insert(opts.volume_scheme.caps.end(), scheme->outer_caps,
+                                           scheme->outer_caps + scheme->n_outer_caps);
+        }
         if (scheme->explicit_sizes)
             opts.volume_scheme.explicit_sizes.assign(scheme->explicit_sizes,
                                                      scheme->explicit_sizes + scheme->n_explicit);

@@ -34,8 +34,9 @@ struct SourceFile {
 struct VolumeScheme {
     int kind = P2G_SCHEME_SINGLE;  // library default is Single, src/creator.rs:163-173
     uint32_t uniform_count = 0;
-    bool limited = false;
-    uint32_t limit_cap = 0;
+    // Limited { max_blocks_per_volume, inner } wrappers around the scheme
+    // (src/creator.rs:131-134), their caps from the innermost outward
+    std::vector<uint32_t> caps;
     std::vector<uint32_t> explicit_sizes;
 };
 
@@ -68,8 +69,11 @@ int create_resident(p2g_ctx *ctx, const CreateOptions &opts, const std::vector<s
                     const std::vector<std::vector<uint8_t>> &names, std::vector<std::string> &written,
                     Error &err, bool *declined);
 
-// resolve_volume_sizes, src/creator.rs:325-431.
+// resolve_volume_sizes, src/creator.rs:325-431 (par2_scheme.cpp).
 int resolve_volume_sizes(const VolumeScheme &s, uint32_t total, std::vector<uint32_t> &out,
                          Error &err);
+
+// derive_volume_filename, src/creator.rs:1046-1057 (par2_scheme.cpp).
+std::string volume_name(const std::string &index, uint32_t first, uint32_t count);
 
 }  // namespace par2

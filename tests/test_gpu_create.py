@@ -34,7 +34,7 @@ def both(tmp_path, inputs, S, R, scheme: p2.VolumeScheme, ctx, out_name="recover
     db.mkdir()
     got = ctx.run_create(p2.CreateOptions(str(da / out_name), S, R, scheme), inputs)
     want = O.run_create(str(db / out_name), S, R, inputs, scheme.kind, scheme.uniform_count,
-                        scheme.limited, scheme.limit_cap, scheme.explicit_sizes)
+                        scheme.limited, scheme.limit_cap, scheme.explicit_sizes, scheme.outer_caps)
     assert [os.path.basename(p) for p in got] == [os.path.basename(p) for p in want]
     for g, w in zip(got, want):
         assert open(g, "rb").read() == open(w, "rb").read(), os.path.basename(g)
@@ -77,6 +77,12 @@ def test_explicit_and_limited(tmp_path, ctx, create_path):
     (tmp_path / "lim").mkdir()
     both(tmp_path / "lim", [(str(src), b"x.bin")], 1024, 20,
          p2.VolumeScheme.exponential().capped(3), ctx, out_name="lim.par2")
+    # Limited nested in Limited (src/creator.rs:131-134): caps 6 then 4
+    (tmp_path / "nest").mkdir()
+    got = both(tmp_path / "nest", [(str(src), b"x.bin")], 1024, 10,
+               p2.VolumeScheme.explicit([10]).capped(6).capped(4), ctx, out_name="n.par2")
+    assert [os.path.basename(p) for p in got][1:] == ["n.vol0+4.par2", "n.vol4+2.par2",
+                                                       "n.vol6+4.par2"]
 
 
 def test_output_name_without_extension(tmp_path, ctx):

@@ -504,3 +504,30 @@ def test_limited_and_explicit():
     assert O.resolve_volume_sizes(13, X, explicit=[2, 10, 1], limited=True, limit_cap=3) == \
         [2, 3, 3, 3, 1, 1]
     assert O.resolve_volume_sizes(8, X, explicit=[8], limited=True, limit_cap=4) == [4, 4]
+
+
+def test_nested_limited():
+    # Limited { inner: Box<VolumeScheme> } nests (src/creator.rs:131-134,
+    # :330-344): the inner Limited resolves first, the outer one caps its
+    # sizes.  Limited(4, Limited(6, Explicit([10]))): [6, 4] -> [4, 2, 4],
+    # which differs from one cap of 4 ([4, 4, 2]).
+    E, U, X = O.SCHEME_EXPONENTIAL, O.SCHEME_UNIFORM, O.SCHEME_EXPLICIT
+    assert O.resolve_volume_sizes(10, X, explicit=[10], limited=True, limit_cap=6,
+                                  outer_caps=[4]) == [4, 2, 4]
+    assert O.resolve_volume_sizes(10, X, explicit=[10], limited=True, limit_cap=4) == [4, 4, 2]
+    # each level equals a single-level Limited over the level below, taken
+    # as an explicit scheme
+    for kind, cnt, total, caps in ((E, 0, 200, [50, 7, 3]), (U, 3, 100, [40, 9]),
+                                   (E, 0, 1000, [5, 300])):
+        below = O.resolve_volume_sizes(total, kind, cnt)
+        for k, cap in enumerate(caps):
+            below = O.resolve_volume_sizes(total, X, explicit=below, limited=True, limit_cap=cap)
+            assert O.resolve_volume_sizes(total, kind, cnt, limited=True, limit_cap=caps[0],
+                                          outer_caps=caps[1:k + 1]) == below
+    # a zero cap at any level is an error, checked before the inner scheme's
+    # own errors (Uniform count above the total would be one)
+    for caps in ([0, 4], [4, 0], [3, 5, 0]):
+        with pytest.raises(O.SchemeError):
+            O.resolve_volume_sizes(10, U, 99, limited=True, limit_cap=caps[0], outer_caps=caps[1:])
+    with pytest.raises(O.SchemeError):  # outer wrappers need the first one
+        O.resolve_volume_sizes(10, E, outer_caps=[4])
