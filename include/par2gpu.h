@@ -153,6 +153,27 @@ int p2g_dev_create_pass(p2g_ctx *ctx, const p2g_dev_file *files, uint32_t nfiles
                         uint8_t *rec, uint8_t *slice_md5_out, uint32_t *slice_crc_out,
                         uint8_t *pkt_md5_out, void *stream);
 
+/* p2g_dev_create_pass without any allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops).  Same arguments and results as
+ * p2g_dev_create_pass for the sets one fused pass holds; everything beyond
+ * the caller's inputs and outputs (slice descriptors built on the device,
+ * tail blocks, zero-padded copies of partial slices, coefficient fragments,
+ * sync words) lives in one static 64 MiB device array per GPU, kept until
+ * the process ends.  Limits: at most 64 files, 1..2048 slices, R >= 1, and
+ * a scratch need within 64 MiB (coefficient fragments take
+ * ceil(R / 16) * ceil(slices / 16) * 8 KiB; the 1027-slice, R = 200
+ * headline set takes 28 MB); larger sets fail with P2G_ERR_INVALID_ARGUMENT
+ * and a message naming the limit.  The call makes no hipMalloc, hipFree or
+ * host staging copy.  Calls on one GPU share the array: calls of one
+ * context are ordered across its streams, calls of different contexts (or
+ * processes' libraries) that overlap on one GPU race.  p2g_synchronize
+ * reports a pass that could not complete. */
+int p2g_dev_create_pass_noalloc(p2g_ctx *ctx, const p2g_dev_file *files, uint32_t nfiles,
+                                uint64_t slice_size, uint32_t recovery_count,
+                                const uint8_t set_id[16], uint8_t *rec, uint8_t *slice_md5_out,
+                                uint32_t *slice_crc_out, uint8_t *pkt_md5_out, void *stream);
+
 /* Diagnostics, no reference counterpart: after the last p2g_dev_create_pass
  * of this context has finished (it waits for that), the first start and the
  * last end of its packet-MD5, slice-MD5, encode and CRC work, in ms from the

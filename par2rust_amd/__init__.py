@@ -47,7 +47,7 @@ EXPORTED_SYMBOLS = [
     "p2g_server_open", "p2g_server_submit", "p2g_server_poll", "p2g_server_result",
     "p2g_server_release", "p2g_server_stats", "p2g_server_close",
     "p2g_tune_bucket", "p2g_autotune", "p2g_tune_get", "p2g_tune_save", "p2g_tune_load",
-    "p2g_plan_roles",
+    "p2g_plan_roles", "p2g_dev_create_pass_noalloc",
 ]
 
 VARIANT_MFMA, VARIANT_BITSLICE = 0, 1
@@ -134,6 +134,8 @@ def _load() -> ctypes.CDLL:
                                                        ctypes.c_uint32, P, P, P]),
         "p2g_dev_create_pass": (ctypes.c_int, [P, P, ctypes.c_uint32, ctypes.c_uint64,
                                                ctypes.c_uint32, P, P, P, P, P, P]),
+        "p2g_dev_create_pass_noalloc": (ctypes.c_int, [P, P, ctypes.c_uint32, ctypes.c_uint64,
+                                                       ctypes.c_uint32, P, P, P, P, P, P]),
         "p2g_pass_timing": (ctypes.c_int, [P, ctypes.POINTER(ctypes.c_double)]),
         "p2g_synchronize": (ctypes.c_int, [P]),
         "p2g_batch_plan": (ctypes.c_int, [P, P, ctypes.c_uint32, ctypes.c_int, ctypes.POINTER(P)]),
@@ -552,6 +554,20 @@ class Context:
                                             ctypes.c_void_p(_dev_ptr(slice_crc_out)),
                                             ctypes.c_void_p(_dev_ptr(pkt_md5_out)),
                                             ctypes.c_void_p(stream) if stream else None))
+
+    def dev_create_pass_noalloc(self, files, slice_size: int, recovery_count: int, set_id: bytes,
+                                rec, slice_md5_out, slice_crc_out, pkt_md5_out,
+                                stream: Optional[int] = None):
+        """dev_create_pass without any allocation (p2g_dev_create_pass_noalloc):
+        scratch in the library's static device array; at most 64 files,
+        2048 slices and 64 MiB of scratch."""
+        arr, items = _dev_files(files)
+        sid = ctypes.create_string_buffer(bytes(set_id), 16)
+        self._check(lib.p2g_dev_create_pass_noalloc(
+            self.h, arr, len(items), slice_size, recovery_count, sid,
+            ctypes.c_void_p(_dev_ptr(rec)), ctypes.c_void_p(_dev_ptr(slice_md5_out)),
+            ctypes.c_void_p(_dev_ptr(slice_crc_out)), ctypes.c_void_p(_dev_ptr(pkt_md5_out)),
+            ctypes.c_void_p(stream) if stream else None))
 
     def dev_create_batch(self, jobs: Sequence[BatchJob], stream: Optional[int] = None):
         """Every job's dev_create_pass work in shared persistent launches

@@ -18,6 +18,7 @@ OUT = os.path.join(HERE, "libpar2gpu.so")
 # (source, language)
 SOURCES = [("p2g_kernels.hip", "hip"), ("p2g_abi.hip", "hip"), ("p2g_batch.cpp", "hip"),
            ("p2g_serve.cpp", "hip"), ("p2g_tune.cpp", "hip"), ("p2g_plan.cpp", "hip"),
+           ("p2g_noalloc.hip", "hip"),
            ("par2_create.cpp", "hip"), ("par2_scheme.cpp", "hip")]
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-result", "-Wno-unused-value"]

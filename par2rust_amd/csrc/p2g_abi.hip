@@ -105,7 +105,10 @@ int Ctx::open(int dev) {
     if (e == hipSuccess) e = hipEventCreateWithFlags(&sum_descs.free, hipEventDisableTiming);
     for (hipEvent_t &ev : scratch_done)
         if (e == hipSuccess) e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&noalloc_done, hipEventDisableTiming);
     if (e != hipSuccess) return fail(P2G_ERR_DEVICE, "hipEventCreate", e);
+    if (noalloc_scratch_init(dev, &noalloc_scratch) != P2G_OK)
+        return fail(P2G_ERR_DEVICE, "address of the allocation-free scratch array");
     build_gf_tables(h_log, h_antilog);
     base_logs = input_base_logs();
     int cus = 0;
@@ -163,6 +166,7 @@ Ctx::~Ctx() {
     d_xs_jumps.release();
     for (hipEvent_t ev : scratch_done)
         if (ev) hipEventDestroy(ev);
+    if (noalloc_done) hipEventDestroy(noalloc_done);
     for (DescSlot *sl : {&enc_descs, &sum_descs}) {
         sl->h.release();
         sl->d.release();

@@ -78,6 +78,10 @@ struct Ctx {
     DevBuf d_mats, d_md5, d_crc, d_stage;
     DevBuf d_sync;                 // fused-pass stamps and band counts
     DevBuf d_err;                  // fused-pass sticky error word (read and cleared by p2g_synchronize)
+    // p2g_dev_create_pass_noalloc: the device address of its static scratch
+    // array on this device, and the end of this context's last use of it
+    uint8_t *noalloc_scratch = nullptr;
+    hipEvent_t noalloc_done = nullptr;
     DevBuf d_partials;             // zero-padded copies of partial slices (fused pass)
     DevBuf d_xs_jumps;             // xorshift64 jump matrices (synthetic input)
     bool pass_pending = false;     // a fused pass may have set error flags not yet read
@@ -119,6 +123,11 @@ struct Job {
 };
 
 int check_slice_size(Ctx *c, uint64_t S);
+
+// The device address of p2g_dev_create_pass_noalloc's static scratch array
+// on `device` (the current device), looked up once per device
+// (p2g_noalloc.hip).
+int noalloc_scratch_init(int device, uint8_t **out);
 
 // Batched create plans (p2g_batch.cpp).  An item is recovery rows [r0, r1)
 // of a job and the checksums of its slices [s0, s1) (s1 past the job's
