@@ -75,14 +75,17 @@ hipError_t launch_encode_mfma(const SliceDesc *d_inputs, uint32_t n_inputs, cons
 // Fused create pass (encode + slice MD5 + packet MD5, one launch).  The
 // sync words (cleared by each launch): [kPassFlags] unused; from kPassTimes, 2 kPassRoles u64
 // real-time counter stamps (100 MHz): the first start and the last end of
-// each role's work (packet MD5, slice MD5, encode, CRC); [kPassBands + b]
+// each role's work (packet MD5, slice MD5, encode, CRC); the per-XCD work
+// queue counters from kPassQueues; [kPassBands + b]
 // finished units of column band b; [kPassBands + n_bands + b] the first
 // column group of band b (n_bands + 1 entries).  Errors go to a separate
 // sticky word (kPassErrWait bits) that no launch clears: the host reads and
 // clears it when it waits, so an error of an earlier pass queued before the
 // wait is not lost to a later pass's reset.
 constexpr uint32_t kPassFlags = 0, kPassTimes = 2, kPassRoles = 4;
-constexpr uint32_t kPassBands = kPassTimes + 4 * kPassRoles;
+// [kPassQueues + x]: units taken so far from the work queue of XCD x (8)
+constexpr uint32_t kPassQueues = kPassTimes + 4 * kPassRoles, kPassXcds = 8;
+constexpr uint32_t kPassBands = kPassQueues + kPassXcds;
 enum { kPassPacket = 0, kPassSlice = 1, kPassEncode = 2, kPassCrc = 3 };
 constexpr uint32_t kPassErrWait = 1u;  // a packet lane stopped waiting for a band
 // Width (in 512-byte column groups) of the next band after `done` groups,

@@ -1486,6 +1486,51 @@ struct PassArgs {
     CrcPowers pw;
 };
 
+// Device work queues of the encode units.  Workgroups are dealt to the 8
+// XCDs round-robin (XCD = blockIdx % 8), and units are taken in rounds of E
+// (the encode workgroups): round r holds units [r E, (r + 1) E), of which
+// XCD x owns cnt_x slots (cnt_x = its encode workgroups).  Inside an XCD the
+// units are not pre-assigned: each of its workgroups takes the XCD's next
+// slot from a counter when it is free, so faster workgroups take more units
+// and a unit that costs more delays only the workgroup that took it.  Queue
+// slots increase, so each workgroup's units come in increasing order (the
+// band walks rely on it).  Which slots of a round an XCD owns:
+//  * contiguous: [off_x, off_x + cnt_x) (off_x = encode workgroups of lower
+//    XCDs), so units adjacent in order (one row block's column groups,
+//    which share A fragments) run on one XCD and share its L2 copy of them;
+//    the batched pass, whose small-slice jobs stream large A fragments
+//    through few column groups;
+//  * strided: x, x + 8, x + 16, ..., the units workgroup index order deals
+//    to XCD x, which puts the row blocks of a column group that meet in one
+//    round (bands 64 column groups wide) on one XCD; the fused pass, whose
+//    source columns cost as much as its A fragments.
+struct UnitQueue {
+    uint32_t *ctr;
+    uint32_t e_blocks, cnt, off, step;
+    __device__ __forceinline__ UnitQueue(uint32_t *sync, uint32_t blk, uint32_t e, bool contiguous) {
+        const uint32_t x = blk % kPassXcds;
+        ctr = sync + kPassQueues + x;
+        e_blocks = e;
+        cnt = e / kPassXcds + (x < e % kPassXcds ? 1u : 0u);  // >= 1: this workgroup is one
+        off = contiguous ? x * (e / kPassXcds) + min(x, e % kPassXcds) : x;
+        step = contiguous ? 1u : kPassXcds;
+    }
+    // The next unit for the calling workgroup (all threads call it): thread
+    // 0 takes a slot and broadcasts it through `slot`; readfirstlane makes the
+    // value a scalar, so every branch on it is uniform and every thread
+    // reaches the barriers of the unit loop.
+    __device__ __forceinline__ uint32_t next(uint32_t *slot) const {
+        if (threadIdx.x == 0) {
+            const uint32_t k = __hip_atomic_fetch_add(ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            *slot = (k / cnt) * e_blocks + off + step * (k % cnt);
+        }
+        __syncthreads();
+        const uint32_t u = __builtin_amdgcn_readfirstlane(*(volatile uint32_t *)slot);
+        __syncthreads();  // `slot` is free for the next call
+        return u;
+    }
+};
+
 // Waits until *cnt >= target, then acquires at agent scope.  The count is
 // read wave-uniformly.  Returns false (and sets kPassErrWait) when it has not
 // moved for stall_ticks of the 100 MHz real-time counter, so a wait always
@@ -1602,23 +1647,22 @@ __global__ __launch_bounds__(kMfThreads, 1) void k_create_pass(PassArgs a) {
     if (tid == 0) pass_stamp_start(a, kPassEncode);
     mf_load_desc<FULL>(lds.desc, a.inputs, a.n, a.nq);
     __syncthreads();  // descriptor table ready
-    // Encode workgroup e takes units e, e + E, e + 2E, ... (E encode
-    // workgroups): all units of a kind cost the same, the order stays band by
-    // band, and the unit index is a scalar every thread computes, so the loop
-    // has no divergent exit around its barriers (a unit index fetched by one
-    // thread and broadcast through LDS let the compiler loop the other lanes
-    // back to the barrier without that thread).
+    // Units come from the per-XCD work queues (strided slots), band by band
+    // (the queues hand out units in increasing order).
+    __shared__ uint32_t s_unit;
     const uint32_t e_blocks = gridDim.x - a.h3_blocks - a.h2_blocks;
+    const UnitQueue queue(a.sync, blockIdx.x, e_blocks, false);
     // Band b holds units [row_blocks * bstart[b], row_blocks * bstart[b + 1]).
     const uint32_t *bstart = a.sync + kPassBands + a.n_bands;
     uint32_t band = 0, b_lo = 0, b_hi = a.row_blocks * bstart[1];
     // Finished units are counted per band and published once per band: when
-    // the next unit of this workgroup lies in another band, or after the last
-    // one.  Publishing = every thread's stores released to the workgroup (they
+    // this workgroup's next unit lies in another band, or there is none.
+    // Publishing = every thread's stores released to the workgroup (they
     // reach L2), a barrier, then one agent-scope release (L2 write-back) and
     // one add by thread 0.
     uint32_t done = 0;  // units of band `band` finished and not yet counted
-    for (uint32_t u = role.index; u < a.n_units; u += e_blocks) {
+    uint32_t u = queue.next(&s_unit);
+    while (u < a.n_units) {
         while (u >= b_hi) {
             band++;
             b_lo = b_hi;
@@ -1633,7 +1677,7 @@ __global__ __launch_bounds__(kMfThreads, 1) void k_create_pass(PassArgs a) {
             mf_unit<FULL>(lds, a.inputs, a.afrag, a.nq, a.nq, a.R, a.S, a.rec, 0, y * kMfTiles,
                           x * (kMfThreads * 2));
         done++;
-        const uint32_t nxt = u + e_blocks;
+        const uint32_t nxt = queue.next(&s_unit);
         if (nxt >= a.n_units || nxt >= b_hi) {
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
             __syncthreads();
@@ -1645,6 +1689,7 @@ __global__ __launch_bounds__(kMfThreads, 1) void k_create_pass(PassArgs a) {
             }
             done = 0;
         }
+        u = nxt;
     }
 }
 
@@ -1664,7 +1709,7 @@ __global__ void k_pass_reset(uint32_t *sync, uint32_t n_bands, uint32_t col_grou
                              uint32_t band_cols) {
     const uint32_t words = kPassBands + n_bands;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < words; i += gridDim.x * blockDim.x)
-        sync[i] = (i >= kPassTimes && i < kPassBands && ((i - kPassTimes) / 2) % 2 == 0)
+        sync[i] = (i >= kPassTimes && i < kPassQueues && ((i - kPassTimes) / 2) % 2 == 0)
                       ? 0xFFFFFFFFu
                       : 0u;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -1944,19 +1989,21 @@ __global__ __launch_bounds__(kMfThreads, 1) void k_create_batch(BatchArgs a) {
     // reached by the whole workgroup.
     const uint32_t e_blocks = gridDim.x - a.h3_blocks - a.h2_blocks;
     if (tid == 0) batch_role(a.sync, kPassEncode, false);
-    // Workgroups are dealt to the 8 XCDs round-robin (XCD = blk % 8).  Each
-    // round of e_blocks units is taken in XCD order: this workgroup's rank
-    // is the encode workgroups [0, e_blocks) of lower XCDs, then those of
-    // its own XCD before it, so units adjacent in order (one row block's
-    // column groups) run on one XCD and share its L2.
-    uint32_t rank = 0;
-    for (uint32_t b = 0; b < e_blocks; b++)
-        if (b % 8 < blk % 8 || (b % 8 == blk % 8 && b < blk)) rank++;
+    // Units come from the per-XCD work queues (UnitQueue, contiguous slots):
+    // the units of one round that fall to an XCD (one row block's column groups) run on that
+    // XCD and share its L2 copy of the row block's A fragments, and inside
+    // an XCD a workgroup takes the next unit when it is free, so units of
+    // different jobs' sizes even out.  Every value the loop branches on is a
+    // scalar (the unit from the queue, the tables), so the barriers inside
+    // are reached by the whole workgroup.
+    __shared__ uint32_t s_unit;
+    const UnitQueue queue(a.sync, blk, e_blocks, true);
     uint32_t pi = 0;
     const SliceDesc *loaded = nullptr;  // first descriptor of the chunk in lds.desc
     uint32_t band = 0, b_lo = 0, b_hi = 0, unit_end = 0;
     uint32_t done = 0;
-    for (uint32_t u = rank; u < a.n_units; u += e_blocks) {
+    uint32_t u = queue.next(&s_unit);
+    while (u < a.n_units) {
         if (u >= unit_end) {
             while (u >= a.pieces[pi].unit_end) pi++;
             const BatchPiece &P = a.pieces[pi];
@@ -1989,7 +2036,7 @@ __global__ __launch_bounds__(kMfThreads, 1) void k_create_batch(BatchArgs a) {
                           k0 > 0, y * kMfTiles, x * (kMfThreads * 2));
         }
         done++;
-        const uint32_t nxt = u + e_blocks;
+        const uint32_t nxt = queue.next(&s_unit);
         if (nxt >= a.n_units || nxt >= b_hi) {
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
             __syncthreads();
@@ -2003,6 +2050,7 @@ __global__ __launch_bounds__(kMfThreads, 1) void k_create_batch(BatchArgs a) {
             }
             done = 0;
         }
+        u = nxt;
     }
 }
 
@@ -2015,7 +2063,7 @@ __global__ void k_batch_reset(uint32_t *sync, uint32_t words, const BatchPiece *
                               int first) {
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x, gsz = gridDim.x * blockDim.x;
     for (uint32_t i = gid; i < words; i += gsz)
-        sync[i] = (i >= kPassTimes && i < kPassBands && ((i - kPassTimes) / 2) % 2 == 0) ? ~0u : 0u;
+        sync[i] = (i >= kPassTimes && i < kPassQueues && ((i - kPassTimes) / 2) % 2 == 0) ? ~0u : 0u;
     if (first)
         for (uint32_t j = gid; j <= n_jobs; j += gsz) job_done[j] = j == n_jobs ? ~0ull : 0ull;
     for (uint32_t p = gid; p < n_pieces; p += gsz) {
