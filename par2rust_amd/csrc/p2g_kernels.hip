@@ -1492,7 +1492,8 @@ struct PassArgs {
 // XCD x owns cnt_x slots (cnt_x = its encode workgroups).  Inside an XCD the
 // units are not pre-assigned: each of its workgroups takes the XCD's next
 // slot from a counter when it is free, so faster workgroups take more units
-// and a unit that costs more delays only the workgroup that took it.  Queue
+// and a unit that costs more delays only the workgroup that took it; a
+// slice-hash workgroup whose hashes are done takes units too.  Queue
 // slots increase, so each workgroup's units come in increasing order (the
 // band walks rely on it).  Which slots of a round an XCD owns:
 //  * contiguous: [off_x, off_x + cnt_x) (off_x = encode workgroups of lower
@@ -1511,7 +1512,9 @@ struct UnitQueue {
         const uint32_t x = blk % kPassXcds;
         ctr = sync + kPassQueues + x;
         e_blocks = e;
-        cnt = e / kPassXcds + (x < e % kPassXcds ? 1u : 0u);  // >= 1: this workgroup is one
+        // 0 only on an XCD without encode workgroups (grids of fewer than 8),
+        // whose hash workgroups then find no units
+        cnt = e / kPassXcds + (x < e % kPassXcds ? 1u : 0u);
         off = contiguous ? x * (e / kPassXcds) + min(x, e % kPassXcds) : x;
         step = contiguous ? 1u : kPassXcds;
     }
@@ -1520,6 +1523,7 @@ struct UnitQueue {
     // value a scalar, so every branch on it is uniform and every thread
     // reaches the barriers of the unit loop.
     __device__ __forceinline__ uint32_t next(uint32_t *slot) const {
+        if (cnt == 0) return 0xFFFFFFFFu;
         if (threadIdx.x == 0) {
             const uint32_t k = __hip_atomic_fetch_add(ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             *slot = (k / cnt) * e_blocks + off + step * (k % cnt);
@@ -1615,34 +1619,36 @@ __global__ __launch_bounds__(kMfThreads, 1) void k_create_pass(PassArgs a) {
     // SIMDs, take the slice CRCs, one slice per wave, with their tables in the
     // LDS the encode would use; the packet-hash workgroups keep their CU's
     // memory path to themselves (they must keep pace with the encode).
-    if (role.role == kPassPacket && tid >= 64) return;
-    if (role.role == kPassSlice && tid >= 64) {
-        const uint32_t cw = role.index * 3 + (tid >> 6) - 1, n_cw = 3 * a.h2_blocks;
-        uint32_t(*t)[256] = reinterpret_cast<uint32_t(*)[256]>(lds.desc) + 4 * ((tid >> 6) - 1);
-        crc_tables_init_wave(t);
-        if (tid % 64 == 0) pass_stamp_start(a, kPassCrc);
-        for (uint32_t sl = cw; sl < a.n; sl += n_cw)
-            crc_slice_wave(t, a.inputs[sl], a.S, a.S % 16 == 0, a.pw, a.slice_crc + sl);
-        if (tid % 64 == 0) pass_stamp_end(a, kPassCrc);
-        return;
-    }
     if (role.role == kPassPacket) {
+        if (tid >= 64) return;
         if (tid == 0) pass_stamp_start(a, kPassPacket);
         for (uint32_t r = role.index * 64 + tid; r < a.R; r += a.h3_blocks * 64) pass_packet_lane(a, r);
         if (tid == 0) pass_stamp_end(a, kPassPacket);
         return;
     }
     if (role.role == kPassSlice) {
-        if (tid == 0) pass_stamp_start(a, kPassSlice);
-        const uint32_t hb = role.index;
-        for (uint32_t s = hb * 64 + tid; s < a.n; s += a.h2_blocks * 64) {
-            // slices start 16-byte aligned when S is a multiple of 16 (file bases are)
-            const uint4 *tb = a.tail_blocks + 4 * (size_t)s;
-            if (a.S % 16 == 0) slice_md5_lane<true>(a.inputs[s], tb, a.S, a.slice_md5 + (size_t)s * 16);
-            else slice_md5_lane<false>(a.inputs[s], tb, a.S, a.slice_md5 + (size_t)s * 16);
+        if (tid >= 64) {
+            const uint32_t cw = role.index * 3 + (tid >> 6) - 1, n_cw = 3 * a.h2_blocks;
+            uint32_t(*t)[256] = reinterpret_cast<uint32_t(*)[256]>(lds.desc) + 4 * ((tid >> 6) - 1);
+            crc_tables_init_wave(t);
+            if (tid % 64 == 0) pass_stamp_start(a, kPassCrc);
+            for (uint32_t sl = cw; sl < a.n; sl += n_cw)
+                crc_slice_wave(t, a.inputs[sl], a.S, a.S % 16 == 0, a.pw, a.slice_crc + sl);
+            if (tid % 64 == 0) pass_stamp_end(a, kPassCrc);
+        } else {
+            if (tid == 0) pass_stamp_start(a, kPassSlice);
+            const uint32_t hb = role.index;
+            for (uint32_t s = hb * 64 + tid; s < a.n; s += a.h2_blocks * 64) {
+                // slices start 16-byte aligned when S is a multiple of 16 (file bases are)
+                const uint4 *tb = a.tail_blocks + 4 * (size_t)s;
+                if (a.S % 16 == 0) slice_md5_lane<true>(a.inputs[s], tb, a.S, a.slice_md5 + (size_t)s * 16);
+                else slice_md5_lane<false>(a.inputs[s], tb, a.S, a.slice_md5 + (size_t)s * 16);
+            }
+            if (tid == 0) pass_stamp_end(a, kPassSlice);
         }
-        if (tid == 0) pass_stamp_end(a, kPassSlice);
-        return;
+        // Its slice hashes done, the workgroup encodes what is left in its
+        // XCD's queue (the CRC tables in lds.desc are not needed any more).
+        __syncthreads();
     }
     if (tid == 0) pass_stamp_start(a, kPassEncode);
     mf_load_desc<FULL>(lds.desc, a.inputs, a.n, a.nq);
@@ -1958,10 +1964,9 @@ __global__ __launch_bounds__(kMfThreads, 1) void k_create_batch(BatchArgs a) {
         if (lane == 0) batch_role(a.sync, kPassPacket, true);
         return;
     }
-    if (role.role == kPassSlice) {
+    if (role.role == kPassSlice && wave < 2) {
         // slice MD5 + CRC-32 lanes on waves 0 and 1 (CRC tables per wave in
         // lds.desc); a lane walks the slice jobs on its own
-        if (wave >= 2) return;
         const uint32_t hb = role.index;
         static_assert(sizeof(BatchLds::desc) >= 2 * 16 * 256 * 4, "two waves of 16 CRC tables");
         uint32_t(*t)[256] = reinterpret_cast<uint32_t(*)[256]>(lds.desc) + 16 * wave;
@@ -1982,16 +1987,19 @@ __global__ __launch_bounds__(kMfThreads, 1) void k_create_batch(BatchArgs a) {
             if (nxt >= a.slice_hi || nxt >= J.slice_end) batch_stamp(a.job_done + J.job);
         }
         if (lane == 0) batch_role(a.sync, kPassSlice, true);
-        return;
     }
-    // Encode.  Every value the loop branches on is computed alike by all
-    // threads from the tables (no broadcast), so the barriers inside are
-    // reached by the whole workgroup.
+    // A slice workgroup whose hashes are done encodes what is left in its
+    // XCD's queue, as the encode workgroups do (waves 2 and 3 wait here for
+    // waves 0 and 1).
+    if (role.role == kPassSlice) __syncthreads();
+
+    // Encode.
     const uint32_t e_blocks = gridDim.x - a.h3_blocks - a.h2_blocks;
     if (tid == 0) batch_role(a.sync, kPassEncode, false);
     // Units come from the per-XCD work queues (UnitQueue, contiguous slots):
-    // the units of one round that fall to an XCD (one row block's column groups) run on that
-    // XCD and share its L2 copy of the row block's A fragments, and inside
+    // the units of one round that fall to an XCD (one row block's column
+    // groups) run on that XCD and share its L2 copy of the row block's A
+    // fragments, and inside
     // an XCD a workgroup takes the next unit when it is free, so units of
     // different jobs' sizes even out.  Every value the loop branches on is a
     // scalar (the unit from the queue, the tables), so the barriers inside
