@@ -290,26 +290,36 @@ void p2g_server_close(p2g_server *s);
 /* ---------------------------------------------------------------------
  * Autotuner (no reference counterpart; the reference picks its SIMD kernel
  * by CPU feature, src/galois_simd.rs:605-636).  Per (N, R, slice size)
- * bucket -- the bit lengths of the three -- the encode variant
- * (P2G_VARIANT_MFMA: matrix-core fused pass; P2G_VARIANT_BITSLICE: the
- * bit-sliced VALU kernels) and the column band width of the fused pass that
- * ran fastest.  p2g_dev_create_pass and the batched planner use a tuned
- * bucket's choice; with P2G_AUTOTUNE=1 a bucket is tuned on its first
- * p2g_dev_create_pass call (on that call's data); P2G_TUNE_CACHE names a
- * file the cache is loaded from at p2g_open and saved to after each tuning.
+ * bucket -- the bit lengths of the three -- the create-pass variant
+ * (P2G_VARIANT_MFMA: matrix-core fused pass; P2G_VARIANT_BITSLICE /
+ * P2G_VARIANT_NIBBLE: separate launches with the bit-sliced or the
+ * split-nibble byte-permute VALU encode) and the column band width of the
+ * fused pass that ran fastest, and the fastest encode kernel alone (mfma,
+ * bitslice or nibble), which p2g_dev_encode and p2g_encode_push use.
+ * p2g_dev_create_pass and the batched planner use a tuned bucket's choice;
+ * with P2G_AUTOTUNE=1 a bucket is tuned on its first p2g_dev_create_pass
+ * call (on that call's data; the create-pass choice only); P2G_TUNE_CACHE
+ * names a file the cache is loaded from at p2g_open and saved to after each
+ * tuning.
  * ------------------------------------------------------------------- */
-enum { P2G_VARIANT_MFMA = 0, P2G_VARIANT_BITSLICE = 1 };
+enum { P2G_VARIANT_MFMA = 0, P2G_VARIANT_BITSLICE = 1, P2G_VARIANT_NIBBLE = 2 };
 
 /* The bucket id of a shape: nbits(slices) | nbits(R) << 8 | nbits(S) << 16. */
 uint32_t p2g_tune_bucket(uint64_t slices, uint32_t recovery_count, uint64_t slice_size);
 /* Tunes the bucket of (slices, recovery_count, slice_size) on synthetic
- * input of that shape and caches the choice; the winner's variant, band
- * width (0: default) and time in ms are returned when the pointers are set. */
+ * input of that shape -- the encode kernels alone, then the create-pass
+ * candidates -- and caches both choices; the create-pass winner's variant,
+ * band width (0: default) and time in ms are returned when the pointers are
+ * set (p2g_tune_get_encode gives the encode kernel's). */
 int p2g_autotune(p2g_ctx *ctx, uint32_t slices, uint32_t recovery_count, uint64_t slice_size,
                  int *variant_out, uint32_t *band_cols_out, double *ms_out);
 /* The choice p2g_dev_create_pass would use for the shape now. */
 int p2g_tune_get(p2g_ctx *ctx, uint32_t slices, uint32_t recovery_count, uint64_t slice_size,
                  int *variant_out, uint32_t *band_cols_out);
+/* The encode kernel (P2G_VARIANT_*) p2g_dev_encode and p2g_encode_push use
+ * for the shape now, and its tuned time in ms (0 when not tuned). */
+int p2g_tune_get_encode(p2g_ctx *ctx, uint32_t slices, uint32_t recovery_count,
+                        uint64_t slice_size, int *variant_out, double *ms_out);
 int p2g_tune_save(p2g_ctx *ctx, const char *path);
 int p2g_tune_load(p2g_ctx *ctx, const char *path);
 

@@ -47,10 +47,10 @@ EXPORTED_SYMBOLS = [
     "p2g_server_open", "p2g_server_submit", "p2g_server_poll", "p2g_server_result",
     "p2g_server_release", "p2g_server_stats", "p2g_server_close",
     "p2g_tune_bucket", "p2g_autotune", "p2g_tune_get", "p2g_tune_save", "p2g_tune_load",
-    "p2g_plan_roles", "p2g_dev_create_pass_noalloc",
+    "p2g_plan_roles", "p2g_dev_create_pass_noalloc", "p2g_tune_get_encode",
 ]
 
-VARIANT_MFMA, VARIANT_BITSLICE = 0, 1
+VARIANT_MFMA, VARIANT_BITSLICE, VARIANT_NIBBLE = 0, 1, 2
 
 
 class Par2Error(RuntimeError):
@@ -166,6 +166,9 @@ def _load() -> ctypes.CDLL:
         "p2g_tune_get": (ctypes.c_int, [P, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64,
                                         ctypes.POINTER(ctypes.c_int),
                                         ctypes.POINTER(ctypes.c_uint32)]),
+        "p2g_tune_get_encode": (ctypes.c_int, [P, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64,
+                                               ctypes.POINTER(ctypes.c_int),
+                                               ctypes.POINTER(ctypes.c_double)]),
         "p2g_tune_save": (ctypes.c_int, [P, ctypes.c_char_p]),
         "p2g_tune_load": (ctypes.c_int, [P, ctypes.c_char_p]),
         "p2g_run_create": (ctypes.c_int, [P, ctypes.c_char_p, ctypes.c_uint64, ctypes.c_uint32,
@@ -593,6 +596,13 @@ class Context:
         self._check(lib.p2g_tune_get(self.h, slices, recovery_count, slice_size, ctypes.byref(v),
                                      ctypes.byref(b)))
         return {"variant": v.value, "band_cols": b.value}
+
+    def tune_get_encode(self, slices: int, recovery_count: int, slice_size: int) -> dict:
+        """The encode kernel dev_encode / encode_push use for the shape now."""
+        v, ms = ctypes.c_int(), ctypes.c_double()
+        self._check(lib.p2g_tune_get_encode(self.h, slices, recovery_count, slice_size,
+                                            ctypes.byref(v), ctypes.byref(ms)))
+        return {"variant": v.value, "ms": ms.value}
 
     def tune_save(self, path: str):
         self._check(lib.p2g_tune_save(self.h, os.fsencode(path)))

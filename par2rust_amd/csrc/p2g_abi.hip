@@ -134,6 +134,7 @@ int Ctx::open(int dev) {
     }
     if (const char *v = getenv("P2G_ENCODE")) {
         if (!strcmp(v, "bitslice")) encode_variant = kEncodeBitslice;
+        else if (!strcmp(v, "nibble")) encode_variant = kEncodeNibble;
         else if (!strcmp(v, "mfma")) encode_variant = kEncodeMfma;
     }
     if (d_antilog.reserve(65536 * 2) != P2G_OK) return fail(P2G_ERR_OUT_OF_MEMORY, "antilog table");
@@ -246,7 +247,12 @@ int upload_descs(Ctx *c, DescSlot &slot, const std::vector<SliceDesc> &descs, hi
 int dev_encode(Ctx *c, const SliceDesc *d_descs, uint32_t n, uint64_t S, uint32_t first_input,
                uint32_t total_inputs, uint32_t R, uint8_t *d_rec, int accumulate,
                hipStream_t st, int variant) {
-    if (variant < 0) variant = c->encode_variant;
+    if (variant < 0) {
+        // the bucket's tuned encode kernel, else the context's
+        TuneChoice ch;
+        variant = tune_lookup(c, total_inputs, R, S, &ch) && ch.enc_variant >= 0 ? ch.enc_variant
+                                                                                 : c->encode_variant;
+    }
     if (R > P2G_MAX_RECOVERY_BLOCKS)
         return c->fail(P2G_ERR_TOO_MANY_RECOVERY_BLOCKS, "recovery count above 65535");
     if (total_inputs > kMaxInputs)
@@ -281,6 +287,24 @@ int dev_encode(Ctx *c, const SliceDesc *d_descs, uint32_t n, uint64_t S, uint32_
         }
         return P2G_OK;
     }
+    if (variant == kEncodeNibble) {
+        // Inputs go in chunks so the byte tables (96 B per (r, i)) stay
+        // within kMatsBudget.
+        uint64_t ch = kMatsBudget / ((uint64_t)R * kNibbleTableBytes);
+        ch = std::max<uint64_t>(1, std::min<uint64_t>(ch, n));
+        if (c->d_mats.reserve((size_t)ch * R * kNibbleTableBytes) != P2G_OK)
+            return c->fail(P2G_ERR_OUT_OF_MEMORY, "split-nibble tables");
+        for (uint64_t c0 = 0; c0 < n; c0 += ch) {
+            const uint32_t cn = (uint32_t)std::min<uint64_t>(ch, n - c0);
+            uint32_t *tables = static_cast<uint32_t *>(c->d_mats.ptr);
+            e = launch_nibble_tables(dalog, dlog + c0, cn, R, tables, st);
+            if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "split-nibble table kernel", e);
+            e = launch_encode_nibble(d_descs + c0, cn, tables, R, (uint32_t)S, d_rec,
+                                     (c0 > 0 || accumulate) ? 1 : 0, st);
+            if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "encode kernel", e);
+        }
+        return P2G_OK;
+    }
     e = launch_desc_tails(const_cast<SliceDesc *>(d_descs), n, nullptr, st);
     if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "descriptor tail kernel", e);
     // Matrix-core form: A fragments (1 KiB per 32 rows x 4 inputs) for a chunk
@@ -300,6 +324,31 @@ int dev_encode(Ctx *c, const SliceDesc *d_descs, uint32_t n, uint64_t S, uint32_
         if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "encode kernel", e);
     }
     return P2G_OK;
+}
+
+int dev_encode_files(Ctx *c, const p2g_dev_file *files, uint32_t nfiles, uint64_t slice_size,
+                     uint32_t first_input_index, uint32_t total_inputs, uint32_t recovery_count,
+                     uint8_t *rec, int accumulate, hipStream_t st, int variant) {
+    if ((nfiles && !files) || (recovery_count && !rec))
+        return c->fail(P2G_ERR_INVALID_ARGUMENT, "null argument");
+    if (reinterpret_cast<uintptr_t>(rec) % 4 != 0)
+        return c->fail(P2G_ERR_INVALID_ARGUMENT, "recovery buffer must be 4-byte aligned");
+    int rc = check_slice_size(c, slice_size);
+    if (rc != P2G_OK) return rc;
+    hipSetDevice(c->device);
+    std::vector<SliceDesc> descs;
+    rc = append_descs(c, files, nfiles, slice_size, descs);
+    if (rc != P2G_OK) return rc;
+    if (descs.size() > kMaxInputs)
+        return c->fail(P2G_ERR_TOO_MANY_INPUT_BLOCKS, "more than 32768 input slices");
+    rc = c->scratch_acquire(Ctx::kScratchEnc, st);
+    if (rc == P2G_OK) rc = upload_descs(c, c->enc_descs, descs, st);
+    if (rc == P2G_OK)
+        rc = dev_encode(c, static_cast<const SliceDesc *>(c->enc_descs.d.ptr), (uint32_t)descs.size(),
+                        slice_size, first_input_index, total_inputs, recovery_count, rec, accumulate,
+                        st, variant);
+    if (rc != P2G_OK) return rc;
+    return c->scratch_release(Ctx::kScratchEnc, st);
 }
 
 }  // namespace p2g
@@ -426,27 +475,9 @@ int p2g_dev_encode(p2g_ctx *ctx, const p2g_dev_file *files, uint32_t nfiles, uin
     return guarded(reinterpret_cast<Ctx *>(ctx), [&]() -> int {
         Ctx *c = reinterpret_cast<Ctx *>(ctx);
         if (!c) return P2G_ERR_INVALID_ARGUMENT;
-        if ((nfiles && !files) || (recovery_count && !rec))
-            return c->fail(P2G_ERR_INVALID_ARGUMENT, "null argument");
-        if (reinterpret_cast<uintptr_t>(rec) % 4 != 0)
-            return c->fail(P2G_ERR_INVALID_ARGUMENT, "recovery buffer must be 4-byte aligned");
-        int rc = check_slice_size(c, slice_size);
-        if (rc != P2G_OK) return rc;
-        hipSetDevice(c->device);
-        std::vector<SliceDesc> descs;
-        rc = append_descs(c, files, nfiles, slice_size, descs);
-        if (rc != P2G_OK) return rc;
-        if (descs.size() > kMaxInputs)
-            return c->fail(P2G_ERR_TOO_MANY_INPUT_BLOCKS, "more than 32768 input slices");
-        hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c->stream;
-        rc = c->scratch_acquire(Ctx::kScratchEnc, st);
-        if (rc == P2G_OK) rc = upload_descs(c, c->enc_descs, descs, st);
-        if (rc == P2G_OK)
-            rc = dev_encode(c, static_cast<const SliceDesc *>(c->enc_descs.d.ptr), (uint32_t)descs.size(),
-                            slice_size, first_input_index, total_inputs, recovery_count, rec,
-                            accumulate, st);
-        if (rc != P2G_OK) return rc;
-        return c->scratch_release(Ctx::kScratchEnc, st);
+        return dev_encode_files(c, files, nfiles, slice_size, first_input_index, total_inputs,
+                                recovery_count, rec, accumulate,
+                                stream ? static_cast<hipStream_t>(stream) : c->stream, -1);
     });
 }
 

@@ -46,9 +46,10 @@ struct DescSlot {
     hipEvent_t free = nullptr;  // `h` may be rewritten once this fires
 };
 
-// Encode kernel variants: the matrix-core GF(2) contraction (default) and the
-// bit-sliced VALU kernel.  P2G_ENCODE=bitslice|mfma picks one at p2g_open.
-enum { kEncodeMfma = 0, kEncodeBitslice = 1 };
+// Encode kernel variants: the matrix-core GF(2) contraction (default), the
+// bit-sliced VALU kernel and the split-nibble byte-permute kernel.
+// P2G_ENCODE=mfma|bitslice|nibble picks one at p2g_open.
+enum { kEncodeMfma = 0, kEncodeBitslice = 1, kEncodeNibble = 2, kEncodeVariants = 3 };
 
 struct BatchPlan;  // p2g_batch.cpp
 
@@ -57,6 +58,10 @@ struct TuneChoice {
     int variant = kEncodeMfma;
     uint32_t band_cols = 0;  // 0: the default band width
     double ms = 0;           // the winning time when it was measured
+    // the fastest encode kernel alone (p2g_dev_encode, p2g_encode_push), -1
+    // when not measured, and its time
+    int enc_variant = -1;
+    double enc_ms = 0;
 };
 
 struct Ctx {
@@ -157,10 +162,17 @@ int upload_descs(Ctx *c, DescSlot &slot, const std::vector<SliceDesc> &descs, hi
 
 // rec (^)= encode of the n slices described by d_descs (device), whose global
 // input indices start at first_input.
-// variant: kEncodeMfma / kEncodeBitslice, or -1 for the context's.
+// variant: kEncodeMfma / kEncodeBitslice / kEncodeNibble, or -1 for the
+// bucket's tuned encode kernel (else the context's).
 int dev_encode(Ctx *c, const SliceDesc *d_descs, uint32_t n, uint64_t S, uint32_t first_input,
                uint32_t total_inputs, uint32_t R, uint8_t *d_rec, int accumulate,
                hipStream_t st, int variant = -1);
+
+// p2g_dev_encode with an explicit encode kernel (-1: the bucket's tuned one,
+// else the context's).
+int dev_encode_files(Ctx *c, const p2g_dev_file *files, uint32_t nfiles, uint64_t slice_size,
+                     uint32_t first_input_index, uint32_t total_inputs, uint32_t recovery_count,
+                     uint8_t *rec, int accumulate, hipStream_t st, int variant);
 
 // p2g_dev_create_pass with an explicit encode variant and band width
 // (band_cols 0: the default).
@@ -177,6 +189,10 @@ bool tune_lookup(Ctx *c, uint64_t n, uint32_t R, uint64_t S, TuneChoice *out);
 // leaves the outputs of a run of the fastest.
 int tune_and_run(Ctx *c, uint64_t n, uint32_t R, uint64_t S, hipStream_t st,
                  const std::function<int(int, uint32_t)> &run, TuneChoice *out);
+// Times each encode kernel alone (run_encode(variant) queues one encode on
+// st) and records the fastest as the bucket's encode kernel.
+int tune_encode(Ctx *c, uint64_t n, uint32_t R, uint64_t S, hipStream_t st,
+                const std::function<int(int)> &run_encode);
 int tune_load(Ctx *c, const char *path);
 int tune_save(Ctx *c, const char *path);
 

@@ -41,6 +41,17 @@ hipError_t launch_encode(const SliceDesc *d_inputs, uint32_t n_inputs, const uin
                          uint32_t R, uint32_t S, uint8_t *d_rec, int accumulate,
                          hipStream_t stream);
 
+// Split-nibble form: 96 bytes of byte-permute tables per coefficient
+// (tables[(i * R + r) * 24 ..]) for inputs first_input + i, rows 0..R-1, then
+// rec[r][*] (^)= XOR_i coef(r, i) * slice_i from them.
+constexpr uint32_t kNibbleTableBytes = 96;
+hipError_t launch_nibble_tables(const uint16_t *d_antilog, const uint16_t *d_logbase,
+                                uint32_t n_inputs, uint32_t R, uint32_t *d_tables,
+                                hipStream_t stream);
+hipError_t launch_encode_nibble(const SliceDesc *d_inputs, uint32_t n_inputs, const uint32_t *d_tables,
+                                uint32_t R, uint32_t S, uint8_t *d_rec, int accumulate,
+                                hipStream_t stream);
+
 // Fills SliceDesc::tail of n descriptors in place.
 // With d_tail_blocks (64 bytes per descriptor, may be null), also writes each
 // slice's zero-padded 64-byte block holding byte `valid` (for slice MD5).

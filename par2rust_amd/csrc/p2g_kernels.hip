@@ -282,6 +282,216 @@ hipError_t launch_encode(const SliceDesc *d_inputs, uint32_t n_inputs, const uin
 }
 
 // ======================================================================
+// K1 (split-nibble form): byte-permute table lookups
+// ======================================================================
+//
+// The GF(2^16) product c * x is linear in the bits of x, so it splits over
+// chunks of x: c * x = XOR_q T_q[chunk_q(x)], T_q[v] = c * (v << shift_q).
+// The chunks here are three bits wide (bits 0-2, 3-5, 6-7 of each byte of
+// the symbol: six chunks), so each table has at most 8 entries and one
+// v_perm_b32 looks up four of them at once: its selector bytes index the 8
+// bytes of a register pair.  A table of 16-bit products is two byte tables
+// (low and high byte of the product), so a coefficient has 12 byte tables
+// of 8 entries, 96 bytes (the reference's SSSE3 / AVX2 kernels split by
+// 4-bit nibbles for their 16-entry byte shuffle, src/galois_simd.rs:109-137,
+// 432-539; v_perm_b32 indexes 8 bytes, hence 3-bit chunks).
+constexpr int kNibChunks = 6;
+__host__ __device__ constexpr uint32_t nib_shift(int q) { return q < 3 ? 3 * q : 8 + 3 * (q - 3); }
+__host__ __device__ constexpr uint32_t nib_width(int q) { return (q % 3) == 2 ? 2 : 3; }
+
+// tables[(i * R + r) * 24 + 4 q + 2 o + h]: byte table o (0: low byte of the
+// product, 1: high byte) of chunk q of coefficient coef(r, first_input + i),
+// entries 4 h .. 4 h + 3 (h = 0, 1), entry v in byte v % 4.  Thread per
+// (input, row).
+__global__ __launch_bounds__(256) void k_nibble_tables(const uint16_t *__restrict__ antilog,
+                                                       const uint16_t *__restrict__ logbase,
+                                                       uint32_t n, uint32_t R,
+                                                       uint32_t *__restrict__ tables) {
+    const uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (uint64_t)n * R) return;
+    const uint32_t i = (uint32_t)(idx / R), r = (uint32_t)(idx % R);
+    uint32_t v = antilog[(uint32_t)(((uint64_t)logbase[i] * r) % kFieldLimit)];
+    uint32_t pw[16];  // coef * x^m
+#pragma unroll
+    for (int m = 0; m < 16; m++) {
+        pw[m] = v;
+        v = ((v << 1) & 0xFFFFu) ^ ((v & 0x8000u) ? (kFieldGenerator & 0xFFFFu) : 0u);
+    }
+    uint32_t *out = tables + idx * 24;
+#pragma unroll
+    for (int q = 0; q < kNibChunks; q++) {
+        uint32_t t[2][2] = {{0, 0}, {0, 0}};  // [byte o][half h]
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            uint32_t prod = 0;
+#pragma unroll
+            for (int j = 0; j < 3; j++)
+                if (j < (int)nib_width(q) && ((e >> j) & 1)) prod ^= pw[nib_shift(q) + j];
+            // entries past the chunk's width are never selected (selectors
+            // are masked to the width); they stay the products of their bits
+            t[0][e >> 2] |= (prod & 0xFFu) << (8 * (e & 3));
+            t[1][e >> 2] |= (prod >> 8) << (8 * (e & 3));
+        }
+        out[4 * q + 0] = t[0][0];
+        out[4 * q + 1] = t[0][1];
+        out[4 * q + 2] = t[1][0];
+        out[4 * q + 3] = t[1][1];
+    }
+}
+
+hipError_t launch_nibble_tables(const uint16_t *d_antilog, const uint16_t *d_logbase,
+                                uint32_t n_inputs, uint32_t R, uint32_t *d_tables,
+                                hipStream_t stream) {
+    const uint64_t total = (uint64_t)n_inputs * R;
+    if (total == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_nibble_tables, dim3((uint32_t)((total + 255) / 256)), dim3(256), 0, stream,
+                       d_antilog, d_logbase, n_inputs, R, d_tables);
+    return hipGetLastError();
+}
+
+// XOR of three words in one v_bitop3_b32 (truth table 0x96).
+__device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) {
+    return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
+}
+
+constexpr int kNibThreads = 256;
+constexpr int kNibRT = 8;   // recovery rows per workgroup
+constexpr int kNibSym = 8;  // symbols per lane (16 bytes)
+
+// A lane owns 8 consecutive symbols (16 bytes) of the slice column range and
+// kNibRT recovery rows, over the inputs of its workgroup's input chunk
+// (blockIdx.z: chunks of in_chunk inputs, so that small slices still fill the
+// GPU; with more than one chunk the partial blocks are XORed into rec with
+// atomics, XOR being the addition of the field).  Per input it gathers the low bytes of its 8 symbols
+// into two words and the high bytes into two more (v_perm), splits them into
+// the six chunk selector planes once, then per row looks every chunk up in
+// the row's byte tables (one v_perm per 4 symbols, per output byte) and
+// XORs the results into byte-plane accumulators; the accumulators are
+// interleaved back into symbols at the end.  The tables of a (row, input)
+// are the same for the whole workgroup, so they are read as scalars.
+template <bool VEC>
+__global__ __launch_bounds__(kNibThreads) void k_encode_nibble(
+    const SliceDesc *__restrict__ inputs, uint32_t n_inputs, const uint32_t *__restrict__ tables,
+    uint32_t R, uint32_t S, uint8_t *__restrict__ rec, int accumulate, uint32_t in_chunk) {
+    const uint32_t col = (blockIdx.x * kNibThreads + threadIdx.x) * (2 * kNibSym);
+    if (col >= S) return;
+    const uint32_t i_lo = blockIdx.z * in_chunk, i_hi = min(i_lo + in_chunk, n_inputs);
+    const bool split = gridDim.z > 1;
+    const uint32_t r0 = blockIdx.y * kNibRT;
+    const uint32_t rt = min(R - r0, (uint32_t)kNibRT);
+    uint32_t acc[kNibRT][2][2];  // [row][byte o][symbols 4 j .. 4 j + 3]
+#pragma unroll
+    for (int rr = 0; rr < kNibRT; rr++)
+#pragma unroll
+        for (int o = 0; o < 2; o++) acc[rr][o][0] = acc[rr][o][1] = 0;
+
+    for (uint32_t i = i_lo; i < i_hi; i++) {
+        const SliceDesc d = inputs[i];
+        uint32_t x[4];
+        if (VEC && col + 16 <= d.valid) {
+            const uint4 v = gload(reinterpret_cast<const uint4 *>(d.ptr + col));
+            x[0] = v.x; x[1] = v.y; x[2] = v.z; x[3] = v.w;
+        } else {
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const uint32_t off = col + 4u * q;
+                x[q] = (off < S) ? load_word(d.ptr, off, d.valid) : 0u;
+            }
+        }
+        // low bytes of symbols 0-3 / 4-7, high bytes likewise
+        uint32_t sel[kNibChunks][2];
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+            const uint32_t lo = __builtin_amdgcn_perm(x[2 * j + 1], x[2 * j], 0x06040200u);
+            const uint32_t hi = __builtin_amdgcn_perm(x[2 * j + 1], x[2 * j], 0x07050301u);
+            sel[0][j] = lo & 0x07070707u;
+            sel[1][j] = (lo >> 3) & 0x07070707u;
+            sel[2][j] = (lo >> 6) & 0x03030303u;
+            sel[3][j] = hi & 0x07070707u;
+            sel[4][j] = (hi >> 3) & 0x07070707u;
+            sel[5][j] = (hi >> 6) & 0x03030303u;
+        }
+        const uint32_t *tb = tables + ((size_t)i * R + r0) * 24;
+#pragma unroll
+        for (int rr = 0; rr < kNibRT; rr++) {
+            if ((uint32_t)rr >= rt) break;
+            const uint32_t *t = tb + rr * 24;
+#pragma unroll
+            for (int o = 0; o < 2; o++) {
+#pragma unroll
+                for (int j = 0; j < 2; j++) {
+                    uint32_t p[kNibChunks];
+#pragma unroll
+                    for (int q = 0; q < kNibChunks; q++)
+                        p[q] = __builtin_amdgcn_perm(t[4 * q + 2 * o + 1], t[4 * q + 2 * o], sel[q][j]);
+                    uint32_t a = xor3(acc[rr][o][j], p[0], p[1]);
+                    a = xor3(a, p[2], p[3]);
+                    acc[rr][o][j] = xor3(a, p[4], p[5]);
+                }
+            }
+        }
+    }
+
+#pragma unroll
+    for (int rr = 0; rr < kNibRT; rr++) {
+        if ((uint32_t)rr >= rt) break;
+        uint32_t y[4];
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+            // symbol s: low byte acc[.][0][j] byte s, high byte acc[.][1][j] byte s
+            y[2 * j] = __builtin_amdgcn_perm(acc[rr][1][j], acc[rr][0][j], 0x05010400u);
+            y[2 * j + 1] = __builtin_amdgcn_perm(acc[rr][1][j], acc[rr][0][j], 0x07030602u);
+        }
+        uint8_t *out = rec + (size_t)(r0 + rr) * S + col;
+        if (split) {
+            uint32_t *o = reinterpret_cast<uint32_t *>(out);
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+                if (col + 4u * q < S) atomicXor(o + q, y[q]);
+        } else if (VEC && col + 16 <= S) {
+            uint4 *o4 = reinterpret_cast<uint4 *>(out);
+            uint4 v = make_uint4(y[0], y[1], y[2], y[3]);
+            if (accumulate) {
+                const uint4 old = *o4;
+                v.x ^= old.x; v.y ^= old.y; v.z ^= old.z; v.w ^= old.w;
+            }
+            *o4 = v;
+        } else {
+            uint32_t *o = reinterpret_cast<uint32_t *>(out);
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+                if (col + 4u * q < S) o[q] = accumulate ? (o[q] ^ y[q]) : y[q];
+        }
+    }
+}
+
+hipError_t launch_encode_nibble(const SliceDesc *d_inputs, uint32_t n_inputs, const uint32_t *d_tables,
+                                uint32_t R, uint32_t S, uint8_t *d_rec, int accumulate,
+                                hipStream_t stream) {
+    if (R == 0) return hipSuccess;
+    const uint32_t per_block = kNibThreads * 2 * kNibSym;
+    dim3 grid((S + per_block - 1) / per_block, (R + kNibRT - 1) / kNibRT);
+    // input chunks of at least 16 inputs until the grid has ~2048 workgroups
+    const uint64_t xy = (uint64_t)grid.x * grid.y;
+    const uint32_t want = (uint32_t)std::max<uint64_t>(1, (2048 + xy - 1) / xy);
+    const uint32_t chunks = std::max<uint32_t>(1, std::min<uint32_t>(want, (n_inputs + 15) / 16));
+    const uint32_t in_chunk = (n_inputs + chunks - 1) / chunks;
+    grid.z = (n_inputs + in_chunk - 1) / in_chunk;
+    if (grid.z > 1 && !accumulate) {
+        // the chunks' partials are XORed into a zeroed block
+        const hipError_t e = hipMemsetAsync(d_rec, 0, (size_t)R * S, stream);
+        if (e != hipSuccess) return e;
+    }
+    if (S % 16 == 0 && reinterpret_cast<uintptr_t>(d_rec) % 16 == 0)
+        hipLaunchKernelGGL(k_encode_nibble<true>, grid, dim3(kNibThreads), 0, stream, d_inputs,
+                           n_inputs, d_tables, R, S, d_rec, accumulate, in_chunk);
+    else
+        hipLaunchKernelGGL(k_encode_nibble<false>, grid, dim3(kNibThreads), 0, stream, d_inputs,
+                           n_inputs, d_tables, R, S, d_rec, accumulate, in_chunk);
+    return hipGetLastError();
+}
+
+// ======================================================================
 // K1 (matrix-core form): GF(2) contraction on fp4 MFMA
 // ======================================================================
 //

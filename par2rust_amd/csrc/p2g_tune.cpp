@@ -1,7 +1,9 @@
-// p2g_tune.cpp -- the autotuner: per (N, R, slice size) bucket, the encode
-// variant (matrix-core fused pass or bit-sliced VALU kernels) and the column
-// band width of the fused pass that ran fastest, cached in the context and
-// optionally in a text file.
+// p2g_tune.cpp -- the autotuner: per (N, R, slice size) bucket, the create
+// pass variant (matrix-core fused pass, or separate launches with the
+// bit-sliced or the split-nibble VALU encode) and the column band width of
+// the fused pass that ran fastest, plus the fastest encode kernel alone (for
+// p2g_dev_encode and p2g_encode_push), cached in the context and optionally
+// in a text file.
 //
 // Buckets are the bit lengths of N (slices), R (recovery blocks) and S
 // (slice bytes), so one measurement covers shapes within a factor of two.
@@ -13,8 +15,9 @@
 // bucket from the same cache.
 //
 // Cache file (P2G_TUNE_CACHE, p2g_tune_save / p2g_tune_load): one line per
-// bucket, "bucket variant band_cols ms", bucket = nbits(N) | nbits(R) << 8 |
-// nbits(S) << 16.
+// bucket, "bucket variant band_cols ms enc_variant enc_ms", bucket =
+// nbits(N) | nbits(R) << 8 | nbits(S) << 16, enc_variant -1 when the
+// encode kernels were not timed (lines of the four first fields load too).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -60,6 +63,7 @@ int tune_and_run(Ctx *c, uint64_t n, uint32_t R, uint64_t S, hipStream_t st,
         cand.push_back({kEncodeMfma, 0});
     }
     cand.push_back({kEncodeBitslice, 0});
+    cand.push_back({kEncodeNibble, 0});
     double best = 1e300;
     size_t best_k = 0;
     for (size_t k = 0; k < cand.size(); k++) {
@@ -86,28 +90,64 @@ int tune_and_run(Ctx *c, uint64_t n, uint32_t R, uint64_t S, hipStream_t st,
         int rc = run(cand[best_k].first, cand[best_k].second);
         if (rc != P2G_OK) return rc;
     }
-    TuneChoice ch;
+    TuneChoice &ch = c->tune[tune_bucket(n, R, S)];  // keeps an encode-kernel choice
     ch.variant = cand[best_k].first;
     ch.band_cols = cand[best_k].second;
     ch.ms = best;
-    c->tune[tune_bucket(n, R, S)] = ch;
     if (out) *out = ch;
     if (!c->tune_path.empty()) return tune_save(c, c->tune_path.c_str());
     return P2G_OK;
 }
 
+int tune_encode(Ctx *c, uint64_t n, uint32_t R, uint64_t S, hipStream_t st,
+                const std::function<int(int)> &run_encode) {
+    double best = 1e300;
+    int best_v = kEncodeMfma;
+    for (int v = 0; v < kEncodeVariants; v++) {
+        double t_best = 1e300;
+        for (int rep = 0; rep < 3; rep++) {  // the first run also warms the kernel up
+            hipError_t e = hipStreamSynchronize(st);
+            if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "autotune sync", e);
+            const auto t0 = std::chrono::steady_clock::now();
+            int rc = run_encode(v);
+            if (rc != P2G_OK) return rc;
+            e = hipStreamSynchronize(st);
+            if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "autotune sync", e);
+            t_best = std::min(
+                t_best,
+                std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+            if (t_best > 4 * best) break;  // clearly slower
+        }
+        if (t_best < best) {
+            best = t_best;
+            best_v = v;
+        }
+    }
+    TuneChoice &ch = c->tune[tune_bucket(n, R, S)];
+    ch.enc_variant = best_v;
+    ch.enc_ms = best;
+    return P2G_OK;
+}
+
+// One bucket per line: "bucket variant band_cols ms [enc_variant enc_ms]".
 int tune_load(Ctx *c, const char *path) {
     FILE *f = fopen(path, "r");
     if (!f) return c->fail(P2G_ERR_IO, "cannot open the tuning cache");
-    unsigned bucket, band;
-    int variant;
-    double ms;
-    while (fscanf(f, "%u %d %u %lf", &bucket, &variant, &band, &ms) == 4) {
-        if (variant != kEncodeMfma && variant != kEncodeBitslice) continue;
+    char line[256];
+    while (fgets(line, sizeof line, f)) {
+        unsigned bucket, band;
+        int variant, enc = -1;
+        double ms, enc_ms = 0;
+        const int got = sscanf(line, "%u %d %u %lf %d %lf", &bucket, &variant, &band, &ms, &enc, &enc_ms);
+        if (got < 4 || variant < 0 || variant >= kEncodeVariants) continue;
         TuneChoice ch;
         ch.variant = variant;
         ch.band_cols = band;
         ch.ms = ms;
+        if (got == 6 && enc >= 0 && enc < kEncodeVariants) {
+            ch.enc_variant = enc;
+            ch.enc_ms = enc_ms;
+        }
         c->tune[bucket] = ch;
     }
     fclose(f);
@@ -119,7 +159,8 @@ int tune_save(Ctx *c, const char *path) {
     FILE *f = fopen(tmp.c_str(), "w");
     if (!f) return c->fail(P2G_ERR_IO, "cannot write the tuning cache");
     for (const auto &kv : c->tune)
-        fprintf(f, "%u %d %u %.4f\n", kv.first, kv.second.variant, kv.second.band_cols, kv.second.ms);
+        fprintf(f, "%u %d %u %.4f %d %.4f\n", kv.first, kv.second.variant, kv.second.band_cols,
+                kv.second.ms, kv.second.enc_variant, kv.second.enc_ms);
     const bool ok = fclose(f) == 0 && rename(tmp.c_str(), path) == 0;
     return ok ? P2G_OK : c->fail(P2G_ERR_IO, "cannot write the tuning cache");
 }
@@ -163,6 +204,13 @@ int p2g_autotune(p2g_ctx *ctx, uint32_t slices, uint32_t recovery_count, uint64_
         rc = p2g_dev_fill_xorshift64(ctx, static_cast<uint8_t *>(src.ptr), bytes, 1, c->stream);
         p2g_dev_file file{static_cast<const uint8_t *>(src.ptr), bytes};
         TuneChoice ch;
+        // the encode kernels alone (p2g_dev_encode, p2g_encode_push), then the
+        // create pass candidates
+        if (rc == P2G_OK)
+            rc = tune_encode(c, slices, recovery_count, slice_size, c->stream, [&](int variant) {
+                return dev_encode_files(c, &file, 1, slice_size, 0, slices, recovery_count, rec, 0,
+                                        c->stream, variant);
+            });
         if (rc == P2G_OK)
             rc = tune_and_run(c, slices, recovery_count, slice_size, c->stream,
                               [&](int variant, uint32_t band) {
@@ -193,6 +241,19 @@ int p2g_tune_get(p2g_ctx *ctx, uint32_t slices, uint32_t recovery_count, uint64_
         }
         if (variant_out) *variant_out = ch.variant;
         if (band_cols_out) *band_cols_out = ch.band_cols;
+        return P2G_OK;
+    });
+}
+
+int p2g_tune_get_encode(p2g_ctx *ctx, uint32_t slices, uint32_t recovery_count, uint64_t slice_size,
+                        int *variant_out, double *ms_out) {
+    return guarded(reinterpret_cast<Ctx *>(ctx), [&]() -> int {
+        Ctx *c = reinterpret_cast<Ctx *>(ctx);
+        if (!c) return P2G_ERR_INVALID_ARGUMENT;
+        TuneChoice ch;
+        const bool tuned = tune_lookup(c, slices, recovery_count, slice_size, &ch) && ch.enc_variant >= 0;
+        if (variant_out) *variant_out = tuned ? ch.enc_variant : c->encode_variant;
+        if (ms_out) *ms_out = tuned ? ch.enc_ms : 0.0;
         return P2G_OK;
     });
 }

@@ -68,7 +68,9 @@ def env_ctx():
 def test_autotune_caches_saves_loads(env_ctx, tmp_path):
     c = env_ctx()
     got = c.autotune(600, 40, 16384)
-    assert got["variant"] in (0, 1) and got["ms"] > 0
+    assert got["variant"] in (0, 1, 2) and got["ms"] > 0
+    enc = c.tune_get_encode(600, 40, 16384)
+    assert enc["variant"] in (0, 1, 2) and enc["ms"] > 0
     assert c.tune_get(600, 40, 16384) == {"variant": got["variant"], "band_cols": got["band_cols"]}
     # the same bucket: shapes within the same bit lengths
     assert c.tune_get(1000, 60, 32000) == c.tune_get(600, 40, 16384)
@@ -77,6 +79,8 @@ def test_autotune_caches_saves_loads(env_ctx, tmp_path):
     d = env_ctx()
     d.tune_load(path)
     assert d.tune_get(600, 40, 16384) == c.tune_get(600, 40, 16384)
+    a, b = d.tune_get_encode(600, 40, 16384), c.tune_get_encode(600, 40, 16384)
+    assert a["variant"] == b["variant"] and abs(a["ms"] - b["ms"]) < 1e-3  # 4 decimals in the file
 
 
 def test_online_tuning_keeps_results(env_ctx, tmp_path):
@@ -91,10 +95,11 @@ def test_online_tuning_keeps_results(env_ctx, tmp_path):
 
     n = sum((len(f) + S - 1) // S for f in files)
     lines = cache.read_text().split()
-    assert str(p2.tune_bucket(n, R, S)) == lines[0] and len(lines) == 4
+    # bucket, variant, band, ms, encode kernel (-1: not timed online), its ms
+    assert str(p2.tune_bucket(n, R, S)) == lines[0] and len(lines) == 6 and lines[4] == "-1"
 
 
-@pytest.mark.parametrize("variant,band", [(1, 0), (0, 1), (0, 3)])
+@pytest.mark.parametrize("variant,band", [(1, 0), (2, 0), (0, 1), (0, 3)])
 def test_cached_choice_drives_pass_and_batch(env_ctx, tmp_path, variant, band):
     import par2rust_amd as p2
     from par2rust_amd.hipmem import DeviceBuffer
