@@ -482,6 +482,55 @@ def run_end_to_end(ctx, host: bytes, S: int, R: int, cpu: bool) -> dict:
         shutil.rmtree(tmp, ignore_errors=True)
 
 
+ENCODE_VARIANT_SHAPES = [  # (slices, slice size, recovery count)
+    (1024, 4096, 1), (1024, 4096, 2), (1024, 4096, 4), (1024, 4096, 8), (1024, 4096, 16),
+    (1024, 4096, 32), (1027, 524288, 1), (1027, 524288, 4), (1027, 524288, 16), (1027, 524288, 64),
+    (1027, 524288, 200), (16000, 65536, 1), (16000, 65536, 4), (16000, 65536, 16),
+    (16000, 65536, 160), (2, 786432, 1), (20, 786432, 1), (200, 786432, 2), (1226, 786432, 13),
+]
+
+
+def run_encode_variants(reps: int) -> dict:
+    """The encode kernels alone (p2g_dev_encode with P2G_ENCODE = mfma,
+    bitslice, nibble) on one resident file of each shape: best wall time of
+    `reps` synchronized calls after a warm-up call, and whether the three
+    outputs are identical."""
+    import par2rust_amd as p2
+    from par2rust_amd import hipmem
+
+    ctxs = {}
+    for name in ("mfma", "bitslice", "nibble"):
+        os.environ["P2G_ENCODE"] = name
+        ctxs[name] = p2.Context(0)
+    os.environ.pop("P2G_ENCODE", None)
+    out = []
+    for N, S, R in ENCODE_VARIANT_SHAPES:
+        src = hipmem.DeviceBuffer(N * S)
+        ctxs["mfma"].dev_fill_xorshift64(src.addr, N * S, 7)
+        rec = hipmem.DeviceBuffer(R * S)
+        ms, blocks = {}, {}
+        for name, c in ctxs.items():
+            c.dev_encode([src], S, R, rec)
+            c.synchronize()
+            best = 1e9
+            for _ in range(reps):
+                t = time.perf_counter()
+                c.dev_encode([src], S, R, rec)
+                c.synchronize()
+                best = min(best, time.perf_counter() - t)
+            ms[name] = round(best * 1e3, 4)
+            blocks[name] = rec.download()
+        same = blocks["mfma"] == blocks["bitslice"] == blocks["nibble"]
+        out.append({"slices": N, "slice_size": S, "recovery": R, "ms": ms,
+                    "fastest": min(ms, key=ms.get), "equal": bool(same)})
+        log(f"bench: encode variants N={N} S={S} R={R}: {ms} equal {same}")
+        del src, rec
+    for c in ctxs.values():
+        c.close()
+    return {"shapes": out, "timing": "host wall time per synchronized p2g_dev_encode call, "
+                                      "inputs resident in HBM"}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -507,6 +556,9 @@ def main() -> int:
                     help="skip the many-files (64 files, R = 1000) pass timing")
     ap.add_argument("--no-end-to-end", action="store_true",
                     help="skip the whole-create wall-time comparison")
+    ap.add_argument("--encode-variants", action="store_true",
+                    help="only time the three encode kernels alone over small- and large-R "
+                         "shapes (JSON line) and exit")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run one 512 MiB device-to-device copy (the reference reading "
                          "for the FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -522,6 +574,10 @@ def main() -> int:
 
     import par2rust_amd as p2
     from par2rust_amd import hipmem, synth
+
+    if args.encode_variants:
+        print(json.dumps({"encode_variants": run_encode_variants(max(1, args.steps))}), flush=True)
+        return 0
 
     if args.copy_calibration:
         n = 512 << 20

@@ -143,6 +143,7 @@ int build_plan(Ctx *c, const std::vector<PlanItem> &items, bool use_graph, Batch
     B->c = c;
     B->njobs = nitems;
     B->groups.clear();
+    B->work.clear();
     B->sync_off.clear();
     B->drop_graph();
     std::vector<std::vector<SliceDesc>> jd(nitems);
@@ -328,7 +329,8 @@ int build_plan(Ctx *c, const std::vector<PlanItem> &items, bool use_graph, Batch
             add_slice_work(w, cnt, sjobs[q].S);
         }
         for (uint32_t p = sp.p_lo; p < sp.p_hi; p++) add_piece_work(w, pieces[p]);
-        plan_batch_group(c->num_cus, c->batch_resident, tasks, hashed, units, w, &g);
+        plan_batch_group(c->num_cus, c->batch_resident, tasks, hashed, units, w, c->rates, &g);
+        B->work.push_back(w);
         B->sync_off.push_back(sync_words_total);
         sync_words_total += (uint32_t)align_up(starts, 64);
     }
@@ -419,6 +421,22 @@ int wait_plan(BatchPlan *B) {
     e = hipMemcpy(&bad, B->d_err.ptr, 4, hipMemcpyDeviceToHost);
     if (e == hipSuccess && bad) e = hipMemset(B->d_err.ptr, 0, 4);
     if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "read batch flags", e);
+    // the run's own role times refine the rates the next plans are sized by
+    for (size_t k = 0; k < B->groups.size() && k < B->work.size(); k++) {
+        unsigned long long t[2 * kPassRoles];
+        e = hipMemcpy(t, static_cast<uint32_t *>(B->d_sync.ptr) + B->sync_off[k] + kPassTimes, sizeof(t),
+                      hipMemcpyDeviceToHost);
+        if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "read batch stamps", e);
+        auto span_us = [&](int role) {  // 100 MHz ticks
+            const unsigned long long a = t[2 * role], b = t[2 * role + 1];
+            return b > a && b != 0 ? (double)(b - a) / 100.0 : 0.0;
+        };
+        if (!bad) {
+            const BatchGroupLaunch &g = B->groups[k];
+            update_rates(c->rates, B->work[k], g.grid, g.h2_blocks, g.h3_blocks, span_us(kPassEncode),
+                         span_us(kPassSlice));
+        }
+    }
     if (getenv("P2G_BATCH_VERBOSE")) {
         // per group: first start and last end of each role, ms from the batch start
         std::vector<unsigned long long> t0(1);
@@ -551,7 +569,9 @@ int p2g_plan_roles(int kind, uint32_t num_cus, uint32_t max_resident, uint32_t s
             add_slice_work(w, n, S);
             add_piece_work(w, P);
             BatchGroupLaunch g{};
-            plan_batch_group(num_cus, max_resident, (R + 63) / 64, n, P.row_blocks * P.col_groups, w, &g);
+            const CostRates seed;
+            plan_batch_group(num_cus, max_resident, (R + 63) / 64, n, P.row_blocks * P.col_groups, w,
+                             seed, &g);
             grid = g.grid;
             h2 = g.h2_blocks;
             h3 = g.h3_blocks;

@@ -24,6 +24,7 @@ struct BatchPlan {
     DevBuf own_afrag;
     DevBuf *afrag = &own_afrag;  // or a buffer shared by plans run in one stream order
     std::vector<BatchGroupLaunch> groups;
+    std::vector<BatchGroupWork> work;  // each group's work, for the rates its run measures
     std::vector<uint32_t> sync_off;  // word offset of each group's sync words
     hipStream_t cap = nullptr;
     hipGraph_t graph = nullptr;

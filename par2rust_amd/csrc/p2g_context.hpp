@@ -100,6 +100,7 @@ struct Ctx {
     DescSlot enc_descs;  // encode launches
     DescSlot sum_descs;  // slice checksum launches
     std::vector<BatchPlan *> pending_batches;  // p2g_dev_create_batch, until p2g_synchronize
+    CostRates rates;                           // planners' cost estimates, updated from runs
     std::map<uint32_t, TuneChoice> tune;       // autotuner cache by bucket
     bool autotune = false;                     // P2G_AUTOTUNE=1: tune a bucket on its first call
     std::string tune_path;                     // P2G_TUNE_CACHE: cache file, loaded at open

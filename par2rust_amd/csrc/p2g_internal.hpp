@@ -238,10 +238,25 @@ struct BatchGroupWork {
     double big_slices = 0, big_blocks = 0;  // the slices of the largest size and their blocks
     std::vector<double> piece_steps, piece_tasks, piece_task_blocks;
 };
+// What the planners estimate times from (µs): an encode unit's K step on
+// one workgroup; a 64-byte block of a slice lane (MD5 and CRC-32) and of a
+// packet lane; the fixed cost of a slice and of a packet task.  A context
+// starts from values measured on MI355X at the batched mix and updates them
+// from the role stamps of every batched run it finishes (update_rates).
+struct CostRates {
+    double step_us = 0.38, slice_block_us = 1.31, slice_us = 36;
+    double packet_block_us = 0.65, task_us = 40;
+    uint32_t updates = 0;  // runs that updated the rates
+};
+// Folds one finished group's measured role times (µs from its first start
+// to its last end; 0 when the role had no work) into the rates.
+void update_rates(CostRates &r, const BatchGroupWork &w, uint32_t grid, uint32_t h2, uint32_t h3,
+                  double enc_us, double slice_us);
 // Group sizing: MD5 workgroups for n_tasks packet tasks and n_slices slices,
 // as few as are estimated to finish within the encode (the rest encode).
 void plan_batch_group(uint32_t num_cus, uint32_t max_resident, uint32_t n_tasks, uint32_t n_slices,
-                      uint32_t n_units, const BatchGroupWork &w, BatchGroupLaunch *g);
+                      uint32_t n_units, const BatchGroupWork &w, const CostRates &rates,
+                      BatchGroupLaunch *g);
 constexpr uint32_t kBatchProgress = 1;  // sync word: units published so far
 
 // Synthetic input: the xorshift64 stream from `seed` into d_dst.  d_jumps

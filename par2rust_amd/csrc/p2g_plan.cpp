@@ -31,19 +31,41 @@ bool plan_create_pass(uint32_t n, uint32_t R, uint32_t S, uint32_t num_cus, uint
     return true;
 }
 
-// Costs in µs measured on MI355X at the batched mix, for sizing the roles of
-// a group: an encode unit's K step on one workgroup; a 64-byte block of a
-// slice lane (MD5 and CRC-32) and of a packet lane; the fixed cost of a
-// slice and of a packet task.
-constexpr double kStepUs = 0.38, kSliceBlockUs = 1.31, kSliceUs = 36;
-constexpr double kPacketBlockUs = 0.65, kTaskUs = 40;
+// A measurement is taken when the role ran long enough to be timed well,
+// kept within [1/2, 2] x the seed value (a run sharing the GPU with other
+// work must not swing the sizing), and averaged in with weight 1/2.
+static void fold(double &rate, double seed, double meas) {
+    if (!(meas > 0)) return;
+    meas = std::min(std::max(meas, 0.5 * seed), 2.0 * seed);
+    rate = 0.5 * rate + 0.5 * meas;
+}
+
+void update_rates(CostRates &r, const BatchGroupWork &w, uint32_t grid, uint32_t h2, uint32_t h3,
+                  double enc_us, double slice_us) {
+    const CostRates seed;
+    bool used = false;
+    const uint32_t e = grid > h2 + h3 ? grid - h2 - h3 : 1;
+    if (enc_us > 500 && w.enc_steps > 0) {  // the encode: e workgroups over its unit-steps
+        fold(r.step_us, seed.step_us, enc_us * e / w.enc_steps);
+        used = true;
+    }
+    if (slice_us > 500 && h2 && w.slice_blocks > 0) {  // the slice lanes, net of the per-slice cost
+        const double lanes = 128.0 * h2;
+        fold(r.slice_block_us, seed.slice_block_us,
+             (slice_us * lanes - w.slices * r.slice_us) / w.slice_blocks);
+        used = true;
+    }
+    if (used) r.updates++;
+}
+
 
 // Estimated end (µs) of the packet role on 4 h3 waves when the encode takes
 // t_enc: pieces are encoded in order, each over its share of t_enc by its
 // unit-steps; task t runs on wave t mod 4 h3 after the wave's earlier tasks,
 // starts no earlier than its piece's encode and, following its bands, ends
 // no earlier than the piece's encode ends.
-static double packet_role_end(const BatchGroupWork &w, uint32_t h3, double t_enc) {
+static double packet_role_end(const BatchGroupWork &w, uint32_t h3, double t_enc,
+                              const CostRates &rt) {
     if (h3 == 0 || w.enc_steps <= 0) return 0;
     std::vector<double> free_at(4 * h3, 0.0);
     double done = 0, end = 0;
@@ -52,7 +74,7 @@ static double packet_role_end(const BatchGroupWork &w, uint32_t h3, double t_enc
         const double start = t_enc * done / w.enc_steps;
         done += w.piece_steps[p];
         const double finish = t_enc * done / w.enc_steps;
-        const double chain = kTaskUs + w.piece_task_blocks[p] * kPacketBlockUs;
+        const double chain = rt.task_us + w.piece_task_blocks[p] * rt.packet_block_us;
         for (double k = 0; k < w.piece_tasks[p]; k++, t++) {
             double &f = free_at[t % free_at.size()];
             f = std::max(std::max(f, start) + chain, finish);
@@ -63,7 +85,8 @@ static double packet_role_end(const BatchGroupWork &w, uint32_t h3, double t_enc
 }
 
 void plan_batch_group(uint32_t num_cus, uint32_t max_resident, uint32_t n_tasks, uint32_t n_slices,
-                      uint32_t n_units, const BatchGroupWork &w, BatchGroupLaunch *g) {
+                      uint32_t n_units, const BatchGroupWork &w, const CostRates &rt,
+                      BatchGroupLaunch *g) {
     // one workgroup per CU, never more than the device holds at once
     num_cus = std::max<uint32_t>(1, std::min(num_cus, max_resident));
     g->grid = num_cus;
@@ -85,24 +108,24 @@ void plan_batch_group(uint32_t num_cus, uint32_t max_resident, uint32_t n_tasks,
     // the encode band by band: workgroups are added while the simulated end
     // of their role lies more than 2% past the encode and more workgroups
     // would bring it closer.
-    const double slice_us = w.slices * kSliceUs + w.slice_blocks * kSliceBlockUs;
+    const double slice_us = w.slices * rt.slice_us + w.slice_blocks * rt.slice_block_us;
     uint32_t h3 = std::min<uint32_t>(h3_cap, 1), h2 = std::min<uint32_t>(h2_cap, 1);
     for (;;) {
         const uint32_t e = num_cus > h2 + h3 + 1 ? num_cus - h2 - h3 : 1;
-        const double t_enc = w.enc_steps * kStepUs / e;
+        const double t_enc = w.enc_steps * rt.step_us / e;
         bool grew = false;
         const double big_rounds = std::ceil(w.big_slices / (128.0 * std::max<uint32_t>(h2, 1)));
-        const double big_us = big_rounds > 1 ? big_rounds * (kSliceUs + w.big_blocks * kSliceBlockUs) : 0;
+        const double big_us = big_rounds > 1 ? big_rounds * (rt.slice_us + w.big_blocks * rt.slice_block_us) : 0;
         if (h2 < h2_cap && std::max(slice_us / (128.0 * h2), big_us) > 0.75 * t_enc) {
             h2++;
             grew = true;
         }
-        const double end = packet_role_end(w, h3, t_enc);
+        const double end = packet_role_end(w, h3, t_enc, rt);
         if (h3 < h3_cap && end > 1.02 * t_enc) {
             // the fewest more workgroups that bring the end closer (waves
             // that still hold two tasks of one piece gain nothing)
             for (uint32_t h = h3 + 1; h <= h3_cap; h++) {
-                if (packet_role_end(w, h, t_enc) < end - 0.01 * t_enc) {
+                if (packet_role_end(w, h, t_enc, rt) < end - 0.01 * t_enc) {
                     h3 = h;
                     grew = true;
                     break;

@@ -32,9 +32,20 @@
 
 namespace p2g {
 
-constexpr double kServeOpsPerMs = 4.0e12;  // device rate assumed when sizing pieces (4 PF)
-constexpr double kServeMd5BlockMs = 1.1e-3;  // one lane's MD5 + CRC-32 of a 64-byte block
 constexpr uint32_t kServeSliceLanes = 24 * 128;  // slice-hash lanes of a batched launch
+
+// Estimated device time of one recovery row of a job and of one slice's
+// hashes, from the context's measured rates (CostRates): a row block (16
+// rows) of the job is S / 512 units of mfma_steps(n) K steps each, spread
+// over the encode workgroups (all CUs but the hash roles' ~24); a slice is
+// its MD5 blocks on one lane.
+static double ms_per_row(const Ctx *c, uint32_t n, uint64_t S) {
+    const double e = c->num_cus > 32 ? c->num_cus - 24.0 : c->num_cus;
+    return (double)((S + 511) / 512) * mfma_steps(n) * c->rates.step_us / 16.0 / e / 1000.0;
+}
+static double ms_per_slice(const Ctx *c, uint64_t S) {
+    return (double)((S + 8) / 64 + 1) * c->rates.slice_block_us / 1000.0;
+}
 
 // First-fit allocator over one device allocation, 256-byte granules.
 struct Pool {
@@ -90,7 +101,6 @@ struct Request {
     p2g_batch_job job{};
     uint32_t rows_assigned = 0, rows_done = 0;
     uint32_t slices_assigned = 0, slices_done = 0;
-    double ms_per_row = 0, ms_per_slice = 0;  // estimated device time (encode row; one slice's hash)
     double t_submit = 0, t_admit = 0, t_done = 0;  // ms on the server clock
 };
 
@@ -187,14 +197,15 @@ int Server::launch_step(int k) {
         if (q->rows_assigned == q->R && q->slices_assigned == q->n) continue;
         const uint32_t left = q->R - q->rows_assigned;
         uint32_t take = left;
-        if (left && q->ms_per_row * left > step_ms) {
+        const double row_ms = ms_per_row(c, q->n, q->S);
+        if (left && row_ms * left > step_ms) {
             // a piece of whole row blocks sized to the step target
-            const uint32_t rows = (uint32_t)(step_ms / q->ms_per_row);
+            const uint32_t rows = (uint32_t)(step_ms / row_ms);
             take = std::min(left, std::max<uint32_t>(16, rows / 16 * 16));
         }
         // slice checksums: as many rounds of the slice lanes as fit the target
         const uint32_t sleft = q->n - q->slices_assigned;
-        const uint32_t rounds = std::max<uint32_t>(1, (uint32_t)(step_ms / q->ms_per_slice));
+        const uint32_t rounds = std::max<uint32_t>(1, (uint32_t)(step_ms / ms_per_slice(c, q->S)));
         const uint32_t stake = (uint32_t)std::min<uint64_t>(sleft, (uint64_t)rounds * kServeSliceLanes);
         PlanItem it{&q->job, q->rows_assigned, q->rows_assigned + take, q->slices_assigned,
                     q->slices_assigned + stake};
@@ -337,8 +348,6 @@ int p2g_server_submit(p2g_server *srv, const p2g_request *r, uint64_t *id_out) {
         q->footprint = bytes + a256((uint64_t)q->R * q->S) + a256(16 * n) + a256(4 * n) + a256(16ull * q->R);
         if (q->footprint > s->pool.size)
             return c->fail(P2G_ERR_OUT_OF_MEMORY, "request larger than the server's HBM budget");
-        q->ms_per_row = 256.0 * (double)n * (double)q->S / kServeOpsPerMs;
-        q->ms_per_slice = (double)((q->S + 8) / 64 + 1) * kServeMd5BlockMs;
         q->id = s->next_id++;
         q->t_submit = s->now_ms();
         *id_out = q->id;

@@ -103,7 +103,13 @@ int main() {
         const uint32_t cus = cus_list[lcg(rng) % 9];
         const uint32_t resident = t % 3 ? cus : 1 + lcg(rng) % (2 * cus);
         BatchGroupLaunch g{};
-        plan_batch_group(cus, resident, tasks, slices, units, w, &g);
+        CostRates rates;
+        if (t % 2) {  // rates a run measured, within the range update_rates keeps
+            update_rates(rates, w, cus, 2, 1, 1000.0 + lcg(rng) % 100000, 1000.0 + lcg(rng) % 50000);
+            CHECK(rates.step_us >= 0.19 && rates.step_us <= 0.76);
+            CHECK(rates.slice_block_us >= 0.65 && rates.slice_block_us <= 2.62);
+        }
+        plan_batch_group(cus, resident, tasks, slices, units, w, rates, &g);
         CHECK(g.h3_blocks <= 8 && g.h2_blocks <= 64);
         if (units) {
             check_roles(g.grid, g.h2_blocks, g.h3_blocks);
