@@ -1663,20 +1663,21 @@ hipError_t launch_recovery_md5(const uint8_t *d_rec, uint32_t count, uint32_t S,
 // Fused create pass: encode, slice MD5 and packet MD5 in one launch
 // ======================================================================
 //
-// One workgroup per CU.  Workgroups [0, h3) hash recovery packets (one lane
-// per packet), [h3, h3 + h2) hash slices (one lane per slice), both on wave
-// 0, the other waves of the slice-hash workgroups compute slice CRCs; the
-// rest take encode units in a fixed stride.  A unit is one row block (8
-// tiles = 16 recovery rows) of one 512-byte column group, all K steps.  Units
-// go band by band (next_band_width column groups per band, narrow at both
-// ends) and row block by row block inside a band, so the workgroups
-// running at one time share the A fragments of one or two row blocks and
-// re-read only the source columns of one band.  Finished units are counted
-// in their band's counter (stores released at agent scope first); a packet
-// lane hashes the blocks of band b once that count is full (acquire), so the
-// packet hashes run under the encode and only the last, narrowest band is
-// hashed after it.  The MD5 workgroups keep one CU each, one wave
-// per SIMD, for the serial hashes of their lanes.
+// One workgroup per CU, producers first: workgroups [0, E) take encode
+// units, [E, E + h2) hash slices (one lane per slice, on wave 0; waves 1-3
+// compute slice CRCs) and then take encode units too, and the last h3 hash
+// recovery packets (one lane per packet), so the roles that wait are
+// dispatched after the ones they wait on.  A unit is one row block (8 tiles
+// = 16 recovery rows) of one 512-byte column group, all K steps.  Units go
+// band by band (next_band_width column groups per band, narrow at both ends)
+// and row block by row block inside a band, so the workgroups running at one
+// time share the A fragments of a few row blocks and the source columns of
+// one band; they come from per-XCD work queues (UnitQueue).  Finished units
+// are counted in their band's counter (stores released at agent scope
+// first); a packet lane hashes the blocks of band b once that count is full
+// (acquire), so the packet hashes run under the encode and only the last,
+// narrowest band is hashed after it.  The MD5 workgroups keep one CU each,
+// one wave per SIMD, for the serial hashes of their lanes.
 
 struct PassArgs {
     const SliceDesc *inputs;

@@ -153,7 +153,8 @@ extern "C" int p2g_dev_create_pass_noalloc(p2g_ctx *ctx, const p2g_dev_file *fil
         if (fixed > kNoAllocScratchBytes)
             return c->fail(P2G_ERR_INVALID_ARGUMENT,
                            "set too large for p2g_dev_create_pass_noalloc: its scratch exceeds the "
-                           "64 MiB limit (A fragments take 1 KiB per 16 recovery blocks x 16 slices)");
+                           "64 MiB limit (A fragments take 128 bytes per recovery block per slice, "
+                           "rounded up to 16 recovery blocks x 16 slices)");
         // partial slices are staged (whole-slice loads) when their copies fit
         // the rest of the array, else read in place
         const uint64_t stage_b = a256((uint64_t)parts * S);
