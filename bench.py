@@ -165,10 +165,17 @@ def run_batch_mix(ctx, n_jobs: int, reps: int, tune: bool = True) -> dict:
         for c in picks:
             if c["variant"] == p2.VARIANT_MFMA:
                 bands[str(c["band_cols"])] = bands.get(str(c["band_cols"]), 0) + 1
+        names = {p2.VARIANT_MFMA: "mfma", p2.VARIANT_BITSLICE: "bitslice",
+                 p2.VARIANT_NIBBLE: "nibble"}
+        encode_kernels = {}
+        for shape in shapes.values():
+            k = names.get(ctx.tune_get_encode(*shape)["variant"], "other")
+            encode_kernels[k] = encode_kernels.get(k, 0) + 1
         tuned = {"buckets": len(picks), "seconds": round(t_tune, 2),
                  "mfma_pass": sum(1 for c in picks if c["variant"] == p2.VARIANT_MFMA),
                  "bitslice": sum(1 for c in picks if c["variant"] == p2.VARIANT_BITSLICE),
-                 "mfma_band_cols": bands}
+                 "nibble": sum(1 for c in picks if c["variant"] == p2.VARIANT_NIBBLE),
+                 "mfma_band_cols": bands, "encode_kernel": encode_kernels}
         log(f"bench: autotuned {len(picks)} buckets in {t_tune:.1f}s: {tuned}")
 
     def plain_run():

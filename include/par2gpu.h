@@ -245,7 +245,9 @@ int p2g_dev_create_batch(p2g_ctx *ctx, const p2g_batch_job *jobs, uint32_t njobs
  * Serving (no reference counterpart): a request loop for a service that
  * creates PAR2 sets on demand.  Requests are admitted in arrival order
  * while their footprint (files, recovery blocks, digests) fits an HBM
- * budget taken at open; admitted requests stay resident and run in
+ * budget taken at open (a request that does not fit lets later ones that
+ * fit go ahead of it for a bounded time, p2g_server_set_bypass);
+ * admitted requests stay resident and run in
  * continuous batching steps (batched launches over every admitted job with
  * work left; a job longer than the step target runs a piece of its
  * recovery rows per step), so requests that arrive while the device is
@@ -271,6 +273,11 @@ typedef struct {
 /* hbm_budget: bytes of device memory for admitted requests (allocated at
  * open); step_ms: target device time of one step (0: 50 ms). */
 int p2g_server_open(p2g_ctx *ctx, uint64_t hbm_budget, double step_ms, p2g_server **out);
+/* How long a queued request that finds no memory may be overtaken by later
+ * requests that fit, from when it first found none (default: 4 step
+ * targets, 200 ms at the default target; 0: strict arrival order).  After
+ * that it holds back every later request until it is admitted. */
+int p2g_server_set_bypass(p2g_server *s, double bypass_ms);
 /* Queues a request; id_out identifies it.  Fails at once for a request
  * that could never be admitted. */
 int p2g_server_submit(p2g_server *s, const p2g_request *req, uint64_t *id_out);

@@ -45,7 +45,7 @@ EXPORTED_SYMBOLS = [
     "p2g_batch_plan", "p2g_batch_launch", "p2g_batch_wait", "p2g_batch_job_ms",
     "p2g_batch_free", "p2g_dev_create_batch", "p2g_dev_fill_xorshift64",
     "p2g_server_open", "p2g_server_submit", "p2g_server_poll", "p2g_server_result",
-    "p2g_server_release", "p2g_server_stats", "p2g_server_close",
+    "p2g_server_release", "p2g_server_stats", "p2g_server_close", "p2g_server_set_bypass",
     "p2g_tune_bucket", "p2g_autotune", "p2g_tune_get", "p2g_tune_save", "p2g_tune_load",
     "p2g_plan_roles", "p2g_dev_create_pass_noalloc", "p2g_tune_get_encode",
 ]
@@ -147,6 +147,7 @@ def _load() -> ctypes.CDLL:
         "p2g_dev_fill_xorshift64": (ctypes.c_int, [P, P, ctypes.c_uint64, ctypes.c_uint64, P]),
         "p2g_server_open": (ctypes.c_int, [P, ctypes.c_uint64, ctypes.c_double, ctypes.POINTER(P)]),
         "p2g_server_submit": (ctypes.c_int, [P, P, ctypes.POINTER(ctypes.c_uint64)]),
+        "p2g_server_set_bypass": (ctypes.c_int, [P, ctypes.c_double]),
         "p2g_server_poll": (ctypes.c_int, [P, ctypes.c_double, ctypes.POINTER(ctypes.c_uint64),
                                            ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32)]),
         "p2g_server_result": (ctypes.c_int, [P, ctypes.c_uint64, P, ctypes.POINTER(ctypes.c_double)]),
@@ -359,6 +360,11 @@ class Server:
         ctx._check(lib.p2g_server_open(ctx.h, hbm_budget, step_ms, ctypes.byref(h)))
         self.h = h
         self._keep = {}
+
+    def set_bypass(self, bypass_ms: float):
+        """How long a request waiting for memory may be overtaken by later
+        requests that fit (p2g_server_set_bypass; 0: arrival order)."""
+        self.ctx._check(lib.p2g_server_set_bypass(self.h, bypass_ms))
 
     def submit(self, file_lens: Sequence[int], slice_size: int, recovery_count: int,
                set_id: bytes, host_data: Optional[Sequence[bytes]] = None,

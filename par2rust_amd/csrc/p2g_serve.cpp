@@ -5,8 +5,9 @@
 // A request brings its files (host bytes to upload, or a seed per file for
 // the synthetic xorshift64 stream written on the device) and its slice size,
 // recovery count and set id.  Requests are admitted in arrival order while
-// their footprint (files, recovery blocks, digests) fits the budget; they
-// take their memory from one pool allocated at open.
+// their footprint (files, recovery blocks, digests) fits the budget; one that
+// does not fit lets later ones that fit go ahead of it for a bounded time
+// (Server::admit).  They take their memory from one pool allocated at open.
 //
 // A step is one batched plan (build_plan) over every admitted job with work
 // left: whole jobs when their estimated device time fits the step target
@@ -102,6 +103,7 @@ struct Request {
     uint32_t rows_assigned = 0, rows_done = 0;
     uint32_t slices_assigned = 0, slices_done = 0;
     double t_submit = 0, t_admit = 0, t_done = 0;  // ms on the server clock
+    double t_blocked = -1;  // when admission first found no memory for it (-1: not yet)
 };
 
 struct Step {
@@ -114,6 +116,9 @@ struct Server {
     Ctx *c = nullptr;
     Pool pool;
     double step_ms = 50.0;
+    // How long a request that finds no memory may be overtaken by later
+    // requests that fit (0: strict arrival order).
+    double bypass_ms = 200.0;
     uint64_t next_id = 1;
     std::deque<Request *> queued;
     std::vector<Request *> active;
@@ -148,12 +153,23 @@ Server::~Server() {
 
 static uint64_t a256(uint64_t x) { return (x + 255) / 256 * 256; }
 
+// Admits queued requests in arrival order while they fit.  One that does not
+// fit lets later requests that fit go ahead of it for bypass_ms from when it
+// first found no memory; after that it holds back every later request until
+// it is admitted, so a large request waits at most that long plus the time
+// the memory it needs takes to come free.
 int Server::admit() {
-    while (!queued.empty()) {
-        Request *q = queued.front();
+    const double now = now_ms();
+    for (auto it = queued.begin(); it != queued.end();) {
+        Request *q = *it;
         uint64_t off = 0;
-        if (!pool.alloc(q->footprint, &off)) break;  // arrival order: wait for memory
-        queued.pop_front();
+        if (!pool.alloc(q->footprint, &off)) {
+            if (q->t_blocked < 0) q->t_blocked = now;
+            if (now - q->t_blocked >= bypass_ms) break;
+            ++it;
+            continue;
+        }
+        it = queued.erase(it);
         q->off = off;
         uint8_t *p = pool.base + off;
         q->files.resize(q->lens.size());
@@ -300,6 +316,7 @@ int p2g_server_open(p2g_ctx *ctx, uint64_t hbm_budget, double step_ms, p2g_serve
         if (!s) return c->fail(P2G_ERR_OUT_OF_MEMORY, "server");
         s->c = c;
         if (step_ms > 0) s->step_ms = step_ms;
+        s->bypass_ms = 4 * s->step_ms;
         s->pool.size = a256(hbm_budget);
         void *p = nullptr;
         hipError_t e = hipMalloc(&p, s->pool.size);
@@ -316,6 +333,15 @@ int p2g_server_open(p2g_ctx *ctx, uint64_t hbm_budget, double step_ms, p2g_serve
         }
         for (Step &st : s->steps) st.plan.afrag = &s->afrag;
         *out = reinterpret_cast<p2g_server *>(s);
+        return P2G_OK;
+    });
+}
+
+int p2g_server_set_bypass(p2g_server *srv, double bypass_ms) {
+    return guarded(srv ? reinterpret_cast<Server *>(srv)->c : nullptr, [&]() -> int {
+        Server *s = reinterpret_cast<Server *>(srv);
+        if (!s || !(bypass_ms >= 0)) return P2G_ERR_INVALID_ARGUMENT;
+        s->bypass_ms = bypass_ms;
         return P2G_OK;
     });
 }

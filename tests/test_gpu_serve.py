@@ -78,10 +78,15 @@ def test_server_row_pieces_across_steps(ctx):
     srv.close()
 
 
+ADMIT_REQS = [([500000], 4096, 20), ([450000, 3], 4096, 12), ([400000], 2048, 9)]
+
+
 def test_server_admission_waits_for_memory(ctx):
-    # room for about one request at a time: the others wait in the queue
-    reqs = [([500000], 4096, 20), ([450000, 3], 4096, 12), ([400000], 2048, 9)]
+    # room for about one request at a time: the others wait in the queue,
+    # in arrival order (no overtaking)
+    reqs = ADMIT_REQS
     srv = ctx.server(1 << 20)
+    srv.set_bypass(0.0)
     ids = {}
     for k, (lens, S, R) in enumerate(reqs):
         files = [data(n, 1700 + 10 * k + i) for i, n in enumerate(lens)]
@@ -100,6 +105,66 @@ def test_server_admission_waits_for_memory(ctx):
         assert time.time() - t < 60, "server made no progress"
         finished = srv.poll(50.0)
     assert done == sorted(ids)  # admitted and finished in arrival order
+    srv.close()
+
+
+def submit_admit_reqs(srv, ids, ks):
+    for k in ks:
+        lens, S, R = ADMIT_REQS[k]
+        files = [data(n, 1700 + 10 * k + i) for i, n in enumerate(lens)]
+        sid = data(16, 1900 + k)
+        ids[srv.submit(lens, S, R, sid, host_data=files)] = (files, S, R, sid)
+
+
+def finish_all(srv, ids, finished=()):
+    """Polls until every request is done (`finished`: ids earlier polls
+    reported), checks each against the oracle, releases it; returns
+    {id: admit time}."""
+    admitted, t = {}, time.time()
+    finished = list(finished)
+    while len(admitted) < len(ids):
+        for rid in finished:
+            files, S, R, sid = ids[rid]
+            assert srv.download(rid) == alone(files, S, R, sid), rid
+            admitted[rid] = srv.result(rid)[1][1]
+            srv.release(rid)
+        assert time.time() - t < 60, "server made no progress"
+        finished = srv.poll(50.0)
+    return admitted
+
+
+def test_server_admission_bypass(ctx):
+    # A (582 KB) fits the 1 MiB budget, B (500 KB) does not beside it, C
+    # (418 KB) does: C goes ahead of the waiting B, which is admitted once A
+    # and C are released
+    srv = ctx.server(1 << 20)
+    srv.set_bypass(60000.0)
+    ids = {}
+    submit_admit_reqs(srv, ids, [0, 1, 2])
+    finished = srv.poll(0.0)
+    st = srv.stats()
+    assert st["queued"] == 1 and st["active"] + len(finished) == 2
+    a, b, c = sorted(ids)
+    t = finish_all(srv, ids, finished)
+    assert t[a] <= t[c] < t[b]
+    srv.close()
+
+
+def test_server_admission_bypass_window_ends(ctx):
+    # once B has waited its window, C queues behind it although it would fit
+    srv = ctx.server(1 << 20)
+    srv.set_bypass(50.0)
+    ids = {}
+    submit_admit_reqs(srv, ids, [0, 1])
+    finished = srv.poll(0.0)
+    assert srv.stats()["queued"] == 1
+    time.sleep(0.2)
+    submit_admit_reqs(srv, ids, [2])
+    finished += srv.poll(0.0)  # A may be done by now: not released yet
+    assert srv.stats()["queued"] == 2
+    a, b, c = sorted(ids)
+    t = finish_all(srv, ids, finished)
+    assert t[a] < t[b] <= t[c]
     srv.close()
 
 
