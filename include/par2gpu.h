@@ -45,6 +45,11 @@ enum {
 #define P2G_MAX_FILES 32768u
 #define P2G_MAX_INPUT_BLOCKS 32768u
 #define P2G_MAX_RECOVERY_BLOCKS 65535u
+/* Largest slice size the kernels take (they index a slice with 32 bits);
+ * the reference accepts any multiple of 4.  A larger slice size is refused
+ * with P2G_ERR_INVALID_ARGUMENT (a limit of this engine, not a PAR2 error).
+ * Such a set has recovery blocks of over 2 GiB each. */
+#define P2G_MAX_SLICE_BYTES 2147483648ull
 
 typedef struct p2g_ctx p2g_ctx;
 typedef struct p2g_job p2g_job;

@@ -16,7 +16,7 @@
 namespace p2g {
 
 constexpr uint32_t kMaxInputs = 32768;            // MAX_INPUT_BLOCKS, src/creator.rs:106
-constexpr uint64_t kMaxSliceBytes = 1ull << 31;   // kernels index slices with u32
+constexpr uint64_t kMaxSliceBytes = P2G_MAX_SLICE_BYTES;  // kernels index slices with u32
 constexpr uint64_t kMatsBudget = 256ull << 20;    // coefficient matrices per launch chunk
 constexpr uint64_t kPassAfragBudget = 4ull << 30; // A fragments of one fused create pass
 constexpr uint64_t kStageBudget = 256ull << 20;   // zero-padded copies of partial slices (P2G_STAGE_BYTES overrides)
