@@ -141,6 +141,12 @@ struct PassPlan {
     uint32_t grid = 0, h2_blocks = 0, h3_blocks = 0;
     uint32_t row_blocks = 0, col_groups = 0, band_cols = 0, n_bands = 0, n_units = 0;
 };
+// Default band width of the fused pass, in 512-byte column groups: the
+// packet lanes hash a band once it is encoded, so the last full band is
+// hashed after the encode ends; 32 column groups (16 KiB of each packet,
+// 256 MD5 blocks) keep that tail short at the 512 KiB-slice headline shape,
+// where 24-32 measured 5.17-5.20 ms per pass against 5.31-5.35 at 64.
+constexpr uint32_t kPassDefaultBand = 32;
 // False when the fused pass does not apply (no inputs or recovery blocks,
 // more inputs than one encode launch takes, too few CUs).  band_cols = 0
 // picks the default band width.  max_resident: how many workgroups of the

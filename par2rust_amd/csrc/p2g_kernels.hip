@@ -1355,11 +1355,14 @@ __device__ __forceinline__ uint32_t crc_shift_op(const CrcPowers &pw, uint32_t n
     return p;
 }
 
-// Four bytes per step (slicing by 4), reflected: t[0] is the byte table and
-// t[k][i] = t[k-1][i] >> 8 ^ t[0][t[k-1][i] & 0xFF], the effect of byte i
-// followed by k zero bytes.  Filled by a whole 256-thread workgroup.
+// Reflected table CRC: t[0] is the byte table and t[k][i] = t[k-1][i] >> 8
+// ^ t[0][t[k-1][i] & 0xFF], the effect of byte i followed by k zero bytes;
+// four tables step four bytes at a time (slicing by 4), sixteen step sixteen.
+// Filled by a whole 256-thread workgroup.
+constexpr uint32_t kCrcAhead = 8;  // 16-byte loads in flight per lane ahead of the CRC steps
+
 struct CrcLds {
-    uint32_t t[4][256];
+    uint32_t t[16][256];
     uint32_t part[4];  // per-wave partial CRCs
 };
 
@@ -1369,7 +1372,7 @@ __device__ __forceinline__ void crc_tables_init(CrcLds &L) {
     for (int k = 0; k < 8; k++) c = (c & 1u) ? (kCrcPoly ^ (c >> 1)) : (c >> 1);
     L.t[0][i] = c;
     __syncthreads();
-    for (int k = 1; k < 4; k++) {
+    for (int k = 1; k < 16; k++) {
         const uint32_t q = L.t[k - 1][i];
         L.t[k][i] = (q >> 8) ^ L.t[0][q & 0xFFu];
         __syncthreads();
@@ -1416,8 +1419,11 @@ __device__ __forceinline__ void crc_tables_init_wave(uint32_t (*t)[256]) {
 }
 
 // CRC-32 of one zero-padded S-byte slice by one wave: lane l takes bytes
-// [l seg, (l + 1) seg), merged in a shuffle tree as in crc_slice_block.
-// Lane 0 stores the result.
+// [l seg, (l + 1) seg), merged in a shuffle tree as in crc_slice_block.  t
+// holds the 16 tables of slicing by 16 (crc_tables_init_wave<16>).  A lane's
+// 16-byte loads run kCrcAhead chunks ahead of its CRC steps, so the steps
+// wait on one load latency per segment rather than one per chunk.  Lane 0
+// stores the result.
 __device__ __forceinline__ void crc_slice_wave(const uint32_t (*t)[256], const SliceDesc d,
                                                uint32_t S, bool a16, const CrcPowers &pw,
                                                uint32_t *out) {
@@ -1428,14 +1434,25 @@ __device__ __forceinline__ void crc_slice_wave(const uint32_t (*t)[256], const S
     uint32_t c = 0xFFFFFFFFu;
     uint32_t o = lo;
     if (a16) {
+        // whole 16-byte chunks inside the real bytes, then words
         const uint32_t whole = d.valid > lo ? lo + (min(hi, d.valid) - lo) / 16 * 16 : lo;
-        for (; o < whole; o += 16) {
-            const uint4 v = gload(reinterpret_cast<const uint4 *>(d.ptr + o));
-            c = crc_step4(t, c, v.x);
-            c = crc_step4(t, c, v.y);
-            c = crc_step4(t, c, v.z);
-            c = crc_step4(t, c, v.w);
+        const uint32_t nch = (whole - lo) / 16;
+        const uint4 *p = reinterpret_cast<const uint4 *>(d.ptr + lo);
+        uint4 ring[kCrcAhead];
+#pragma unroll
+        for (uint32_t k = 0; k < kCrcAhead; k++)
+            ring[k] = k < nch ? gload(p + k) : make_uint4(0, 0, 0, 0);
+        for (uint32_t i = 0; i < nch; i += kCrcAhead) {
+#pragma unroll
+            for (uint32_t k = 0; k < kCrcAhead; k++) {
+                if (i + k < nch) {
+                    const uint4 v = ring[k];
+                    if (i + k + kCrcAhead < nch) ring[k] = gload(p + i + k + kCrcAhead);
+                    c = crc_step16(t, c, v.x, v.y, v.z, v.w);
+                }
+            }
         }
+        o = whole;
     }
     for (; o < hi; o += 4) c = crc_step4(t, c, load_word(d.ptr, o, d.valid));
     c ^= 0xFFFFFFFFu;
@@ -1464,15 +1481,26 @@ __device__ __forceinline__ void crc_slice_block(CrcLds &L, const SliceDesc d, ui
     uint32_t c = 0xFFFFFFFFu;
     uint32_t o = lo;
     if (a16) {
-        // whole 16-byte chunks inside the real bytes, then words
+        // whole 16-byte chunks inside the real bytes (slicing by 16, loads
+        // kCrcAhead chunks ahead as in crc_slice_wave), then words
         const uint32_t whole = d.valid > lo ? lo + (min(hi, d.valid) - lo) / 16 * 16 : lo;
-        for (; o < whole; o += 16) {
-            const uint4 v = gload(reinterpret_cast<const uint4 *>(d.ptr + o));
-            c = crc_step4(L, c, v.x);
-            c = crc_step4(L, c, v.y);
-            c = crc_step4(L, c, v.z);
-            c = crc_step4(L, c, v.w);
+        const uint32_t nch = (whole - lo) / 16;
+        const uint4 *p = reinterpret_cast<const uint4 *>(d.ptr + lo);
+        uint4 ring[kCrcAhead];
+#pragma unroll
+        for (uint32_t k = 0; k < kCrcAhead; k++)
+            ring[k] = k < nch ? gload(p + k) : make_uint4(0, 0, 0, 0);
+        for (uint32_t i = 0; i < nch; i += kCrcAhead) {
+#pragma unroll
+            for (uint32_t k = 0; k < kCrcAhead; k++) {
+                if (i + k < nch) {
+                    const uint4 v = ring[k];
+                    if (i + k + kCrcAhead < nch) ring[k] = gload(p + i + k + kCrcAhead);
+                    c = crc_step16(L.t, c, v.x, v.y, v.z, v.w);
+                }
+            }
         }
+        o = whole;
     }
     for (; o < hi; o += 4) c = crc_step4(L, c, load_word(d.ptr, o, d.valid));
     c ^= 0xFFFFFFFFu;
@@ -1840,8 +1868,12 @@ __global__ __launch_bounds__(kMfThreads, 1) void k_create_pass(PassArgs a) {
     if (role.role == kPassSlice) {
         if (tid >= 64) {
             const uint32_t cw = role.index * 3 + (tid >> 6) - 1, n_cw = 3 * a.h2_blocks;
-            uint32_t(*t)[256] = reinterpret_cast<uint32_t(*)[256]>(lds.desc) + 4 * ((tid >> 6) - 1);
-            crc_tables_init_wave(t);
+            // 16 tables (16 KiB) per CRC wave, 48 KiB in all: the descriptor
+            // table and the register-ring A buffer of lds, unused until the
+            // workgroup encodes
+            static_assert(sizeof(MfLds) >= 3 * 16 * 256 * 4, "three waves of 16 CRC tables");
+            uint32_t(*t)[256] = reinterpret_cast<uint32_t(*)[256]>(&lds) + 16 * ((tid >> 6) - 1);
+            crc_tables_init_wave<16>(t);
             if (tid % 64 == 0) pass_stamp_start(a, kPassCrc);
             for (uint32_t sl = cw; sl < a.n; sl += n_cw)
                 crc_slice_wave(t, a.inputs[sl], a.S, a.S % 16 == 0, a.pw, a.slice_crc + sl);

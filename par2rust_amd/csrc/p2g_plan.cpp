@@ -23,7 +23,7 @@ bool plan_create_pass(uint32_t n, uint32_t R, uint32_t S, uint32_t num_cus, uint
     if (4 * (p->h2_blocks + p->h3_blocks) >= p->grid) return false;
     p->row_blocks = mfma_row_tiles(R) / kMfTiles;
     p->col_groups = (S + kMfThreads * 2 - 1) / (kMfThreads * 2);
-    p->band_cols = std::max<uint32_t>(1, std::min(band_cols ? band_cols : 64u, p->col_groups));
+    p->band_cols = std::max<uint32_t>(1, std::min(band_cols ? band_cols : kPassDefaultBand, p->col_groups));
     p->n_bands = 0;
     for (uint32_t c = 0; c < p->col_groups; p->n_bands++)
         c += next_band_width(c, p->col_groups - c, p->band_cols);
