@@ -504,37 +504,38 @@ hipError_t launch_encode_nibble(const SliceDesc *d_inputs, uint32_t n_inputs, co
 //
 // One MFMA covers 32 rows (recovery rows 2t, 2t + 1), 64 k (inputs
 // 4g .. 4g + 3) and 32 columns (symbols).  A lane in half h (lane / 32)
-// holds k values of inputs 4g + 2h and 4g + 2h + 1, 16 nibbles each (two
-// dwords per input), the same order in A and in B, so each (A lane, nibble)
-// meets the B nibble of the same k.
+// holds k values of inputs i0 = 4g + 2h and i1 = 4g + 2h + 1, 16 bits each,
+// as four dwords of eight nibbles, the same order in A and in B, so each
+// (A lane, nibble) meets the B nibble of the same k.
 //
-// Within an input, nibble n of dword d (d = 0, 1; n = 0..7) carries symbol
-// bit 4 (n % 4) + n / 4 + 2 d.  Any order shared by A and B gives the same
-// sums; this one makes the B build of a loaded word (two symbols) four
-// masked shifts and four byte permutes.
+// Nibble n of dword d carries bit 4 (n % 4) + d of input i0 (n < 4) or i1
+// (n >= 4).  Any order shared by A and B gives the same sums; this one makes
+// the B operand of a symbol one byte permute (the symbol's 16 bits of both
+// inputs side by side) and four masks of the result, with the set bit left
+// where it lies in its nibble: fp4 0.5 (bit 0), 1.0 (bit 1), 2.0 (bit 2);
+// bit 3 (the fp4 sign) is shifted down to bit 2 (2.0).  The A nibble of the
+// same k is the reciprocal (2.0, 1.0, 0.5, 0.5 for d = 0..3), so every
+// product is exactly 1.0 and the sums stay integers.
 
 typedef int v8i_t __attribute__((ext_vector_type(8)));
 typedef float v16f_t __attribute__((ext_vector_type(16)));
 
-// B dwords of the two symbols of word w: fp4 1.0 (nibble 0x2) where the bit
-// is set.  m0..m3 hold bits 4q, 4q + 1, 4q + 2, 4q + 3 of w at bit 1 of
-// nibble q; the permutes gather the low (symbol 0) or high (symbol 1) halves.
-__device__ __forceinline__ void mf_b_words(uint32_t w, uint32_t out[4]) {
-    const uint32_t m0 = (w << 1) & 0x22222222u, m1 = w & 0x22222222u;
-    const uint32_t m2 = (w >> 1) & 0x22222222u, m3 = (w >> 2) & 0x22222222u;
-    out[0] = __builtin_amdgcn_perm(m1, m0, 0x05040100u);  // symbol 0, dword 0
-    out[1] = __builtin_amdgcn_perm(m3, m2, 0x05040100u);  // symbol 0, dword 1
-    out[2] = __builtin_amdgcn_perm(m1, m0, 0x07060302u);  // symbol 1, dword 0
-    out[3] = __builtin_amdgcn_perm(m3, m2, 0x07060302u);  // symbol 1, dword 1
+// B dwords of one symbol: x = the symbol's 16 bits of input i0 (low half)
+// and of input i1 (high half).
+__device__ __forceinline__ void mf_b_dwords(uint32_t x, uint32_t out[4]) {
+    out[0] = x & 0x11111111u;         // bits 4q     -> 0.5
+    out[1] = x & 0x22222222u;         // bits 4q + 1 -> 1.0
+    out[2] = x & 0x44444444u;         // bits 4q + 2 -> 2.0
+    out[3] = (x >> 1) & 0x44444444u;  // bits 4q + 3 -> 2.0
 }
 
-// A dword d of matrix row m (bit b = coefficient of symbol bit b), in the
-// same nibble order: nibble n holds bit 4 (n % 4) + n / 4 + 2 d of m at its
-// bit 1, so dword 0 takes bits 4q (nibbles 0-3) and 4q + 1 (nibbles 4-7),
-// dword 1 bits 4q + 2 and 4q + 3.
-__host__ __device__ __forceinline__ uint32_t mf_a_word(uint32_t m, int d) {
-    return d == 0 ? ((m & 0x1111u) << 1) | ((m & 0x2222u) << 16)
-                  : ((m & 0x4444u) >> 1) | ((m & 0x8888u) << 14);
+// A dword d for the matrix rows m0 (input i0) and m1 (input i1) of one
+// output bit (bit b = coefficient of symbol bit b): nibble n holds bit
+// 4 (n % 4) + d of its row as the fp4 reciprocal of the B value of dword d.
+__host__ __device__ __forceinline__ uint32_t mf_a_dword(uint32_t m0, uint32_t m1, int d) {
+    const int sh = d == 0 ? 2 : d == 1 ? 1 : 0;  // 2.0 (0x4), 1.0 (0x2), 0.5 (0x1)
+    const uint32_t lo = ((m0 >> d) & 0x1111u) << sh, hi = ((m1 >> d) & 0x1111u) << sh;
+    return lo | (hi << 16);
 }
 
 // E8M0 block scale of 1.0 (biased exponent 127) for both operands; a scale
@@ -598,25 +599,25 @@ __device__ __forceinline__ uint4 mfma_afrag_entry(const AfragLane &L,
                                                   uint32_t g, uint32_t lane) {
     const uint32_t row = lane & 31, h = lane >> 5;
     const uint32_t r = 2 * t + (row >> 4);
-    uint32_t d[4] = {0, 0, 0, 0};
+    uint32_t m[2] = {0, 0};
 #pragma unroll
     for (uint32_t e = 0; e < 2; e++) {
         const uint32_t i = 4 * g + 2 * h + e;
         if (r >= R || i >= n) continue;
         // exponents and logs are below 65535, so the product fits 32 bits
         const uint32_t v = antilog[mod_field_limit((uint32_t)logbase[i] * (first_exp + r))];
-        uint32_t m = 0;  // row j of the matrix of multiplication by v
+        uint32_t me = 0;  // row j of the matrix of multiplication by v
 #pragma unroll
         for (int k = 0; k < 16; k++) {
-            // all ones when bit k of v is set; then m ^= rows[k] & mask as one
-            // v_bitop3 (truth-table bit (m << 2 | row << 1 | mask): 0x78)
+            // all ones when bit k of v is set; then me ^= rows[k] & mask as one
+            // v_bitop3 (truth-table bit (me << 2 | row << 1 | mask): 0x78)
             const uint32_t mask = (uint32_t)__builtin_amdgcn_sbfe((int)v, k, 1);
-            m = __builtin_amdgcn_bitop3_b32(m, L.rows[k], mask, 0x78);
+            me = __builtin_amdgcn_bitop3_b32(me, L.rows[k], mask, 0x78);
         }
-        d[2 * e] = mf_a_word(m, 0);
-        d[2 * e + 1] = mf_a_word(m, 1);
+        m[e] = me;
     }
-    return make_uint4(d[0], d[1], d[2], d[3]);
+    return make_uint4(mf_a_dword(m[0], m[1], 0), mf_a_dword(m[0], m[1], 1),
+                      mf_a_dword(m[0], m[1], 2), mf_a_dword(m[0], m[1], 3));
 }
 
 // Every fragment (t, g) of tiles x nq, entry (t * nq + g) * 64 + lane: a
@@ -722,13 +723,13 @@ __device__ __forceinline__ void mf_load_words(const uint4 *s_desc, const SliceDe
 }
 
 // B operands of the two column tiles (cb = symbol 0 / 1 of each word) from
-// the words of inputs 4g + 2h (w0) and 4g + 2h + 1 (w1).
+// the words of inputs 4g + 2h (w0) and 4g + 2h + 1 (w1): 12 VALU.
 __device__ __forceinline__ void mf_make_b(uint32_t w0, uint32_t w1, v8i_t B[2]) {
     uint32_t p[4], q[4];
-    mf_b_words(w0, p);
-    mf_b_words(w1, q);
-    B[0] = v8i_t{(int)p[0], (int)p[1], (int)q[0], (int)q[1], 0, 0, 0, 0};
-    B[1] = v8i_t{(int)p[2], (int)p[3], (int)q[2], (int)q[3], 0, 0, 0, 0};
+    mf_b_dwords(__builtin_amdgcn_perm(w1, w0, 0x05040100u), p);  // symbol 0 of both
+    mf_b_dwords(__builtin_amdgcn_perm(w1, w0, 0x07060302u), q);  // symbol 1 of both
+    B[0] = v8i_t{(int)p[0], (int)p[1], (int)p[2], (int)p[3], 0, 0, 0, 0};
+    B[1] = v8i_t{(int)q[0], (int)q[1], (int)q[2], (int)q[3], 0, 0, 0, 0};
 }
 
 // LDS of the encode: the padded descriptor table and the A double buffer.
