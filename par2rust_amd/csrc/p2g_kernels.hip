@@ -931,24 +931,31 @@ __device__ __forceinline__ void mf_unit_dma(const uint4 *s_desc, uint4 *A0, uint
                                             uint8_t *__restrict__ rec, int accumulate,
                                             uint32_t tile0, uint32_t colbase) {
     (void)inputs;
-    const uint32_t tid = threadIdx.x;
-    const uint32_t lane = tid & 63, w = tid >> 6;
+    const uint32_t lane = threadIdx.x & 63;
+    // the wave index as a scalar, so the DMA's LDS destinations (M0) are
+    // scalar arithmetic
+    const uint32_t w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const uint32_t c = lane & 31, h = lane >> 5;
-    const uint32_t off = colbase + w * 128 + 4 * c;
-    const bool col_ok = off < S;
-    const uint32_t off_ld = min(off, S - 4);
+    // DMA share of wave w: column quarter w (128 bytes) of the step's four
+    // inputs and the A fragments of tiles w and w + 4
+    const uint32_t off_ld = min(colbase + w * 128 + 4 * c, S - 4);
     const uint4 *ga0 = afrag + ((size_t)(tile0 + w) * frag_stride) * 64 + lane;
     const uint4 *ga1 = afrag + ((size_t)(tile0 + 4 + w) * frag_stride) * 64 + lane;
-    // this lane's words in a ring buffer: run 2 w (word 0) and 2 w + 1 (word 1)
-    const uint32_t wrd0 = kMfTiles * 64 * 4 + (2 * w) * 64 + lane;
+    // Compute share of wave w = 2 wr + wc: row tiles 4 wr .. 4 wr + 3 over
+    // column quarters 2 wc and 2 wc + 1.  Column tile 2 p + s is symbol s of
+    // the lane's word in quarter 2 wc + p; the word of input 4g + 2h + j
+    // there is run 2 (2 wc + p) + j, position lane, of the ring buffer.
+    const uint32_t wr = w >> 1, wc = w & 1;
+    const uint32_t wrd = kMfTiles * 64 * 4 + (4 * wc) * 64 + lane;
+    const uint32_t ard = (4 * wr) * 64 + lane;
 
-    v16f_t acc[kMfTiles][2];
+    v16f_t acc[4][4];
 #pragma unroll
-    for (int a = 0; a < kMfTiles; a++)
+    for (int a = 0; a < 4; a++)
 #pragma unroll
-        for (int cb = 0; cb < 2; cb++)
+        for (int ct = 0; ct < 4; ct++)
 #pragma unroll
-            for (int v = 0; v < 16; v++) acc[a][cb][v] = 0.0f;
+            for (int v = 0; v < 16; v++) acc[a][ct][v] = 0.0f;
 
     // word addresses of the next DMA, read from the descriptors one step ahead
     const uint32_t *pw0, *pw1;
@@ -957,30 +964,36 @@ __device__ __forceinline__ void mf_unit_dma(const uint4 *s_desc, uint4 *A0, uint
         const uint32_t gq = min((uint32_t)(G), nq - 1);                                       \
         __builtin_amdgcn_global_load_lds(ga0 + (size_t)gq * 64, (BUF) + 64 * w, 16, 0, 0);    \
         __builtin_amdgcn_global_load_lds(ga1 + (size_t)gq * 64, (BUF) + 256 + 64 * w, 16, 0, 0); \
-        __builtin_amdgcn_global_load_lds(pw0, reinterpret_cast<uint32_t *>(BUF) + wrd0 - lane, 4, 0, 0); \
-        __builtin_amdgcn_global_load_lds(pw1, reinterpret_cast<uint32_t *>(BUF) + wrd0 + 64 - lane, 4, 0, 0); \
+        __builtin_amdgcn_global_load_lds(pw0, reinterpret_cast<uint32_t *>(BUF) + kMfTiles * 64 * 4 + 2 * w * 64, 4, 0, 0); \
+        __builtin_amdgcn_global_load_lds(pw1, reinterpret_cast<uint32_t *>(BUF) + kMfTiles * 64 * 4 + (2 * w + 1) * 64, 4, 0, 0); \
         asm volatile("" ::: "memory");                                                        \
         mf_word_addrs(s_desc, min((uint32_t)(G) + 1, nq - 1), h, off_ld, pw0, pw1);           \
     } while (0)
+    // B operands of the four column tiles from the words in BUF: the words
+    // are read first, so the B build waits only for them
 #define MF_B(BUF, OUT)                                                                        \
-    mf_make_b(reinterpret_cast<const uint32_t *>(BUF)[wrd0],                                  \
-              reinterpret_cast<const uint32_t *>(BUF)[wrd0 + 64], OUT)
+    do {                                                                                      \
+        const uint32_t *wb = reinterpret_cast<const uint32_t *>(BUF) + wrd;                   \
+        const uint32_t x00 = wb[0], x01 = wb[64], x10 = wb[128], x11 = wb[192];               \
+        mf_make_b(x00, x01, OUT);                                                             \
+        mf_make_b(x10, x11, OUT + 2);                                                         \
+    } while (0)
     mf_word_addrs(s_desc, 0, h, off_ld, pw0, pw1);
     MF_DMA(0, A0);
     MF_DMA(1, A1);
     MF_DMA(2, A2);
-    v8i_t Bc[2];
-    uint4 Ac[kMfTiles];
+    v8i_t Bc[4];
+    uint4 Ac[4];
     __builtin_amdgcn_s_waitcnt(kWaitVmcnt8);  // group 0 of this wave
     asm volatile("" ::: "memory");
     __builtin_amdgcn_s_barrier();
     asm volatile("" ::: "memory");
-#pragma unroll
-    for (int a = 0; a < kMfTiles; a++) Ac[a] = A0[a * 64 + lane];
     MF_B(A0, Bc);
+#pragma unroll
+    for (int a = 0; a < 4; a++) Ac[a] = A0[ard + a * 64];
 
     // step GG: MFMAs on Ac = A(GG), Bc = B(GG); DMA(GG + 3) into BUFW;
-    // A(GG + 1) and B(GG + 1) from BUFN.
+    // B(GG + 1) and A(GG + 1) from BUFN.
 #define MF_STEP_DMA(GG, BUFN, BUFW)                                                           \
     do {                                                                                      \
         __builtin_amdgcn_sched_barrier(0);                                                    \
@@ -999,37 +1012,34 @@ __device__ __forceinline__ void mf_unit_dma(const uint4 *s_desc, uint4 *A0, uint
         asm volatile("" ::: "memory");                                                        \
         __builtin_amdgcn_sched_barrier(0);                                                    \
         MF_DMA((GG) + 3, BUFW);                                                               \
-        uint4 An[kMfTiles];                                                                   \
-        _Pragma("unroll") for (int a = 0; a < kMfTiles; a++) An[a] = (BUFN)[a * 64 + lane];  \
-        v8i_t Bn[2];                                                                          \
+        v8i_t Bn[4];                                                                          \
         MF_B(BUFN, Bn);                                                                       \
-        _Pragma("unroll") for (int a = 0; a < kMfTiles; a++) {                                \
+        uint4 An[4];                                                                          \
+        _Pragma("unroll") for (int a = 0; a < 4; a++) An[a] = (BUFN)[ard + a * 64];           \
+        _Pragma("unroll") for (int a = 0; a < 4; a++) {                                       \
             const uint4 A = Ac[a];                                                            \
             const v8i_t av = v8i_t{(int)A.x, (int)A.y, (int)A.z, (int)A.w, 0, 0, 0, 0};       \
-            _Pragma("unroll") for (int cb = 0; cb < 2; cb++) if (a > 0 || cb > 0) acc[a][cb] = \
-                __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(av, Bc[cb], acc[a][cb], 4, 4, \
+            _Pragma("unroll") for (int ct = 0; ct < 4; ct++) if (a > 0 || ct > 0) acc[a][ct] = \
+                __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(av, Bc[ct], acc[a][ct], 4, 4, \
                                                                 0, kE8M0One, 0, kE8M0One);    \
         }                                                                                     \
-        /* Three more MFMAs straight after the barrier (their operands are */                \
-        /* in registers), so the matrix pipe runs while the DMA is issued; */                \
-        /* then the DMA, the LDS reads two per MFMA, and the B build's VALU */               \
-        /* between the last MFMAs (an MFMA issue holds the wave until the */                 \
-        /* pipe takes it, so only work placed between MFMAs overlaps them). */               \
+        /* Three MFMAs straight after the barrier (operands in registers) */                 \
+        /* while the DMA is issued, then the LDS reads two per MFMA, then */                 \
+        /* the B build's VALU three per MFMA (an MFMA issue holds the wave */                \
+        /* until the pipe takes it, so only work placed between MFMAs */                     \
+        /* overlaps them). */                                                                 \
         __builtin_amdgcn_sched_group_barrier(0x008, 3, 0);                                    \
         __builtin_amdgcn_sched_group_barrier(0x010, 4, 0);                                    \
-        _Pragma("unroll") for (int k = 0; k < 5; k++) {                                       \
+        _Pragma("unroll") for (int k = 0; k < 3; k++) {                                       \
             __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);                                \
             __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);                                \
         }                                                                                     \
-        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);                                    \
-        __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);                                    \
-        _Pragma("unroll") for (int k = 0; k < 6; k++) {                                       \
+        _Pragma("unroll") for (int k = 0; k < 9; k++) {                                       \
             __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);                                \
-            __builtin_amdgcn_sched_group_barrier(0x002, 5, 0);                                \
+            __builtin_amdgcn_sched_group_barrier(0x002, 3, 0);                                \
         }                                                                                     \
-        Bc[0] = Bn[0];                                                                        \
-        Bc[1] = Bn[1];                                                                        \
-        _Pragma("unroll") for (int a = 0; a < kMfTiles; a++) Ac[a] = An[a];                   \
+        _Pragma("unroll") for (int ct = 0; ct < 4; ct++) Bc[ct] = Bn[ct];                    \
+        _Pragma("unroll") for (int a = 0; a < 4; a++) Ac[a] = An[a];                          \
     } while (0)
 
     for (uint32_t g = 0; g < nq; g += 4) {
@@ -1048,30 +1058,36 @@ __device__ __forceinline__ void mf_unit_dma(const uint4 *s_desc, uint4 *A0, uint
     __builtin_amdgcn_s_barrier();
     asm volatile("" ::: "memory");
 
+    // Epilogue, as in mf_unit: half h of the lanes ends with the symbols of
+    // recovery row 2 t + h of row tile t.
 #pragma unroll
-    for (int a = 0; a < kMfTiles; a++) {
-        uint32_t sym[2];
+    for (int a = 0; a < 4; a++) {
+        const uint32_t r = 2 * (tile0 + 4 * wr + a) + h;
 #pragma unroll
-        for (int cb = 0; cb < 2; cb++) {
-            uint32_t lo = 0, hi = 0;
+        for (int p = 0; p < 2; p++) {
+            uint32_t sym[2];
 #pragma unroll
-            for (int v = 0; v < 16; v++) {
-                const uint32_t bit = ((uint32_t)acc[a][cb][v]) & 1u;
-                const int pos = (v & 3) + ((v >> 2) & 1) * 8;
-                if (v < 8) lo |= bit << pos;
-                else hi |= bit << pos;
+            for (int s = 0; s < 2; s++) {
+                uint32_t lo = 0, hi = 0;
+#pragma unroll
+                for (int v = 0; v < 16; v++) {
+                    const uint32_t bit = ((uint32_t)acc[a][2 * p + s][v]) & 1u;
+                    const int pos = (v & 3) + ((v >> 2) & 1) * 8;
+                    if (v < 8) lo |= bit << pos;
+                    else hi |= bit << pos;
+                }
+                lo <<= 4 * h;
+                hi <<= 4 * h;
+                const uint32_t give = h ? lo : hi;
+                const uint32_t got = (uint32_t)__shfl_xor((int)give, 32, 64);
+                sym[s] = (h ? hi : lo) | got;
             }
-            lo <<= 4 * h;
-            hi <<= 4 * h;
-            const uint32_t give = h ? lo : hi;
-            const uint32_t got = (uint32_t)__shfl_xor((int)give, 32, 64);
-            sym[cb] = (h ? hi : lo) | got;
-        }
-        const uint32_t r = 2 * (tile0 + a) + h;
-        if (r < R && col_ok) {
-            uint32_t *o = reinterpret_cast<uint32_t *>(rec + (size_t)r * S + off);
-            const uint32_t v = sym[0] | (sym[1] << 16);
-            *o = accumulate ? (*o ^ v) : v;
+            const uint32_t off = colbase + wc * 256 + p * 128 + 4 * c;
+            if (r < R && off < S) {
+                uint32_t *o = reinterpret_cast<uint32_t *>(rec + (size_t)r * S + off);
+                const uint32_t v = sym[0] | (sym[1] << 16);
+                *o = accumulate ? (*o ^ v) : v;
+            }
         }
     }
 }
