@@ -121,6 +121,7 @@ static void add_slice_work(BatchGroupWork &w, uint32_t cnt, uint32_t S) {
         w.big_slices = 0;
     }
     if (cnt && blocks == w.big_blocks) w.big_slices += cnt;
+    if (cnt && (w.small_blocks == 0 || blocks < w.small_blocks)) w.small_blocks = blocks;
 }
 
 // The sizing work of one piece: its encode unit-steps and packet tasks.

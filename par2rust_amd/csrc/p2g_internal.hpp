@@ -242,6 +242,7 @@ struct BatchGroupWork {
     double enc_steps = 0;
     double slice_blocks = 0, slices = 0;
     double big_slices = 0, big_blocks = 0;  // the slices of the largest size and their blocks
+    double small_blocks = 0;                // blocks of a slice of the smallest size
     std::vector<double> piece_steps, piece_tasks, piece_task_blocks;
 };
 // What the planners estimate times from (µs): an encode unit's K step on
@@ -264,6 +265,9 @@ void plan_batch_group(uint32_t num_cus, uint32_t max_resident, uint32_t n_tasks,
                       uint32_t n_units, const BatchGroupWork &w, const CostRates &rates,
                       BatchGroupLaunch *g);
 constexpr uint32_t kBatchProgress = 1;  // sync word: units published so far
+// Sync word of a batched group: slice tasks (64 consecutive slices) taken so
+// far by the slice-hash waves (the fused pass leaves this word unused).
+constexpr uint32_t kBatchSliceQueue = kPassFlags;
 
 // Synthetic input: the xorshift64 stream from `seed` into d_dst.  d_jumps
 // holds 32 GF(2) matrices of 64 columns, step^(kXorshiftLaneWords 2^b); the

@@ -2226,15 +2226,25 @@ __global__ __launch_bounds__(kMfThreads, 1) void k_create_batch(BatchArgs a) {
     }
     if (role.role == kPassSlice && wave < 2) {
         // slice MD5 + CRC-32 lanes on waves 0 and 1 (CRC tables per wave in
-        // lds.desc); a lane walks the slice jobs on its own
-        const uint32_t hb = role.index;
+        // lds.desc).  A wave takes tasks of 64 consecutive slices (one per
+        // lane) from the group's queue; the table runs from the largest
+        // slices to the smallest, so the waves end within about one small
+        // slice of each other.
         static_assert(sizeof(BatchLds::desc) >= 2 * 16 * 256 * 4, "two waves of 16 CRC tables");
         uint32_t(*t)[256] = reinterpret_cast<uint32_t(*)[256]>(lds.desc) + 16 * wave;
         crc_tables_init_wave<16>(t);
-        const uint32_t n_lanes = a.h2_blocks * 128;
+        const uint32_t n_tasks = (a.slice_hi - a.slice_lo + 63) / 64;
         if (lane == 0) batch_role(a.sync, kPassSlice, false);
         uint32_t sj = 0;
-        for (uint32_t sl = a.slice_lo + (hb * 2 + wave) * 64 + lane; sl < a.slice_hi; sl += n_lanes) {
+        for (;;) {
+            uint32_t task = 0;
+            if (lane == 0)
+                task = __hip_atomic_fetch_add(a.sync + kBatchSliceQueue, 1u, __ATOMIC_RELAXED,
+                                              __HIP_MEMORY_SCOPE_AGENT);
+            task = __builtin_amdgcn_readfirstlane(task);
+            if (task >= n_tasks) break;
+            const uint32_t sl = a.slice_lo + task * 64 + lane;
+            if (sl >= a.slice_hi) continue;
             while (sl >= a.sjobs[sj].slice_end) sj++;
             const BatchSlices J = a.sjobs[sj];
             if (sl < J.slice0) continue;  // descriptors of an item without slice checksums
@@ -2243,8 +2253,7 @@ __global__ __launch_bounds__(kMfThreads, 1) void k_create_batch(BatchArgs a) {
             uint32_t *co = J.crc + (sl - J.slice0);
             if (J.S % 16 == 0) slice_md5_crc_lane<true>(a.descs[sl], tb, J.S, t, mo, co);
             else slice_md5_crc_lane<false>(a.descs[sl], tb, J.S, t, mo, co);
-            const uint32_t nxt = sl + n_lanes;
-            if (nxt >= a.slice_hi || nxt >= J.slice_end) batch_stamp(a.job_done + J.job);
+            batch_stamp(a.job_done + J.job);
         }
         if (lane == 0) batch_role(a.sync, kPassSlice, true);
     }

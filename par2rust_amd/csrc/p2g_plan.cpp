@@ -102,9 +102,10 @@ void plan_batch_group(uint32_t num_cus, uint32_t max_resident, uint32_t n_tasks,
     }
     // A hash workgroup holds its CU until its role is done, so each role
     // gets the fewest workgroups that keep it within the encode on the
-    // remaining CUs.  The slice lanes' estimate is the larger of their
-    // average work and the rounds of the largest slices (lanes take slices
-    // in stride), kept within 3/4 of the encode.  The packet tasks follow
+    // remaining CUs.  The slice waves take 64-slice tasks from a queue that
+    // runs from the largest slices to the smallest, so the slice lanes'
+    // estimate is their average work plus one slice of the smallest size,
+    // kept within 3/4 of the encode.  The packet tasks follow
     // the encode band by band: workgroups are added while the simulated end
     // of their role lies more than 2% past the encode and more workgroups
     // would bring it closer.
@@ -114,9 +115,8 @@ void plan_batch_group(uint32_t num_cus, uint32_t max_resident, uint32_t n_tasks,
         const uint32_t e = num_cus > h2 + h3 + 1 ? num_cus - h2 - h3 : 1;
         const double t_enc = w.enc_steps * rt.step_us / e;
         bool grew = false;
-        const double big_rounds = std::ceil(w.big_slices / (128.0 * std::max<uint32_t>(h2, 1)));
-        const double big_us = big_rounds > 1 ? big_rounds * (rt.slice_us + w.big_blocks * rt.slice_block_us) : 0;
-        if (h2 < h2_cap && std::max(slice_us / (128.0 * h2), big_us) > 0.75 * t_enc) {
+        const double tail_us = rt.slice_us + w.small_blocks * rt.slice_block_us;
+        if (h2 < h2_cap && slice_us / (128.0 * h2) + tail_us > 0.75 * t_enc) {
             h2++;
             grew = true;
         }

@@ -99,6 +99,7 @@ int main() {
                 w.big_slices = 0;
             }
             if (blocks == w.big_blocks) w.big_slices += n;
+            if (w.small_blocks == 0 || blocks < w.small_blocks) w.small_blocks = blocks;
         }
         const uint32_t cus = cus_list[lcg(rng) % 9];
         const uint32_t resident = t % 3 ? cus : 1 + lcg(rng) % (2 * cus);
