@@ -924,6 +924,13 @@ __device__ __forceinline__ void mf_word_addrs(const uint4 *s_desc, uint32_t g, u
     p1 = reinterpret_cast<const uint32_t *>(((uint64_t)d1.y << 32 | d1.x) + off);
 }
 
+// DENSE selects the step schedule.  The dense one keeps the matrix pipe
+// busiest, and the GPU then runs at a lower clock under its power limit
+// (2.12-2.17 against 2.27-2.32 GHz measured in the fused pass), which
+// lengthens every serial MD5 chain by the same factor.  The batched kernel,
+// bound by its encode, takes the dense schedule; the fused pass, bound by
+// its packet-MD5 chain once the encode is done, takes the light one.
+template <bool DENSE>
 __device__ __forceinline__ void mf_unit_dma(const uint4 *s_desc, uint4 *A0, uint4 *A1, uint4 *A2,
                                             uint4 *A3, const SliceDesc *__restrict__ inputs,
                                             const uint4 *__restrict__ afrag, uint32_t nq,
@@ -939,8 +946,10 @@ __device__ __forceinline__ void mf_unit_dma(const uint4 *s_desc, uint4 *A0, uint
     // DMA share of wave w: column quarter w (128 bytes) of the step's four
     // inputs and the A fragments of tiles w and w + 4
     const uint32_t off_ld = min(colbase + w * 128 + 4 * c, S - 4);
-    const uint4 *ga0 = afrag + ((size_t)(tile0 + w) * frag_stride) * 64 + lane;
-    const uint4 *ga1 = afrag + ((size_t)(tile0 + 4 + w) * frag_stride) * 64 + lane;
+    // wave-uniform fragment bases (scalar registers); the lane offset is
+    // added at each load, so the DMA takes a scalar base and a lane offset
+    const uint4 *ga0 = afrag + ((size_t)(tile0 + w) * frag_stride) * 64;
+    const uint4 *ga1 = afrag + ((size_t)(tile0 + 4 + w) * frag_stride) * 64;
     // Compute share of wave w = 2 wr + wc: row tiles 4 wr .. 4 wr + 3 over
     // column quarters 2 wc and 2 wc + 1.  Column tile 2 p + s is symbol s of
     // the lane's word in quarter 2 wc + p; the word of input 4g + 2h + j
@@ -962,8 +971,8 @@ __device__ __forceinline__ void mf_unit_dma(const uint4 *s_desc, uint4 *A0, uint
 #define MF_DMA(G, BUF)                                                                        \
     do {                                                                                      \
         const uint32_t gq = min((uint32_t)(G), nq - 1);                                       \
-        __builtin_amdgcn_global_load_lds(ga0 + (size_t)gq * 64, (BUF) + 64 * w, 16, 0, 0);    \
-        __builtin_amdgcn_global_load_lds(ga1 + (size_t)gq * 64, (BUF) + 256 + 64 * w, 16, 0, 0); \
+        __builtin_amdgcn_global_load_lds(ga0 + (size_t)gq * 64 + lane, (BUF) + 64 * w, 16, 0, 0); \
+        __builtin_amdgcn_global_load_lds(ga1 + (size_t)gq * 64 + lane, (BUF) + 256 + 64 * w, 16, 0, 0); \
         __builtin_amdgcn_global_load_lds(pw0, reinterpret_cast<uint32_t *>(BUF) + kMfTiles * 64 * 4 + 2 * w * 64, 4, 0, 0); \
         __builtin_amdgcn_global_load_lds(pw1, reinterpret_cast<uint32_t *>(BUF) + kMfTiles * 64 * 4 + (2 * w + 1) * 64, 4, 0, 0); \
         asm volatile("" ::: "memory");                                                        \
@@ -992,9 +1001,12 @@ __device__ __forceinline__ void mf_unit_dma(const uint4 *s_desc, uint4 *A0, uint
 #pragma unroll
     for (int a = 0; a < 4; a++) Ac[a] = A0[ard + a * 64];
 
-    // step GG: MFMAs on Ac = A(GG), Bc = B(GG); DMA(GG + 3) into BUFW;
-    // B(GG + 1) and A(GG + 1) from BUFN.
-#define MF_STEP_DMA(GG, BUFN, BUFW)                                                           \
+    // step GG (both schedules): MFMAs on Ac = A(GG), Bc = B(GG); DMA(GG + 3)
+    // into BUFW; B(GG + 1) and A(GG + 1) from BUFN.
+    //
+    // Light schedule: scheduling-group hints only, which leave the DMA issue
+    // and most of the B build in runs between few MFMAs.
+#define MF_STEP_LIGHT(GG, BUFN, BUFW)                                                           \
     do {                                                                                      \
         __builtin_amdgcn_sched_barrier(0);                                                    \
         /* the step's first MFMA ahead of its wait and barrier: the pipe */                  \
@@ -1042,13 +1054,130 @@ __device__ __forceinline__ void mf_unit_dma(const uint4 *s_desc, uint4 *A0, uint
         _Pragma("unroll") for (int a = 0; a < 4; a++) Ac[a] = An[a];                          \
     } while (0)
 
-    for (uint32_t g = 0; g < nq; g += 4) {
-        MF_STEP_DMA(g, A1, A3);
-        MF_STEP_DMA(g + 1, A2, A0);
-        MF_STEP_DMA(g + 2, A3, A1);
-        MF_STEP_DMA(g + 3, A0, A2);
+    // Dense schedule: the step is written out in the order it issues, each
+    // group of other instructions fenced between two
+    // MFMAs (an MFMA issue holds the wave until the pipe takes it, so only
+    // work placed between MFMAs overlaps them): the first MFMA ahead of the
+    // wait and barrier, then one DMA per MFMA, the LDS reads (words first,
+    // so the B build waits only for them), and the B build three VALU per
+    // MFMA.
+#define MF_MM(A_, CT_)                                                                        \
+    acc[A_][CT_] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(                           \
+        v8i_t{(int)Ac[A_].x, (int)Ac[A_].y, (int)Ac[A_].z, (int)Ac[A_].w, 0, 0, 0, 0}, Bc[CT_], \
+        acc[A_][CT_], 4, 4, 0, kE8M0One, 0, kE8M0One)
+#define MF_SB __builtin_amdgcn_sched_barrier(0)
+    // half of the B operand of one symbol: x from a byte permute, then two masks
+#define MF_BH0(X, W0, W1, SEL, OUT)                                                           \
+    do {                                                                                      \
+        X = __builtin_amdgcn_perm(W1, W0, SEL);                                               \
+        OUT[0] = (int)(X & 0x11111111u);                                                      \
+        OUT[1] = (int)(X & 0x22222222u);                                                      \
+    } while (0)
+#define MF_BH1(X, OUT)                                                                        \
+    do {                                                                                      \
+        OUT[2] = (int)(X & 0x44444444u);                                                      \
+        OUT[3] = (int)((X >> 1) & 0x44444444u);                                               \
+        OUT[4] = OUT[5] = OUT[6] = OUT[7] = 0;                                                \
+    } while (0)
+#define MF_STEP_DENSE(GG, BUFN, BUFW)                                                         \
+    do {                                                                                      \
+        const uint32_t gq = min((uint32_t)(GG) + 3, nq - 1);                                  \
+        const uint32_t *wb = reinterpret_cast<const uint32_t *>(BUFN) + wrd;                  \
+        uint32_t x00, x01, x10, x11, xs;                                                      \
+        uint2 d0, d1;                                                                         \
+        v8i_t Bn[4];                                                                          \
+        uint4 An[4];                                                                          \
+        MF_SB;                                                                                \
+        MF_MM(0, 0);                                                                          \
+        MF_SB;                                                                                \
+        __builtin_amdgcn_s_waitcnt(kWaitVmcnt4);                                              \
+        asm volatile("" ::: "memory");                                                        \
+        __builtin_amdgcn_s_barrier();                                                         \
+        asm volatile("" ::: "memory");                                                        \
+        MF_SB;                                                                                \
+        MF_MM(0, 1);                                                                          \
+        __builtin_amdgcn_global_load_lds(ga0 + (size_t)gq * 64 + lane, (BUFW) + 64 * w, 16, 0, 0); \
+        x00 = wb[0];                                                                          \
+        x01 = wb[64];                                                                         \
+        MF_SB;                                                                                \
+        MF_MM(0, 2);                                                                          \
+        __builtin_amdgcn_global_load_lds(ga1 + (size_t)gq * 64 + lane, (BUFW) + 256 + 64 * w, 16, 0, 0); \
+        x10 = wb[128];                                                                        \
+        x11 = wb[192];                                                                        \
+        {                                                                                     \
+            const uint32_t gd = min((uint32_t)(GG) + 4, nq - 1);                              \
+            d0 = reinterpret_cast<const uint2 *>(s_desc + 4 * gd + 2 * h)[0];                 \
+            d1 = reinterpret_cast<const uint2 *>(s_desc + 4 * gd + 2 * h + 1)[0];             \
+        }                                                                                     \
+        MF_SB;                                                                                \
+        MF_MM(0, 3);                                                                          \
+        __builtin_amdgcn_global_load_lds(pw0, reinterpret_cast<uint32_t *>(BUFW) + kMfTiles * 64 * 4 + 2 * w * 64, 4, 0, 0); \
+        MF_SB;                                                                                \
+        MF_MM(1, 0);                                                                          \
+        __builtin_amdgcn_global_load_lds(pw1, reinterpret_cast<uint32_t *>(BUFW) + kMfTiles * 64 * 4 + (2 * w + 1) * 64, 4, 0, 0); \
+        asm volatile("" ::: "memory");                                                        \
+        MF_SB;                                                                                \
+        MF_MM(1, 1);                                                                          \
+        MF_BH0(xs, x00, x01, 0x05040100u, Bn[0]);                                             \
+        MF_SB;                                                                                \
+        MF_MM(1, 2);                                                                          \
+        MF_BH1(xs, Bn[0]);                                                                    \
+        An[0] = (BUFN)[ard];                                                                  \
+        MF_SB;                                                                                \
+        MF_MM(1, 3);                                                                          \
+        MF_BH0(xs, x00, x01, 0x07060302u, Bn[1]);                                             \
+        An[1] = (BUFN)[ard + 64];                                                             \
+        MF_SB;                                                                                \
+        MF_MM(2, 0);                                                                          \
+        MF_BH1(xs, Bn[1]);                                                                    \
+        An[2] = (BUFN)[ard + 128];                                                            \
+        MF_SB;                                                                                \
+        MF_MM(2, 1);                                                                          \
+        MF_BH0(xs, x10, x11, 0x05040100u, Bn[2]);                                             \
+        An[3] = (BUFN)[ard + 192];                                                            \
+        MF_SB;                                                                                \
+        MF_MM(2, 2);                                                                          \
+        MF_BH1(xs, Bn[2]);                                                                    \
+        MF_SB;                                                                                \
+        MF_MM(2, 3);                                                                          \
+        MF_BH0(xs, x10, x11, 0x07060302u, Bn[3]);                                             \
+        MF_SB;                                                                                \
+        MF_MM(3, 0);                                                                          \
+        MF_BH1(xs, Bn[3]);                                                                    \
+        MF_SB;                                                                                \
+        MF_MM(3, 1);                                                                          \
+        pw0 = reinterpret_cast<const uint32_t *>(((uint64_t)d0.y << 32 | d0.x) + off_ld);     \
+        pw1 = reinterpret_cast<const uint32_t *>(((uint64_t)d1.y << 32 | d1.x) + off_ld);     \
+        MF_SB;                                                                                \
+        MF_MM(3, 2);                                                                          \
+        MF_SB;                                                                                \
+        MF_MM(3, 3);                                                                          \
+        MF_SB;                                                                                \
+        _Pragma("unroll") for (int ct = 0; ct < 4; ct++) Bc[ct] = Bn[ct];                    \
+        _Pragma("unroll") for (int a = 0; a < 4; a++) Ac[a] = An[a];                          \
+    } while (0)
+
+    if constexpr (DENSE) {
+        for (uint32_t g = 0; g < nq; g += 4) {
+            MF_STEP_DENSE(g, A1, A3);
+            MF_STEP_DENSE(g + 1, A2, A0);
+            MF_STEP_DENSE(g + 2, A3, A1);
+            MF_STEP_DENSE(g + 3, A0, A2);
+        }
+    } else {
+        for (uint32_t g = 0; g < nq; g += 4) {
+            MF_STEP_LIGHT(g, A1, A3);
+            MF_STEP_LIGHT(g + 1, A2, A0);
+            MF_STEP_LIGHT(g + 2, A3, A1);
+            MF_STEP_LIGHT(g + 3, A0, A2);
+        }
     }
-#undef MF_STEP_DMA
+#undef MF_STEP_DENSE
+#undef MF_STEP_LIGHT
+#undef MF_BH0
+#undef MF_BH1
+#undef MF_SB
+#undef MF_MM
 #undef MF_B
 #undef MF_DMA
     // the clamped DMAs of steps past nq, and this wave's last LDS reads, are
@@ -1937,7 +2066,7 @@ __global__ __launch_bounds__(kMfThreads, 1) void k_create_pass(PassArgs a) {
         const uint32_t c0 = bstart[band], w = bstart[band + 1] - c0;
         const uint32_t v = u - b_lo, y = v / w, x = c0 + v % w;
         if (FULL)
-            mf_unit_dma(lds.desc, s_a0, s_a1, s_a2, s_a3, a.inputs, a.afrag, a.nq, a.nq, a.R, a.S,
+            mf_unit_dma<false>(lds.desc, s_a0, s_a1, s_a2, s_a3, a.inputs, a.afrag, a.nq, a.nq, a.R, a.S,
                         a.rec, 0, y * kMfTiles, x * (kMfThreads * 2));
         else
             mf_unit<FULL>(lds, a.inputs, a.afrag, a.nq, a.nq, a.R, a.S, a.rec, 0, y * kMfTiles,
@@ -2309,7 +2438,7 @@ __global__ __launch_bounds__(kMfThreads, 1) void k_create_batch(BatchArgs a) {
                 __syncthreads();
                 loaded = cin;
             }
-            mf_unit_dma(lds.desc, s_a0, s_a1, s_a2, s_a3, cin, afrag + (size_t)(k0 / 4) * 64, cq, P.nq, P.R, P.S, P.rec,
+            mf_unit_dma<true>(lds.desc, s_a0, s_a1, s_a2, s_a3, cin, afrag + (size_t)(k0 / 4) * 64, cq, P.nq, P.R, P.S, P.rec,
                           k0 > 0, y * kMfTiles, x * (kMfThreads * 2));
         }
         done++;
