@@ -227,6 +227,7 @@ def run_batch_mix(ctx, n_jobs: int, reps: int, tune: bool = True) -> dict:
     log(f"bench: graph replay {replay_wall:.3f}s per batch; batch == plain: {same}")
 
     pct = lambda v, q: round(float(np.percentile(np.array(v) * 1e3, q)), 2)
+    fp4_ops = sum(256 * j["n"] * j["R"] * j["S"] for j in jobs)
     return {
         "jobs": n_jobs, "source_bytes": src_bytes, "autotune": tuned,
         "slice_sizes": {str(S): sum(1 for j in jobs if j["S"] == S) for S in synth.BATCH_SLICE_SIZES},
@@ -240,6 +241,12 @@ def run_batch_mix(ctx, n_jobs: int, reps: int, tune: bool = True) -> dict:
         "batched_source_GBps": round(src_bytes / batch_wall / 1e9, 2),
         "plain_source_GBps": round(src_bytes / plain_wall / 1e9, 2),
         "replay_last_job_ms": round(max(groups_ms), 2),
+        # fp4 work of the GF(2) matrix products over the last replay's device
+        # time, from the first batched launch's start to the last job's
+        # completion stamp (the first fragment launch, ~3 ms, comes before it)
+        "fp4_ops": fp4_ops,
+        "fp4_TFLOPs_device": round(fp4_ops / (max(groups_ms) * 1e-3) / 1e12, 1),
+        "fp4_frac_device": round(fp4_ops / (max(groups_ms) * 1e-3) / 1e12 / FP4_DENSE_PEAK_TFLOPS, 4),
         "bit_exact_vs_plain": bool(same),
         "latency": "completion time from a common submission at t = 0",
     }
@@ -313,6 +320,7 @@ def run_serving(ctx, seconds: float, rate: float, budget: int, verify: int = 4,
     big_rid = next(r for r, v in info.items() if v[2] is big)
     small = [v for r, v in lat.items() if r != big_rid]
     pct = lambda v, q: round(float(np.percentile(np.array(v) * 1e3, q)), 2)
+    fp4_ops = sum(256 * j["n"] * j["R"] * j["S"] for j in jobs)
     out = {
         "requests": len(reqs), "arrival_rate_jobs_per_s": round(rate, 1), "arrival_seconds": seconds,
         "hbm_budget_bytes": budget, "step_target_ms": step_ms or 50.0, "source_bytes": src_bytes,
