@@ -80,13 +80,26 @@ static void packet_prefix_words(const uint8_t set_id[16], uint32_t out[8]) {
 
 static uint64_t align_up(uint64_t x, uint64_t a) { return (x + a - 1) / a * a; }
 
+// Buffers of a plan grow with room to spare (half again, at most 1 GiB more):
+// a plan rebuilt for steps of varying size (the serving loop) then rarely
+// reallocates, and a reallocation waits for the whole device.
+static int grow(DevBuf &d, size_t n) {
+    return n <= d.cap ? P2G_OK : d.reserve(n + std::min<size_t>(n / 2, 1ull << 30));
+}
+static int grow(PinBuf &h, size_t n) {
+    return n <= h.cap ? P2G_OK : h.reserve(n + std::min<size_t>(n / 2, 1ull << 30));
+}
+
+// A table for the next launch to upload: the device buffer is sized now,
+// the bytes go to the pinned staging area at `off` (reserved by the caller).
 template <typename T>
-static int upload(Ctx *c, DevBuf &d, const std::vector<T> &v, const char *what) {
-    if (d.reserve(std::max<size_t>(v.size() * sizeof(T), 16)) != P2G_OK)
+static int stage(Ctx *c, BatchPlan *B, DevBuf &d, const std::vector<T> &v, size_t off,
+                 const char *what) {
+    if (grow(d, std::max<size_t>(v.size() * sizeof(T), 16)) != P2G_OK)
         return c->fail(P2G_ERR_OUT_OF_MEMORY, what);
     if (v.empty()) return P2G_OK;
-    hipError_t e = hipMemcpy(d.ptr, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
-    if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, what, e);
+    memcpy(static_cast<uint8_t *>(B->h_stage.ptr) + off, v.data(), v.size() * sizeof(T));
+    B->pending.push_back({&d, off, v.size() * sizeof(T)});
     return P2G_OK;
 }
 
@@ -172,9 +185,9 @@ int build_plan(Ctx *c, const std::vector<PlanItem> &items, bool use_graph, Batch
         n_slices += jd[j].size();
     }
     if (n_slices > 0x7FFFFFFFull) return c->fail(P2G_ERR_INVALID_ARGUMENT, "too many slices");
-    if (B->d_staging.reserve(std::max<uint64_t>(stage_bytes, 256)) != P2G_OK ||
-        B->d_descs.reserve(n_slices * sizeof(SliceDesc)) != P2G_OK ||
-        B->d_tb.reserve(n_slices * 64) != P2G_OK)
+    if (grow(B->d_staging, std::max<uint64_t>(stage_bytes, 256)) != P2G_OK ||
+        grow(B->d_descs, n_slices * sizeof(SliceDesc)) != P2G_OK ||
+        grow(B->d_tb, n_slices * 64) != P2G_OK)
         return c->fail(P2G_ERR_OUT_OF_MEMORY, "batch descriptor tables");
     uint8_t *staging = static_cast<uint8_t *>(B->d_staging.ptr);
     const SliceDesc *d_descs = static_cast<const SliceDesc *>(B->d_descs.ptr);
@@ -281,7 +294,7 @@ int build_plan(Ctx *c, const std::vector<PlanItem> &items, bool use_graph, Batch
         spans[k].s_hi = (uint32_t)(std::upper_bound(job_group.begin(), job_group.end(), k) - job_group.begin());
     }
 
-    if (B->afrag->reserve(std::max<uint64_t>(afrag_max, 256)) != P2G_OK)
+    if (grow(*B->afrag, std::max<uint64_t>(afrag_max, 256)) != P2G_OK)
         return c->fail(P2G_ERR_OUT_OF_MEMORY, "batch A fragments");
     uint8_t *afrag_base = static_cast<uint8_t *>(B->afrag->ptr);
 
@@ -335,14 +348,23 @@ int build_plan(Ctx *c, const std::vector<PlanItem> &items, bool use_graph, Batch
         B->sync_off.push_back(sync_words_total);
         sync_words_total += (uint32_t)align_up(starts, 64);
     }
-    int rc = upload(c, B->d_descs, all, "batch descriptors");
-    if (rc == P2G_OK) rc = upload(c, B->d_parts, parts, "batch staging table");
-    if (rc == P2G_OK) rc = upload(c, B->d_pieces, pieces, "batch pieces");
-    if (rc == P2G_OK) rc = upload(c, B->d_sjobs, sjobs, "batch slice jobs");
+    // the tables, staged in pinned memory for the next launch to upload
+    const size_t o_descs = 0, o_parts = align_up(o_descs + all.size() * sizeof(SliceDesc), 256);
+    const size_t o_pieces = align_up(o_parts + parts.size() * sizeof(StagePart), 256);
+    const size_t o_sjobs = align_up(o_pieces + pieces.size() * sizeof(BatchPiece), 256);
+    const size_t stage_total = o_sjobs + sjobs.size() * sizeof(BatchSlices);
+    B->pending.clear();
+    if (grow(B->h_stage, std::max<size_t>(stage_total, 256)) != P2G_OK)
+        return c->fail(P2G_ERR_OUT_OF_MEMORY, "batch table staging");
+    int rc = stage(c, B, B->d_descs, all, o_descs, "batch descriptors");
+    if (rc == P2G_OK) rc = stage(c, B, B->d_parts, parts, o_parts, "batch staging table");
+    if (rc == P2G_OK) rc = stage(c, B, B->d_pieces, pieces, o_pieces, "batch pieces");
+    if (rc == P2G_OK) rc = stage(c, B, B->d_sjobs, sjobs, o_sjobs, "batch slice jobs");
     if (rc != P2G_OK) return rc;
     const bool new_err = B->d_err.ptr == nullptr;
-    if (B->d_sync.reserve((size_t)sync_words_total * 4) != P2G_OK ||
-        B->d_done.reserve((size_t)(nitems + 1) * 8) != P2G_OK || B->d_err.reserve(256) != P2G_OK)
+    if (grow(B->d_sync, (size_t)sync_words_total * 4) != P2G_OK ||
+        grow(B->d_done, (size_t)(nitems + 1) * 8) != P2G_OK || B->d_err.reserve(256) != P2G_OK ||
+        grow(B->h_back, 8 + (size_t)(nitems + 1) * 8 + spans.size() * 8 * sizeof(unsigned long long)) != P2G_OK)
         return c->fail(P2G_ERR_OUT_OF_MEMORY, "batch sync words");
     if (new_err) {
         hipError_t ez = hipMemset(B->d_err.ptr, 0, 256);
@@ -397,6 +419,13 @@ int launch_plan(BatchPlan *B, hipStream_t st) {
     // launches of one plan share its tables: each waits for the one before
     hipError_t e = hipStreamWaitEvent(st, B->done, 0);
     if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "stream wait", e);
+    // the tables staged by build_plan, once
+    for (const BatchPlan::Upload &u : B->pending) {
+        e = hipMemcpyAsync(u.d->ptr, static_cast<uint8_t *>(B->h_stage.ptr) + u.off, u.bytes,
+                           hipMemcpyHostToDevice, st);
+        if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "upload batch tables", e);
+    }
+    B->pending.clear();
     if (B->exec) {
         e = hipGraphLaunch(B->exec, st);
         if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "batch graph launch", e);
@@ -404,6 +433,16 @@ int launch_plan(BatchPlan *B, hipStream_t st) {
         int rc = B->record(st);
         if (rc != P2G_OK) return rc;
     }
+    // error word, completion stamps and each group's role stamps to the host
+    uint8_t *back = static_cast<uint8_t *>(B->h_back.ptr);
+    e = hipMemcpyAsync(back, B->d_err.ptr, 4, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(back + 8, B->d_done.ptr, (size_t)(B->njobs + 1) * 8, hipMemcpyDeviceToHost, st);
+    for (size_t k = 0; e == hipSuccess && k < B->groups.size(); k++)
+        e = hipMemcpyAsync(back + 8 + (size_t)(B->njobs + 1) * 8 + k * 8 * sizeof(unsigned long long),
+                           static_cast<uint32_t *>(B->d_sync.ptr) + B->sync_off[k] + kPassTimes,
+                           8 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st);
+    if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "copy batch results back", e);
     e = hipEventRecord(B->done, st);
     if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "event record", e);
     B->launched = true;
@@ -417,17 +456,18 @@ int wait_plan(BatchPlan *B) {
     hipError_t e = hipEventSynchronize(B->done);
     if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "batch wait", e);
     B->launched = false;
-    // every launch since the last wait ORs its errors into the sticky word
+    // every launch since the last wait ORs its errors into the sticky word;
+    // the last launch copied it, its stamps and its role stamps to h_back
+    const uint8_t *back = static_cast<const uint8_t *>(B->h_back.ptr);
+    const unsigned long long *back_done = reinterpret_cast<const unsigned long long *>(back + 8);
+    const unsigned long long *back_roles = back_done + B->njobs + 1;
     uint32_t bad = 0;
-    e = hipMemcpy(&bad, B->d_err.ptr, 4, hipMemcpyDeviceToHost);
-    if (e == hipSuccess && bad) e = hipMemset(B->d_err.ptr, 0, 4);
+    memcpy(&bad, back, 4);
+    if (bad) e = hipMemset(B->d_err.ptr, 0, 4);
     if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "read batch flags", e);
     // the run's own role times refine the rates the next plans are sized by
     for (size_t k = 0; k < B->groups.size() && k < B->work.size(); k++) {
-        unsigned long long t[2 * kPassRoles];
-        e = hipMemcpy(t, static_cast<uint32_t *>(B->d_sync.ptr) + B->sync_off[k] + kPassTimes, sizeof(t),
-                      hipMemcpyDeviceToHost);
-        if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "read batch stamps", e);
+        const unsigned long long *t = back_roles + k * 8;
         auto span_us = [&](int role) {  // 100 MHz ticks
             const unsigned long long a = t[2 * role], b = t[2 * role + 1];
             return b > a && b != 0 ? (double)(b - a) / 100.0 : 0.0;
@@ -440,13 +480,9 @@ int wait_plan(BatchPlan *B) {
     }
     if (getenv("P2G_BATCH_VERBOSE")) {
         // per group: first start and last end of each role, ms from the batch start
-        std::vector<unsigned long long> t0(1);
-        hipMemcpy(t0.data(), static_cast<unsigned long long *>(B->d_done.ptr) + B->njobs, 8,
-                  hipMemcpyDeviceToHost);
+        const unsigned long long t0[1] = {back_done[B->njobs]};
         for (size_t k = 0; k < B->groups.size(); k++) {
-            unsigned long long t[2 * kPassRoles];
-            hipMemcpy(t, static_cast<uint32_t *>(B->d_sync.ptr) + B->sync_off[k] + kPassTimes,
-                      sizeof(t), hipMemcpyDeviceToHost);
+            const unsigned long long *t = back_roles + k * 8;
             const BatchGroupLaunch &g = B->groups[k];
             fprintf(stderr, "batch group %zu: pieces %u units %u tasks %u slices %u h3 %u h2 %u |", k,
                     g.n_pieces, g.n_units, g.n_tasks, g.slice_hi - g.slice_lo, g.h3_blocks, g.h2_blocks);
@@ -460,9 +496,7 @@ int wait_plan(BatchPlan *B) {
             fprintf(stderr, "\n");
         }
     }
-    B->h_done.resize(B->njobs + 1);
-    e = hipMemcpy(B->h_done.data(), B->d_done.ptr, (B->njobs + 1) * 8, hipMemcpyDeviceToHost);
-    if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "read batch stamps", e);
+    B->h_done.assign(back_done, back_done + B->njobs + 1);
     if (bad & kPassErrWait)
         return c->fail(P2G_ERR_DEVICE,
                        "batched create: a packet hash stopped waiting for the encode (no progress for 2 s)");

@@ -26,6 +26,18 @@ struct BatchPlan {
     std::vector<BatchGroupLaunch> groups;
     std::vector<BatchGroupWork> work;  // each group's work, for the rates its run measures
     std::vector<uint32_t> sync_off;  // word offset of each group's sync words
+    // Host side of the plan's transfers, all asynchronous on the launch's
+    // stream (pinned memory): the tables are staged here by build_plan and
+    // uploaded by the next launch; each launch copies its error word,
+    // completion stamps and role stamps back before its `done` event.  No
+    // host call then waits on the device while another plan's persistent
+    // launch, which holds every CU, is running.
+    struct Upload {
+        DevBuf *d;
+        size_t off, bytes;
+    };
+    PinBuf h_stage, h_back;
+    std::vector<Upload> pending;
     hipStream_t cap = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;

@@ -271,8 +271,11 @@ constexpr uint32_t kBatchSliceQueue = kPassFlags;
 
 // Synthetic input: the xorshift64 stream from `seed` into d_dst.  d_jumps
 // holds 32 GF(2) matrices of 64 columns, step^(kXorshiftLaneWords 2^b); the
-// destination must be 16-byte aligned.
-constexpr uint32_t kXorshiftLaneWords = 2048;
+// destination must be 16-byte aligned.  A lane writes kXorshiftLaneWords
+// words (1 KiB) after jumping to its start, so a file of a few MB is already
+// thousands of lanes (with 16 KiB per lane a 13 MB file ran on four
+// workgroups at ~100 GB/s).
+constexpr uint32_t kXorshiftLaneWords = 128;
 hipError_t launch_xorshift_fill(uint8_t *d_dst, uint64_t nbytes, uint64_t seed,
                                 const uint64_t *d_jumps, hipStream_t stream);
 

@@ -22,6 +22,8 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <map>
@@ -110,6 +112,7 @@ struct Step {
     BatchPlan plan;
     std::vector<std::pair<Request *, PlanItem>> items;
     bool busy = false;
+    double est_ms = 0, t_launch = 0, host_ms = 0;  // P2G_SERVE_VERBOSE diagnostics
 };
 
 struct Server {
@@ -128,6 +131,7 @@ struct Server {
     std::deque<int> inflight;  // step indices in launch order
     DevBuf afrag;              // shared by both steps' plans (one stream order)
     hipStream_t stream = nullptr;
+    bool verbose = getenv("P2G_SERVE_VERBOSE") != nullptr;  // per-step timing to stderr
     std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
 
     double now_ms() const {
@@ -231,8 +235,16 @@ int Server::launch_step(int k) {
         items.push_back(it);
     }
     if (items.empty()) return P2G_OK;
+    const double t_plan = now_ms();
     int rc = build_plan(c, items, false, &st.plan);
     if (rc == P2G_OK) rc = launch_plan(&st.plan, stream);
+    st.t_launch = now_ms();
+    st.host_ms = st.t_launch - t_plan;
+    st.est_ms = 0;
+    for (auto &pr : st.items) {
+        const Request *q = pr.first;
+        st.est_ms += ms_per_row(c, q->n, q->S) * (pr.second.r1 - pr.second.r0);
+    }
     if (rc != P2G_OK) {
         // whatever part of the step was queued ends before its requests are
         // reported failed (their memory may then go back to the pool)
@@ -260,6 +272,16 @@ int Server::retire(int k) {
     const int rc = wait_plan(&st.plan);
     st.busy = false;
     const double t = now_ms();
+    if (verbose && rc == P2G_OK && st.plan.h_done.size() == st.plan.njobs + 1) {
+        // device start (first batched workgroup) and end (last completion
+        // stamp) on the 100 MHz counter, host planning time, estimate
+        const unsigned long long t0 = st.plan.h_done[st.plan.njobs];
+        unsigned long long t1 = t0;
+        for (uint32_t j = 0; j < st.plan.njobs; j++) t1 = std::max(t1, st.plan.h_done[j]);
+        fprintf(stderr, "serve step: items %zu groups %zu est %.1f ms | device %.3f-%.3f ms (%.1f) | host plan %.1f ms at %.1f, retired at %.1f\n",
+                st.items.size(), st.plan.groups.size(), st.est_ms, t0 * 1e-5, t1 * 1e-5,
+                (t1 - t0) * 1e-5, st.host_ms, st.t_launch, t);
+    }
     for (auto &pr : st.items) {
         Request *q = pr.first;
         if (q->state != Request::kActive) continue;  // failed in an earlier step

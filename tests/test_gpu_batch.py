@@ -191,7 +191,8 @@ def test_batch_errors(ctx):
     assert e.value.kind == "InvalidSliceSize"
 
 
-@pytest.mark.parametrize("n,seed", [(1, 1), (7, 2), (8, 3), (16, 4), (2048 * 8 + 5, 5),
+@pytest.mark.parametrize("n,seed", [(1, 1), (7, 2), (8, 3), (16, 4), (128 * 8 + 5, 8),
+                                    (3 * 128 * 8 - 1, 9), (2048 * 8 + 5, 5),
                                     (3 * (1 << 20) + 3, 6), (40 << 20, 7)])
 def test_fill_xorshift64(ctx, n, seed):
     from par2rust_amd import synth
