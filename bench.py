@@ -320,7 +320,6 @@ def run_serving(ctx, seconds: float, rate: float, budget: int, verify: int = 4,
     big_rid = next(r for r, v in info.items() if v[2] is big)
     small = [v for r, v in lat.items() if r != big_rid]
     pct = lambda v, q: round(float(np.percentile(np.array(v) * 1e3, q)), 2)
-    fp4_ops = sum(256 * j["n"] * j["R"] * j["S"] for j in jobs)
     out = {
         "requests": len(reqs), "arrival_rate_jobs_per_s": round(rate, 1), "arrival_seconds": seconds,
         "hbm_budget_bytes": budget, "step_target_ms": step_ms or 50.0, "source_bytes": src_bytes,
