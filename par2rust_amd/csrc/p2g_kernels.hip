@@ -1387,15 +1387,99 @@ __device__ __forceinline__ void md5_blocks_at(uint32_t h[4], Addr addr, uint32_t
     }
 }
 
-// nblk consecutive blocks at p.
+// nblk consecutive blocks at p: md5_blocks_at's ring, but the loads of a
+// ring turn take their addresses as constant offsets (at most 572 bytes)
+// from one pointer advanced once per turn.  A single MD5 wave issues one
+// instruction per 4 cycles whatever its kind, so per-block address
+// arithmetic (64-bit shifts and adds, ~5 instructions) costs as much as a
+// VALU round step.
+// step(y) consumes each block's 16 loaded words in order.
+template <typename W, typename Step>
+__device__ __forceinline__ void md5_stream_steps(const W *p, uint32_t nblk, Step step) {
+    constexpr int WB = 64 / sizeof(W);  // W per block
+    if (nblk < (uint32_t)(kMd5Ring + kMd5Ahead)) {
+        for (uint32_t b = 0; b < nblk; b++) {
+            uint32_t x[16];
+            md5_load_block(x, p + (size_t)WB * b);
+            step(x);
+        }
+        return;
+    }
+    uint32_t buf[kMd5Ring][16];
+#pragma unroll
+    for (int k = 0; k < kMd5Ahead; k++) {
+        md5_load_block(buf[k], p + WB * k);
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    const W *q = p + WB * kMd5Ahead;  // block b + kMd5Ahead of the turn at b
+    uint32_t b = 0;
+    for (; b + kMd5Ring + kMd5Ahead <= nblk; b += kMd5Ring, q += WB * kMd5Ring) {
+#pragma unroll
+        for (int k = 0; k < kMd5Ring; k++) {
+            md5_load_block(buf[(k + kMd5Ahead) % kMd5Ring], q + WB * k);
+            __builtin_amdgcn_sched_barrier(0);  // as in md5_blocks_at
+            step(buf[k]);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+    for (; b < nblk; b += kMd5Ring) {
+#pragma unroll
+        for (int k = 0; k < kMd5Ring; k++) {
+            if (b + k < nblk) {
+                const uint32_t nxt = b + k + kMd5Ahead;
+                if (nxt < nblk) md5_load_block(buf[(k + kMd5Ahead) % kMd5Ring], p + (size_t)WB * nxt);
+                step(buf[k]);
+            }
+        }
+    }
+}
+
+template <typename W, typename Visit = Md5NoVisit>
+__device__ __forceinline__ void md5_stream(uint32_t h[4], const W *p, uint32_t nblk,
+                                           Visit visit = Visit()) {
+    md5_stream_steps(p, nblk, [&](const uint32_t *x) {
+        visit(x);
+        md5_compress(h, x);
+    });
+}
+
 template <typename W>
 __device__ __forceinline__ void md5_bulk(uint32_t h[4], const uint8_t *p, uint32_t nblk) {
-    md5_blocks_at<W>(h, [p](uint32_t b) { return reinterpret_cast<const W *>(p + 64 * (size_t)b); },
-                     nblk);
+    md5_stream(h, reinterpret_cast<const W *>(p), nblk);
 }
 
 // 64 zero bytes: the message block of a slice wholly past its data.
 __device__ const uint4 kZeroBlock[4] = {};
+
+// The nmsg message blocks of a slice as one lane reads them: blocks below
+// nfull = valid / 64 from the slice, block nfull from its tail block tb,
+// zero blocks after that.  The first nmain blocks, nmain = the smallest
+// min(nfull, nmsg) over the wave's active lanes (all of them when every
+// lane holds a whole slice of one size, the common case), stream from
+// contiguous addresses; only the rest select per block.  Lanes may differ
+// in nmsg (a batched wave can hold slices of two jobs).
+template <typename W, typename Visit = Md5NoVisit>
+__device__ __forceinline__ void md5_slice_blocks(uint32_t h[4], const uint8_t *ptr, uint32_t nfull,
+                                                 const uint4 *tb, uint32_t nmsg,
+                                                 Visit visit = Visit()) {
+    const uint32_t mine = min(nfull, nmsg);
+    // wave minimum: each turn takes the value of a lane below the current
+    // one, so it ends after at most one turn per distinct smaller value
+    uint32_t nmain = __builtin_amdgcn_readfirstlane(mine);
+    for (;;) {
+        const uint64_t m = __ballot(mine < nmain);
+        if (m == 0) break;
+        nmain = __builtin_amdgcn_readlane((int)mine, __builtin_ctzll(m));
+    }
+    md5_stream(h, reinterpret_cast<const W *>(ptr), nmain, visit);
+    if (nmain == nmsg) return;
+    md5_blocks_at<W>(h, [=](uint32_t b) {
+        b += nmain;
+        const void *a = b < nfull ? (const void *)(ptr + 64 * (size_t)b)
+                      : b == nfull ? (const void *)tb : (const void *)kZeroBlock;
+        return reinterpret_cast<const W *>(a);
+    }, nmsg - nmain, visit);
+}
 
 // One MD5 wave per CU: each lane streams its own message, so the waves'
 // scattered 64-byte reads would otherwise share (and thrash) a CU's L1 and
@@ -1410,10 +1494,10 @@ constexpr uint32_t kMd5LdsReserve = 96 * 1024;
 
 // MD5 of one zero-padded S-byte slice into out (one lane).  A16: the slice
 // start is 16-byte aligned.  tb: the slice's tail block (k_desc_tails).  The
-// S / 64 whole message blocks run as one loop of the same length on every
-// lane, whatever `valid`: block b comes from the slice while it is whole
-// data, from tb for the block holding byte `valid`, and from kZeroBlock after
-// that; only the last (padding) blocks take the word-by-word path.
+// S / 64 whole message blocks run through md5_slice_blocks whatever `valid`:
+// block b comes from the slice while it is whole data, from tb for the block
+// holding byte `valid`, and from kZeroBlock after that; only the last
+// (padding) blocks take the word-by-word path.
 template <bool A16>
 __device__ __forceinline__ void slice_md5_lane(const SliceDesc d, const uint4 *tb, uint32_t S,
                                                uint8_t *out) {
@@ -1421,14 +1505,9 @@ __device__ __forceinline__ void slice_md5_lane(const SliceDesc d, const uint4 *t
     const uint32_t nb = md5_blocks(S);
     const uint32_t nmsg = S / 64;         // whole message blocks
     const uint32_t nfull = d.valid / 64;  // of them, whole blocks of real data
-    const uint8_t *ptr = d.ptr;
     uint32_t h[4];
     md5_init(h);
-    md5_blocks_at<W>(h, [=](uint32_t b) {
-        const void *a = b < nfull ? (const void *)(ptr + 64 * (size_t)b)
-                      : b == nfull ? (const void *)tb : (const void *)kZeroBlock;
-        return reinterpret_cast<const W *>(a);
-    }, nmsg);
+    md5_slice_blocks<W>(h, d.ptr, nfull, tb, nmsg);
     for (uint32_t b = nmsg; b < nb; b++) {
         const uint32_t base = b * 64u;
         uint32_t x[16];
@@ -1741,50 +1820,15 @@ __device__ __forceinline__ void md5_packet_bulk16(uint32_t h[4], const uint8_t *
                                                   uint32_t nblk) {
     if (nblk == 0) return;
     uint32_t carry = gload(reinterpret_cast<const uint32_t *>(data + 4 * (16 * (size_t)k0 - 9)));
-    const uint4 *base = reinterpret_cast<const uint4 *>(data + 64 * (size_t)k0 - 32);
-    auto step_words = [&](const uint32_t y[16]) {
-        uint32_t x[16];
-        x[0] = carry;
+    md5_stream_steps(reinterpret_cast<const uint4 *>(data + 64 * (size_t)k0 - 32), nblk,
+                     [&](const uint32_t *y) {
+                         uint32_t x[16];
+                         x[0] = carry;
 #pragma unroll
-        for (int w = 1; w < 16; w++) x[w] = y[w - 1];
-        carry = y[15];
-        md5_compress(h, x);
-    };
-    if (nblk < (uint32_t)(kMd5Ring + kMd5Ahead)) {  // short: as in md5_blocks_at
-        for (uint32_t k = 0; k < nblk; k++) {
-            uint32_t y[16];
-            md5_load_block(y, base + 4 * (size_t)k);
-            step_words(y);
-        }
-        return;
-    }
-    uint32_t buf[kMd5Ring][16];  // block j in slot j % kMd5Ring, as in md5_blocks_at
-#pragma unroll
-    for (int k = 0; k < kMd5Ahead; k++) {  // in block order, as in md5_blocks_at
-        md5_load_block(buf[k], base + 4 * k);
-        __builtin_amdgcn_sched_barrier(0);
-    }
-    auto step = [&](int k) { step_words(buf[k]); };
-    uint32_t b = 0;
-    for (; b + kMd5Ring + kMd5Ahead <= nblk; b += kMd5Ring) {
-#pragma unroll
-        for (int k = 0; k < kMd5Ring; k++) {
-            md5_load_block(buf[(k + kMd5Ahead) % kMd5Ring], base + 4 * (size_t)(b + k + kMd5Ahead));
-            __builtin_amdgcn_sched_barrier(0);  // as in md5_blocks_at
-            step(k);
-            __builtin_amdgcn_sched_barrier(0);
-        }
-    }
-    for (; b < nblk; b += kMd5Ring) {
-#pragma unroll
-        for (int k = 0; k < kMd5Ring; k++) {
-            if (b + k < nblk) {
-                const uint32_t nxt = b + k + kMd5Ahead;
-                if (nxt < nblk) md5_load_block(buf[(k + kMd5Ahead) % kMd5Ring], base + 4 * (size_t)nxt);
-                step(k);
-            }
-        }
-    }
+                         for (int w = 1; w < 16; w++) x[w] = y[w - 1];
+                         carry = y[15];
+                         md5_compress(h, x);
+                     });
 }
 
 __global__ __launch_bounds__(64) void k_recovery_md5(const uint8_t *__restrict__ rec,
@@ -2191,22 +2235,13 @@ __device__ __forceinline__ void slice_md5_crc_lane(const SliceDesc d, const uint
     const uint32_t nb = md5_blocks(S);
     const uint32_t nmsg = S / 64;
     const uint32_t nfull = d.valid / 64;
-    const uint8_t *ptr = d.ptr;
     uint32_t h[4];
     md5_init(h);
     uint32_t c = 0xFFFFFFFFu;
-    md5_blocks_at<W>(
-        h,
-        [=](uint32_t b) {
-            const void *a = b < nfull ? (const void *)(ptr + 64 * (size_t)b)
-                          : b == nfull ? (const void *)tb : (const void *)kZeroBlock;
-            return reinterpret_cast<const W *>(a);
-        },
-        nmsg,
-        [&](const uint32_t *x) {
+    md5_slice_blocks<W>(h, d.ptr, nfull, tb, nmsg, [&](const uint32_t *x) {
 #pragma unroll
-            for (int w = 0; w < 16; w += 4) c = crc_step16(t, c, x[w], x[w + 1], x[w + 2], x[w + 3]);
-        });
+        for (int w = 0; w < 16; w += 4) c = crc_step16(t, c, x[w], x[w + 1], x[w + 2], x[w + 3]);
+    });
     for (uint32_t b = nmsg; b < nb; b++) {
         const uint32_t base = b * 64u;
         uint32_t x[16];

@@ -2,7 +2,7 @@
 every job of a batch gets exactly what the oracle computes for it alone
 (recovery blocks, slice MD5 / CRC-32, packet MD5s), across mixed slice
 sizes and recovery counts, jobs without recovery blocks, jobs past one K
-chunk of 2048 inputs, row pieces and several launch groups, graph replays;
+chunk of 4096 inputs, row pieces and several launch groups, graph replays;
 and the device xorshift64 generator against the documented stream."""
 import os
 import struct
