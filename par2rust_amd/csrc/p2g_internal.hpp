@@ -268,6 +268,10 @@ constexpr uint32_t kBatchProgress = 1;  // sync word: units published so far
 // Sync word of a batched group: slice tasks (64 consecutive slices) taken so
 // far by the slice-hash waves (the fused pass leaves this word unused).
 constexpr uint32_t kBatchSliceQueue = kPassFlags;
+// MD5 + CRC-32 waves of a batched slice-hash workgroup (64 lanes each, one
+// slice per lane; each wave keeps 16 KiB of CRC tables in the workgroup's LDS).
+constexpr uint32_t kBatchSliceWaves = 4;
+constexpr uint32_t kBatchSliceLanes = 64 * kBatchSliceWaves;
 
 // Synthetic input: the xorshift64 stream from `seed` into d_dst.  d_jumps
 // holds 32 GF(2) matrices of 64 columns, step^(kXorshiftLaneWords 2^b); the

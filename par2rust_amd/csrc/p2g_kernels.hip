@@ -2388,13 +2388,14 @@ __global__ __launch_bounds__(kMfThreads, 1) void k_create_batch(BatchArgs a) {
         if (lane == 0) batch_role(a.sync, kPassPacket, true);
         return;
     }
-    if (role.role == kPassSlice && wave < 2) {
-        // slice MD5 + CRC-32 lanes on waves 0 and 1 (CRC tables per wave in
-        // lds.desc).  A wave takes tasks of 64 consecutive slices (one per
-        // lane) from the group's queue; the table runs from the largest
-        // slices to the smallest, so the waves end within about one small
-        // slice of each other.
-        static_assert(sizeof(BatchLds::desc) >= 2 * 16 * 256 * 4, "two waves of 16 CRC tables");
+    if (role.role == kPassSlice && wave < kBatchSliceWaves) {
+        // slice MD5 + CRC-32 lanes on waves 0 .. kBatchSliceWaves - 1 (CRC
+        // tables per wave in lds.desc).  A wave takes tasks of 64 consecutive
+        // slices (one per lane) from the group's queue; the table runs from
+        // the largest slices to the smallest, so the waves end within about
+        // one small slice of each other.
+        static_assert(sizeof(BatchLds::desc) >= kBatchSliceWaves * 16 * 256 * 4,
+                      "16 CRC tables per slice wave");
         uint32_t(*t)[256] = reinterpret_cast<uint32_t(*)[256]>(lds.desc) + 16 * wave;
         crc_tables_init_wave<16>(t);
         const uint32_t n_tasks = (a.slice_hi - a.slice_lo + 63) / 64;
@@ -2422,8 +2423,8 @@ __global__ __launch_bounds__(kMfThreads, 1) void k_create_batch(BatchArgs a) {
         if (lane == 0) batch_role(a.sync, kPassSlice, true);
     }
     // A slice workgroup whose hashes are done encodes what is left in its
-    // XCD's queue, as the encode workgroups do (waves 2 and 3 wait here for
-    // waves 0 and 1).
+    // XCD's queue, as the encode workgroups do (once all its slice waves are
+    // done).
     if (role.role == kPassSlice) __syncthreads();
 
     // Encode.

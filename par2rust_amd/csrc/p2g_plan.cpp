@@ -50,7 +50,7 @@ void update_rates(CostRates &r, const BatchGroupWork &w, uint32_t grid, uint32_t
         used = true;
     }
     if (slice_us > 500 && h2 && w.slice_blocks > 0) {  // the slice lanes, net of the per-slice cost
-        const double lanes = 128.0 * h2;
+        const double lanes = (double)kBatchSliceLanes * h2;
         fold(r.slice_block_us, seed.slice_block_us,
              (slice_us * lanes - w.slices * r.slice_us) / w.slice_blocks);
         used = true;
@@ -90,10 +90,10 @@ void plan_batch_group(uint32_t num_cus, uint32_t max_resident, uint32_t n_tasks,
     // one workgroup per CU, never more than the device holds at once
     num_cus = std::max<uint32_t>(1, std::min(num_cus, max_resident));
     g->grid = num_cus;
-    // four packet waves per packet workgroup, two MD5 + CRC-32 waves (128
-    // lanes) per slice workgroup; the rest encode
+    // four packet waves per packet workgroup, kBatchSliceLanes MD5 + CRC-32
+    // lanes per slice workgroup; the rest encode
     const uint32_t h3_cap = std::min<uint32_t>((n_tasks + 3) / 4, 8);
-    const uint32_t h2_cap = std::min<uint32_t>((n_slices + 127) / 128, 64);
+    const uint32_t h2_cap = std::min<uint32_t>((n_slices + kBatchSliceLanes - 1) / kBatchSliceLanes, 64);
     if (n_units == 0) {
         g->h3_blocks = h3_cap;
         g->h2_blocks = h2_cap;
@@ -116,7 +116,7 @@ void plan_batch_group(uint32_t num_cus, uint32_t max_resident, uint32_t n_tasks,
         const double t_enc = w.enc_steps * rt.step_us / e;
         bool grew = false;
         const double tail_us = rt.slice_us + w.small_blocks * rt.slice_block_us;
-        if (h2 < h2_cap && slice_us / (128.0 * h2) + tail_us > 0.75 * t_enc) {
+        if (h2 < h2_cap && slice_us / ((double)kBatchSliceLanes * h2) + tail_us > 0.75 * t_enc) {
             h2++;
             grew = true;
         }

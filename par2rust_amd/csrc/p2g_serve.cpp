@@ -35,7 +35,7 @@
 
 namespace p2g {
 
-constexpr uint32_t kServeSliceLanes = 24 * 128;  // slice-hash lanes of a batched launch
+constexpr uint32_t kServeSliceLanes = 24 * kBatchSliceLanes;  // slice-hash lanes of a batched launch
 
 // Estimated device time of one recovery row of a job and of one slice's
 // hashes, from the context's measured rates (CostRates): a row block (16
