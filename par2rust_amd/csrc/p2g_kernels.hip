@@ -909,9 +909,16 @@ __device__ __forceinline__ void mf_unit(MfLds &lds, const SliceDesc *__restrict_
 // (g + 3) % 4, was last read in step g - 2.  Scheduling barriers keep each
 // step's instructions between its barriers (MFMAs have no memory operands,
 // so the compiler fences alone would let them sink past later barriers).
+// The dense schedule runs the same ring one step deeper: step g's DMA is for
+// step g + 4 into buffer g % 4 (read in step g - 1, before the barrier that
+// opens step g), and the wait at the top of step g is vmcnt(8), so a load has
+// three steps instead of two to land (the batched mix 0.409-0.412 ->
+// 0.405 s per replay; A fragments there often come from HBM).
 constexpr int kMfRing = kMfTiles * 64 + 128;  // uint4 per ring buffer: A, then 512 words
 constexpr int kWaitVmcnt4 = 0x0074;  // s_waitcnt vmcnt(4) lgkmcnt(0) (this wave's LDS reads done)
 constexpr int kWaitVmcnt8 = 0x0F78;
+constexpr int kWaitVmcnt12 = 0x0F7C;
+constexpr int kWaitVmcnt8Lds = 0x0078;  // vmcnt(8) lgkmcnt(0)
 constexpr int kWaitVmcnt0 = 0x0070;  // vmcnt(0) lgkmcnt(0)
 
 // Addresses of the two source words of K step g of this lane (whole slices).
@@ -991,9 +998,12 @@ __device__ __forceinline__ void mf_unit_dma(const uint4 *s_desc, uint4 *A0, uint
     MF_DMA(0, A0);
     MF_DMA(1, A1);
     MF_DMA(2, A2);
+    if constexpr (DENSE) MF_DMA(3, A3);
     v8i_t Bc[4];
     uint4 Ac[4];
-    __builtin_amdgcn_s_waitcnt(kWaitVmcnt8);  // group 0 of this wave
+    // group 0 of this wave
+    if constexpr (DENSE) __builtin_amdgcn_s_waitcnt(kWaitVmcnt12);
+    else __builtin_amdgcn_s_waitcnt(kWaitVmcnt8);
     asm volatile("" ::: "memory");
     __builtin_amdgcn_s_barrier();
     asm volatile("" ::: "memory");
@@ -1002,7 +1012,8 @@ __device__ __forceinline__ void mf_unit_dma(const uint4 *s_desc, uint4 *A0, uint
     for (int a = 0; a < 4; a++) Ac[a] = A0[ard + a * 64];
 
     // step GG (both schedules): MFMAs on Ac = A(GG), Bc = B(GG); DMA(GG + 3)
-    // into BUFW; B(GG + 1) and A(GG + 1) from BUFN.
+    // (light) or DMA(GG + 4) (dense) into BUFW; B(GG + 1) and A(GG + 1) from
+    // BUFN.
     //
     // Light schedule: scheduling-group hints only, which leave the DMA issue
     // and most of the B build in runs between few MFMAs.
@@ -1081,7 +1092,7 @@ __device__ __forceinline__ void mf_unit_dma(const uint4 *s_desc, uint4 *A0, uint
     } while (0)
 #define MF_STEP_DENSE(GG, BUFN, BUFW)                                                         \
     do {                                                                                      \
-        const uint32_t gq = min((uint32_t)(GG) + 3, nq - 1);                                  \
+        const uint32_t gq = min((uint32_t)(GG) + 4, nq - 1);                                  \
         const uint32_t *wb = reinterpret_cast<const uint32_t *>(BUFN) + wrd;                  \
         uint32_t x00, x01, x10, x11, xs;                                                      \
         uint2 d0, d1;                                                                         \
@@ -1090,7 +1101,7 @@ __device__ __forceinline__ void mf_unit_dma(const uint4 *s_desc, uint4 *A0, uint
         MF_SB;                                                                                \
         MF_MM(0, 0);                                                                          \
         MF_SB;                                                                                \
-        __builtin_amdgcn_s_waitcnt(kWaitVmcnt4);                                              \
+        __builtin_amdgcn_s_waitcnt(kWaitVmcnt8Lds);                                           \
         asm volatile("" ::: "memory");                                                        \
         __builtin_amdgcn_s_barrier();                                                         \
         asm volatile("" ::: "memory");                                                        \
@@ -1105,7 +1116,7 @@ __device__ __forceinline__ void mf_unit_dma(const uint4 *s_desc, uint4 *A0, uint
         x10 = wb[128];                                                                        \
         x11 = wb[192];                                                                        \
         {                                                                                     \
-            const uint32_t gd = min((uint32_t)(GG) + 4, nq - 1);                              \
+            const uint32_t gd = min((uint32_t)(GG) + 5, nq - 1);                              \
             d0 = reinterpret_cast<const uint2 *>(s_desc + 4 * gd + 2 * h)[0];                 \
             d1 = reinterpret_cast<const uint2 *>(s_desc + 4 * gd + 2 * h + 1)[0];             \
         }                                                                                     \
@@ -1159,10 +1170,10 @@ __device__ __forceinline__ void mf_unit_dma(const uint4 *s_desc, uint4 *A0, uint
 
     if constexpr (DENSE) {
         for (uint32_t g = 0; g < nq; g += 4) {
-            MF_STEP_DENSE(g, A1, A3);
-            MF_STEP_DENSE(g + 1, A2, A0);
-            MF_STEP_DENSE(g + 2, A3, A1);
-            MF_STEP_DENSE(g + 3, A0, A2);
+            MF_STEP_DENSE(g, A1, A0);
+            MF_STEP_DENSE(g + 1, A2, A1);
+            MF_STEP_DENSE(g + 2, A3, A2);
+            MF_STEP_DENSE(g + 3, A0, A3);
         }
     } else {
         for (uint32_t g = 0; g < nq; g += 4) {
