@@ -11,7 +11,7 @@
 //
 // A step is one batched plan (build_plan) over every admitted job with work
 // left: whole jobs when their estimated device time fits the step target
-// (50 ms unless the caller sets one), otherwise a piece of recovery rows
+// (25 ms unless the caller sets one), otherwise a piece of recovery rows
 // sized to the target; a job's slice checksums likewise go in as many
 // rounds of the slice lanes as fit the target.  Steps run in order on one
 // stream, at most two queued at a time (the next one is planned while the
@@ -118,7 +118,7 @@ struct Step {
 struct Server {
     Ctx *c = nullptr;
     Pool pool;
-    double step_ms = 50.0;
+    double step_ms = 25.0;
     // How long a request that finds no memory may be overtaken by later
     // requests that fit (0: strict arrival order).
     double bypass_ms = 200.0;
