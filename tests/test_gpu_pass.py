@@ -163,10 +163,14 @@ def test_create_pass_twice_reuses_state(ctx):
     assert run_pass(ctx, b, 4096, 5, sid) == expected(b, 4096, 5, sid)
 
 
-@pytest.mark.parametrize("lens,S,R", [([70000, 4100], 4096, 7), ([777], 12, 4), ([1 << 20, 9], 16384, 33)])
+@pytest.mark.parametrize("lens,S,R", [([70000, 4100], 4096, 7), ([777], 12, 4), ([1 << 20, 9], 16384, 33),
+                                      ([4096 * 10 + 1000], 4096, 5), ([65536 * 70 + 40000], 65536, 3)])
 def test_create_pass_partial_slices_in_place(ctx, monkeypatch, lens, S, R):
     # no staging budget: partial slices are read in place (the general
-    # instantiation of the pass, with tail words and masked word addresses)
+    # instantiation of the pass, with tail words and masked word addresses).
+    # The last two put a partial slice with whole 64-byte blocks beside whole
+    # slices in one wave: its MD5 streams the wave's common whole prefix, then
+    # selects the tail block and zero blocks per block.
     monkeypatch.setenv("P2G_STAGE_BYTES", "0")
     files = [data(n, 500 + i) for i, n in enumerate(lens)]
     sid = data(16, 79)
