@@ -138,8 +138,17 @@ static void add_slice_work(BatchGroupWork &w, uint32_t cnt, uint32_t S) {
 }
 
 // The sizing work of one piece: its encode unit-steps and packet tasks.
+// Blocks of a packet body's hash within data bytes [0, e) (as the packet
+// lanes count them).
+static uint32_t packet_blocks_within(uint64_t e) { return e >= 28 ? (uint32_t)((e - 28) / 64) + 1 : 0u; }
+
 static void add_piece_work(BatchGroupWork &w, const BatchPiece &P) {
-    const double blocks = (double)((36ull + P.S + 8) / 64 + 1);  // of a packet's hash
+    // the packet hash blocks the piece's column groups hold
+    const uint64_t nb = (36ull + P.S + 8) / 64 + 1;
+    const uint64_t e1 = std::min<uint64_t>(512ull * (P.cg0 + P.col_groups), P.S);
+    const uint64_t hi = P.cg0 + P.col_groups >= P.cg_total ? nb : packet_blocks_within(e1);
+    const uint64_t lo = P.cg0 ? packet_blocks_within(512ull * P.cg0) : 0;
+    const double blocks = (double)(hi > lo ? hi - lo : 1);
     const uint32_t t = (P.R + 63) / 64;
     const double steps = (double)P.row_blocks * P.col_groups * P.nq;
     w.enc_steps += steps;
@@ -167,6 +176,11 @@ int build_plan(Ctx *c, const std::vector<PlanItem> &items, bool use_graph, Batch
         if (rc != P2G_OK) return rc;
         if (it.r0 > it.r1 || it.r1 > it.job->recovery_count)
             return c->fail(P2G_ERR_INVALID_ARGUMENT, "recovery row range of a batch item");
+        const uint64_t cg_total = (it.job->slice_size + 511) / 512;
+        if (it.c0 >= std::min<uint64_t>(it.c1, cg_total) && it.r0 < it.r1)
+            return c->fail(P2G_ERR_INVALID_ARGUMENT, "column range of a batch item");
+        if ((it.c0 > 0 || it.c1 < cg_total) && it.r0 < it.r1 && !it.md5_state)
+            return c->fail(P2G_ERR_INVALID_ARGUMENT, "a batch item of part of a row needs its hash state");
     }
     std::vector<uint32_t> order(nitems);
     std::iota(order.begin(), order.end(), 0u);
@@ -268,7 +282,12 @@ int build_plan(Ctx *c, const std::vector<PlanItem> &items, bool use_graph, Batch
             P.first_exp = r0;
             P.job = j;
             P.row_blocks = (Rp + 15) / 16;
-            P.col_groups = (S + 511) / 512;
+            P.cg_total = (S + 511) / 512;
+            P.cg0 = items[j].c0;
+            P.col_groups = std::min(items[j].c1, P.cg_total) - P.cg0;
+            P.md5_state = items[j].md5_state
+                              ? static_cast<uint4 *>(items[j].md5_state) + r0
+                              : nullptr;
             TuneChoice ch;
             const uint32_t band = tune_lookup(c, n, J.recovery_count, S, &ch) &&
                                           ch.variant == kEncodeMfma && ch.band_cols
@@ -599,7 +618,7 @@ int p2g_plan_roles(int kind, uint32_t num_cus, uint32_t max_resident, uint32_t s
             P.R = R;
             P.S = S;
             P.row_blocks = (R + 15) / 16;
-            P.col_groups = (S + 511) / 512;
+            P.col_groups = P.cg_total = (S + 511) / 512;
             BatchGroupWork w;
             add_slice_work(w, n, S);
             add_piece_work(w, P);

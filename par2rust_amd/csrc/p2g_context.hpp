@@ -142,6 +142,11 @@ struct PlanItem {
     const p2g_batch_job *job;
     uint32_t r0, r1;
     uint32_t s0, s1;
+    // column groups [c0, c1) of the rows (c1 past the row: to its end); a
+    // range short of a whole row continues or saves each row's packet hash
+    // in md5_state (the job's R chaining states, 16 bytes each)
+    uint32_t c0 = 0, c1 = 0xFFFFFFFFu;
+    void *md5_state = nullptr;
 };
 int check_batch_job(Ctx *c, const p2g_batch_job &J, std::vector<SliceDesc> &descs);
 int build_plan(Ctx *c, const std::vector<PlanItem> &items, bool use_graph, BatchPlan *b);

@@ -187,8 +187,14 @@ struct BatchPiece {
     uint32_t first_exp, job, row_blocks, col_groups;
     uint32_t band_cols, n_bands, unit0, unit_end;
     uint32_t task0, task_end, band0, bstart0;
+    // Column groups [cg0, cg0 + col_groups) of the rows (all of them unless
+    // a serving step split the rows' columns over several launches).  A
+    // piece that starts past column group 0 continues each row's packet hash
+    // from md5_state; one that ends before the last group saves it there.
+    uint4 *md5_state;         // chaining state of row 0 of the piece (or null)
+    uint32_t cg0, cg_total;   // first column group; column groups of a whole row
 };
-static_assert(sizeof(BatchPiece) == 128, "BatchPiece layout");
+static_assert(sizeof(BatchPiece) == 144, "BatchPiece layout");
 
 // One job's slices: global slice indices [slice0, slice_end) of the batch
 // descriptor table.

@@ -2329,9 +2329,19 @@ __device__ __forceinline__ bool batch_wait(const uint32_t *cnt, uint32_t target,
     return true;
 }
 
+// MD5 blocks of a packet body whose data bytes lie in [0, e): block k holds
+// data bytes up to 64 k + 28.
+__device__ __forceinline__ uint32_t packet_blocks_within(uint64_t e) {
+    return e >= 28 ? (uint32_t)((e - 28) / 64) + 1 : 0u;
+}
+
 // Packet r of piece P (exponent P.first_exp + r), band by band as in
 // pass_packet_lane.  A lane with store = false hashes along (its wave waits
-// together) and writes nothing.
+// together) and writes nothing.  A piece of column groups [cg0, cg0 +
+// col_groups) continues the row's hash from P.md5_state when cg0 > 0 (the
+// earlier pieces hashed every block within column group cg0's start) and,
+// when it ends before the row does, saves the state there instead of the
+// digest.
 __device__ __forceinline__ void batch_packet_lane(const BatchArgs &a, const BatchPiece &P,
                                                   uint32_t r, bool store) {
     const uint32_t S = P.S;
@@ -2341,20 +2351,27 @@ __device__ __forceinline__ void batch_packet_lane(const BatchArgs &a, const Batc
     const uint32_t nb = md5_blocks(len), nfull = (uint32_t)(len / 64);
     const uint32_t bulk_end = nfull > 1 ? nfull : 1;
     const uint32_t exp = P.first_exp + r;
+    const bool last = P.cg0 + P.col_groups >= P.cg_total;
     PacketPrefix pre;
 #pragma unroll
     for (int w = 0; w < 8; w++) pre.w[w] = P.pre[w];
     uint32_t h[4];
-    md5_init(h);
     uint32_t k = 0;
+    if (P.cg0 == 0) {
+        md5_init(h);
+    } else {
+        const uint4 st = P.md5_state[r];
+        h[0] = st.x; h[1] = st.y; h[2] = st.z; h[3] = st.w;
+        k = min(packet_blocks_within((uint64_t)P.cg0 * (kMfThreads * 2)), bulk_end);
+    }
     const uint32_t *bstart = a.sync + P.bstart0;
     for (uint32_t b = 0; b < P.n_bands; b++) {
         const uint32_t c1 = bstart[b + 1], w = c1 - bstart[b];
         if (!batch_wait(a.sync + P.band0 + b, w * P.row_blocks, a.sync + kBatchProgress,
                         a.stall_ticks, a.err))
             return;
-        const uint64_t e = min((uint64_t)c1 * (kMfThreads * 2), (uint64_t)S);
-        const uint32_t avail = b + 1 == P.n_bands ? nb : e >= 28 ? (uint32_t)((e - 28) / 64) + 1 : 0;
+        const uint64_t e = min((uint64_t)(P.cg0 + c1) * (kMfThreads * 2), (uint64_t)S);
+        const uint32_t avail = b + 1 == P.n_bands && last ? nb : packet_blocks_within(e);
         if (k == 0 && avail > 0) {
             md5_packet_block(h, pre, exp, data, len, 0, nb);
             k = 1;
@@ -2365,6 +2382,10 @@ __device__ __forceinline__ void batch_packet_lane(const BatchArgs &a, const Batc
             else md5_bulk<uint32_t>(h, data + 64 * (size_t)k - 36, stop - k);
             k = stop;
         }
+    }
+    if (!last) {
+        if (store) P.md5_state[r] = make_uint4(h[0], h[1], h[2], h[3]);
+        return;
     }
     for (; k < nb; k++) md5_packet_block(h, pre, exp, data, len, k, nb);
     if (store) store_digest(P.pkt_md5 + (size_t)r * 16, h);
@@ -2473,7 +2494,7 @@ __global__ __launch_bounds__(kMfThreads, 1) void k_create_batch(BatchArgs a) {
             b_hi = P.unit0 + P.row_blocks * bstart[band + 1];
         }
         const uint32_t c0 = bstart[band], w = bstart[band + 1] - c0;
-        const uint32_t v = u - b_lo, y = v / w, x = c0 + v % w;
+        const uint32_t v = u - b_lo, y = v / w, x = P.cg0 + c0 + v % w;
         const uint4 *afrag = static_cast<const uint4 *>(P.afrag);
         for (uint32_t k0 = 0; k0 < P.n; k0 += kMfmaChunkInputs) {
             const SliceDesc *cin = P.inputs + k0;

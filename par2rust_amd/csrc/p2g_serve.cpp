@@ -12,8 +12,11 @@
 // A step is one batched plan (build_plan) over every admitted job with work
 // left: whole jobs when their estimated device time fits the step target
 // (25 ms unless the caller sets one), otherwise a piece of recovery rows
-// sized to the target; a job's slice checksums likewise go in as many
-// rounds of the slice lanes as fit the target.  Steps run in order on one
+// sized to the target, or, when one row block (16 rows) is longer than what
+// the whole jobs leave of the step, a run of that row block's column groups
+// (the packet hashes of its rows carry their MD5 state from piece to
+// piece); a job's slice checksums likewise go in as many rounds of the
+// slice lanes as fit the target.  Steps run in order on one
 // stream, at most two queued at a time (the next one is planned while the
 // current one runs), so requests that arrive while the device is busy join
 // the next step and a long job never holds the others back for more than a
@@ -36,6 +39,7 @@
 namespace p2g {
 
 constexpr uint32_t kServeSliceLanes = 24 * kBatchSliceLanes;  // slice-hash lanes of a batched launch
+constexpr uint32_t kServeMinCols = 32;  // column groups (16 KiB) of the smallest column piece
 
 // Estimated device time of one recovery row of a job and of one slice's
 // hashes, from the context's measured rates (CostRates): a row block (16
@@ -104,6 +108,12 @@ struct Request {
     p2g_batch_job job{};
     uint32_t rows_assigned = 0, rows_done = 0;
     uint32_t slices_assigned = 0, slices_done = 0;
+    // A row block too long for one step goes in as pieces of its column
+    // groups over several steps: cols_assigned of the rows from
+    // rows_assigned are assigned; md5_state holds their packet hashes'
+    // chaining states between the pieces.
+    uint32_t cols_assigned = 0;
+    uint8_t *md5_state = nullptr;
     double t_submit = 0, t_admit = 0, t_done = 0;  // ms on the server clock
     double t_blocked = -1;  // when admission first found no memory for it (-1: not yet)
 };
@@ -202,6 +212,8 @@ int Server::admit() {
         q->job.slice_crc = reinterpret_cast<uint32_t *>(p);
         p += a256(4ull * q->n);
         q->job.pkt_md5 = p;
+        p += a256(16ull * q->R);
+        q->md5_state = p;
         q->state = Request::kActive;
         q->t_admit = now_ms();
         active.push_back(q);
@@ -213,15 +225,37 @@ int Server::launch_step(int k) {
     Step &st = steps[k];
     st.items.clear();
     std::vector<PlanItem> items;
+    // The jobs that go whole take their share of the step first; a row block
+    // longer than the step goes in as column pieces sized to what they leave
+    // (at least a quarter of the target), so such a job does not stretch
+    // every step past the target.
+    double whole_ms = 0;
+    for (Request *q : active) {
+        const uint32_t left = q->R - q->rows_assigned;
+        const double w = ms_per_row(c, q->n, q->S) * left;
+        if (q->cols_assigned == 0 && w <= step_ms) whole_ms += w;
+    }
+    const double split_ms = std::max(step_ms - whole_ms, step_ms / 4);
     for (Request *q : active) {
         if (q->rows_assigned == q->R && q->slices_assigned == q->n) continue;
         const uint32_t left = q->R - q->rows_assigned;
-        uint32_t take = left;
+        const uint32_t cg_total = (uint32_t)((q->S + 511) / 512);
+        uint32_t take = left, c0 = 0, c1 = cg_total;
         const double row_ms = ms_per_row(c, q->n, q->S);
-        if (left && row_ms * left > step_ms) {
+        if (left && (q->cols_assigned > 0 || row_ms * left > step_ms)) {
             // a piece of whole row blocks sized to the step target
             const uint32_t rows = (uint32_t)(step_ms / row_ms);
             take = std::min(left, std::max<uint32_t>(16, rows / 16 * 16));
+            const uint32_t rb = std::min<uint32_t>(16, left);
+            const double rb_ms = row_ms * rb;
+            if (q->cols_assigned > 0 || (rb_ms > split_ms && cg_total >= 2 * kServeMinCols)) {
+                // one row block is longer than the step: the next run of
+                // its column groups (never a tail shorter than the minimum)
+                take = rb;
+                c0 = q->cols_assigned;
+                const uint32_t want = std::max<uint32_t>(kServeMinCols, (uint32_t)(split_ms / rb_ms * cg_total));
+                c1 = cg_total - c0 < want + kServeMinCols ? cg_total : c0 + want;
+            }
         }
         // slice checksums: as many rounds of the slice lanes as fit the target
         const uint32_t sleft = q->n - q->slices_assigned;
@@ -229,7 +263,15 @@ int Server::launch_step(int k) {
         const uint32_t stake = (uint32_t)std::min<uint64_t>(sleft, (uint64_t)rounds * kServeSliceLanes);
         PlanItem it{&q->job, q->rows_assigned, q->rows_assigned + take, q->slices_assigned,
                     q->slices_assigned + stake};
-        q->rows_assigned += take;
+        it.c0 = c0;
+        it.c1 = c1;
+        it.md5_state = q->md5_state;
+        if (c1 == cg_total) {
+            q->rows_assigned += take;
+            q->cols_assigned = 0;
+        } else {
+            q->cols_assigned = c1;
+        }
         q->slices_assigned += stake;
         st.items.push_back({q, it});
         items.push_back(it);
@@ -243,7 +285,9 @@ int Server::launch_step(int k) {
     st.est_ms = 0;
     for (auto &pr : st.items) {
         const Request *q = pr.first;
-        st.est_ms += ms_per_row(c, q->n, q->S) * (pr.second.r1 - pr.second.r0);
+        const double cols = (double)((q->S + 511) / 512);
+        st.est_ms += ms_per_row(c, q->n, q->S) * (pr.second.r1 - pr.second.r0) *
+                     (std::min<double>(pr.second.c1, cols) - pr.second.c0) / cols;
     }
     if (rc != P2G_OK) {
         // whatever part of the step was queued ends before its requests are
@@ -296,7 +340,8 @@ int Server::retire(int k) {
             active.erase(std::find(active.begin(), active.end(), q));
             continue;
         }
-        q->rows_done += pr.second.r1 - pr.second.r0;
+        // rows count once their last column piece is done
+        if (pr.second.c1 >= (q->S + 511) / 512) q->rows_done += pr.second.r1 - pr.second.r0;
         q->slices_done += pr.second.s1 - pr.second.s0;
         if (q->rows_done == q->R && q->slices_done == q->n) {
             q->state = Request::kDone;
@@ -393,7 +438,8 @@ int p2g_server_submit(p2g_server *srv, const p2g_request *r, uint64_t *id_out) {
         if (n > kMaxInputs) return c->fail(P2G_ERR_TOO_MANY_INPUT_BLOCKS, "more than 32768 input slices");
         q->n = (uint32_t)n;
         memcpy(q->set_id, r->set_id, 16);
-        q->footprint = bytes + a256((uint64_t)q->R * q->S) + a256(16 * n) + a256(4 * n) + a256(16ull * q->R);
+        q->footprint = bytes + a256((uint64_t)q->R * q->S) + a256(16 * n) + a256(4 * n) +
+                       2 * a256(16ull * q->R);  // packet digests, then their chaining states
         if (q->footprint > s->pool.size)
             return c->fail(P2G_ERR_OUT_OF_MEMORY, "request larger than the server's HBM budget");
         q->id = s->next_id++;
