@@ -1,7 +1,7 @@
 """GPU tests of the serving loop (p2g_server_*): every request's outputs are
 the same as its job computed alone (the oracle), whether it runs whole in a
-step, in row pieces across steps, or waits in the queue for HBM; and the
-loop's error and accounting paths."""
+step, in row or column pieces across steps, or waits in the queue for HBM;
+and the loop's error and accounting paths."""
 import time
 
 import pytest
@@ -68,7 +68,9 @@ def test_server_results_equal_alone(ctx):
 
 def test_server_row_pieces_across_steps(ctx):
     # a tiny step target: every job with more than 16 rows runs 16-row pieces
-    # over several steps, mixed with the other requests
+    # over several steps, mixed with the other requests, and the 64 KiB-slice
+    # job's row blocks (128 column groups) run as column pieces of 32 groups
+    # whose packet hashes continue from the saved MD5 state
     srv = ctx.server(1 << 30, step_ms=1e-6)
     ids = run_all(srv, REQS)
     for rid, (files, S, R, sid) in ids.items():
