@@ -322,7 +322,7 @@ def run_serving(ctx, seconds: float, rate: float, budget: int, verify: int = 4,
     pct = lambda v, q: round(float(np.percentile(np.array(v) * 1e3, q)), 2)
     out = {
         "requests": len(reqs), "arrival_rate_jobs_per_s": round(rate, 1), "arrival_seconds": seconds,
-        "hbm_budget_bytes": budget, "step_target_ms": step_ms or 25.0, "source_bytes": src_bytes,
+        "hbm_budget_bytes": budget, "step_target_ms": step_ms or 35.0, "source_bytes": src_bytes,
         "source_GBps": round(src_bytes / wall / 1e9, 2), "wall_s": round(wall, 3),
         "latency_ms": {"p50": pct(small, 50), "p99": pct(small, 99), "max": pct(small, 100)},
         "big_job": {"bytes": 32 << 30, "slice_size": 1 << 20, "slices": 32768, "recovery": 3000,
@@ -565,7 +565,7 @@ def main() -> int:
                     help="arrival rate as a fraction of the measured batched jobs/s")
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
     ap.add_argument("--serve-step-ms", type=float, default=0.0,
-                    help="the server's step target (0: its default, 25 ms)")
+                    help="the server's step target (0: its default, 35 ms)")
     ap.add_argument("--no-many-files", action="store_true",
                     help="skip the many-files (64 files, R = 1000) pass timing")
     ap.add_argument("--no-end-to-end", action="store_true",

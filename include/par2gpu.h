@@ -276,11 +276,11 @@ typedef struct {
 } p2g_request;
 
 /* hbm_budget: bytes of device memory for admitted requests (allocated at
- * open); step_ms: target device time of one step (0: 25 ms). */
+ * open); step_ms: target device time of one step (0: 35 ms). */
 int p2g_server_open(p2g_ctx *ctx, uint64_t hbm_budget, double step_ms, p2g_server **out);
 /* How long a queued request that finds no memory may be overtaken by later
  * requests that fit, from when it first found none (default: 4 step
- * targets, 100 ms at the default target; 0: strict arrival order).  After
+ * targets, 140 ms at the default target; 0: strict arrival order).  After
  * that it holds back every later request until it is admitted. */
 int p2g_server_set_bypass(p2g_server *s, double bypass_ms);
 /* Queues a request; id_out identifies it.  Fails at once for a request

@@ -11,7 +11,7 @@
 //
 // A step is one batched plan (build_plan) over every admitted job with work
 // left: whole jobs when their estimated device time fits the step target
-// (25 ms unless the caller sets one), otherwise a piece of recovery rows
+// (35 ms unless the caller sets one), otherwise a piece of recovery rows
 // sized to the target, or, when one row block (16 rows) is longer than what
 // the whole jobs leave of the step, a run of that row block's column groups
 // (the packet hashes of its rows carry their MD5 state from piece to
@@ -128,7 +128,7 @@ struct Step {
 struct Server {
     Ctx *c = nullptr;
     Pool pool;
-    double step_ms = 25.0;
+    double step_ms = 35.0;
     // How long a request that finds no memory may be overtaken by later
     // requests that fit (0: strict arrival order).
     double bypass_ms = 200.0;
