@@ -85,8 +85,10 @@ def test_serving_large_job_full_size(ctx):
     """The serving workload's large job at its full size: one 32 GiB file
     (xorshift64, seed 4000, written on the device by the server), slice size
     1 MiB (32768 slices, the PAR2 maximum), 3000 recovery blocks.  The server
-    runs it in row pieces across 50 ms steps; p2g_dev_create_pass then runs it
-    alone on the same device bytes (a one-job batched launch).  Checks:
+    runs it across 10 ms steps: each 16-row block (~27 ms of device time) in
+    three pieces of its column groups, the packet hashes carrying their MD5
+    state from piece to piece; p2g_dev_create_pass then runs it alone on the
+    same device bytes (a one-job batched launch).  Checks:
     * both give the same recovery blocks, slice checksums and packet MD5s;
     * every slice MD5 and CRC-32 equals the oracle's, the source streamed to
       the host in 1 GiB chunks;
@@ -107,7 +109,7 @@ def test_serving_large_job_full_size(ctx):
     S, R, N = 1 << 20, 3000, 32768
     size = N * S
     sid = bytes(range(100, 116))
-    srv = ctx.server(48 << 30)
+    srv = ctx.server(48 << 30, step_ms=10.0)
     try:
         rid = srv.submit([size], S, R, sid, seeds=[4000])
         t0, done = time.time(), []
