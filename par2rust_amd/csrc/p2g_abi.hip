@@ -421,9 +421,20 @@ int p2g_dev_fill_xorshift64(p2g_ctx *ctx, uint8_t *dst, uint64_t nbytes, uint64_
                 for (int b = 0; b < 64; b++) t[b] = xs_apply_host(m.data(), m[b]);
                 m = t;
             }
-            if (c->d_xs_jumps.reserve(jumps.size() * 8) != P2G_OK)
+            // each jump as 8 byte tables: entry v of table q of jump k is the
+            // XOR of the columns 8 q + i of jumps[k] for the set bits i of v
+            std::vector<uint64_t> tables((size_t)32 * kXsJumpTableWords);
+            for (int k = 0; k < 32; k++)
+                for (int q = 0; q < 8; q++)
+                    for (int v = 0; v < 256; v++) {
+                        uint64_t x = 0;
+                        for (int i = 0; i < 8; i++)
+                            if ((v >> i) & 1) x ^= jumps[64 * k + 8 * q + i];
+                        tables[(size_t)k * kXsJumpTableWords + 256 * q + v] = x;
+                    }
+            if (c->d_xs_jumps.reserve(tables.size() * 8) != P2G_OK)
                 return c->fail(P2G_ERR_OUT_OF_MEMORY, "xorshift jump table");
-            hipError_t e = hipMemcpy(c->d_xs_jumps.ptr, jumps.data(), jumps.size() * 8,
+            hipError_t e = hipMemcpy(c->d_xs_jumps.ptr, tables.data(), tables.size() * 8,
                                      hipMemcpyHostToDevice);
             if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "upload xorshift jumps", e);
         }

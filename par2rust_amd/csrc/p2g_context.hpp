@@ -88,7 +88,7 @@ struct Ctx {
     uint8_t *noalloc_scratch = nullptr;
     hipEvent_t noalloc_done = nullptr;
     DevBuf d_partials;             // zero-padded copies of partial slices (fused pass)
-    DevBuf d_xs_jumps;             // xorshift64 jump matrices (synthetic input)
+    DevBuf d_xs_jumps;             // xorshift64 jumps as byte tables (synthetic input)
     bool pass_pending = false;     // a fused pass may have set error flags not yet read
     bool last_pass_fused = false;  // the last p2g_dev_create_pass ran as one k_create_pass
     // Shared device buffers and the event recorded after the last kernel that

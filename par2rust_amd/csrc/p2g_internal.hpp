@@ -280,12 +280,15 @@ constexpr uint32_t kBatchSliceWaves = 4;
 constexpr uint32_t kBatchSliceLanes = 64 * kBatchSliceWaves;
 
 // Synthetic input: the xorshift64 stream from `seed` into d_dst.  d_jumps
-// holds 32 GF(2) matrices of 64 columns, step^(kXorshiftLaneWords 2^b); the
-// destination must be 16-byte aligned.  A lane writes kXorshiftLaneWords
+// holds the 32 GF(2) matrices step^(kXorshiftLaneWords 2^b) as byte tables
+// (kXsJumpTableWords each); the destination must be 16-byte aligned.  A lane writes kXorshiftLaneWords
 // words (1 KiB) after jumping to its start, so a file of a few MB is already
 // thousands of lanes (with 16 KiB per lane a 13 MB file ran on four
 // workgroups at ~100 GB/s).
 constexpr uint32_t kXorshiftLaneWords = 128;
+// A jump step^(W 2^b) as 8 tables of 256 words: table q maps byte q of a
+// state to that byte's share of the jumped state.
+constexpr uint32_t kXsJumpTableWords = 8 * 256;
 hipError_t launch_xorshift_fill(uint8_t *d_dst, uint64_t nbytes, uint64_t seed,
                                 const uint64_t *d_jumps, hipStream_t stream);
 

@@ -2646,13 +2646,16 @@ hipError_t launch_create_batch(const BatchGroupLaunch &g, const SliceDesc *d_des
 // Word k of the stream is the state after k + 1 steps (x ^= x << 13;
 // x ^= x >> 7; x ^= x << 17) from the seed, stored little-endian.  The step
 // is linear over GF(2), so lane j starts from step^(j W) (seed), built from
-// jumps[b] = step^(W 2^b) (64 columns each) for the set bits b of j, and
-// writes words [j W, (j + 1) W).
+// the jumps step^(W 2^b) for the set bits b of j, and writes words
+// [j W, (j + 1) W).  A jump is applied as 8 byte-table lookups (the XOR of
+// each state byte's image): one select-and-XOR per state bit had made the
+// jumps ~87% of a fill wave's instructions, and a serving run of seeded
+// requests spent 7% of its device time in fills.
 
-__device__ __forceinline__ uint64_t xs_apply(const uint64_t *__restrict__ cols, uint64_t v) {
+__device__ __forceinline__ uint64_t xs_apply(const uint64_t *__restrict__ tables, uint64_t v) {
     uint64_t out = 0;
-#pragma unroll 8
-    for (int b = 0; b < 64; b++) out ^= cols[b] & (0ull - ((v >> b) & 1ull));
+#pragma unroll
+    for (int q = 0; q < 8; q++) out ^= tables[256 * q + ((v >> (8 * q)) & 0xFFu)];
     return out;
 }
 
@@ -2664,7 +2667,7 @@ __global__ __launch_bounds__(256) void k_xorshift_fill(uint8_t *__restrict__ dst
     if (j >= lanes) return;
     uint64_t s = seed;
     for (int b = 0; b < 32; b++)
-        if ((j >> b) & 1u) s = xs_apply(jumps + 64 * b, s);
+        if ((j >> b) & 1u) s = xs_apply(jumps + (size_t)kXsJumpTableWords * b, s);
     const uint64_t w0 = (uint64_t)j * kXorshiftLaneWords;
     const uint64_t words = nbytes / 8;
     uint64_t *out = reinterpret_cast<uint64_t *>(dst);
