@@ -351,6 +351,57 @@ int dev_encode_files(Ctx *c, const p2g_dev_file *files, uint32_t nfiles, uint64_
     return c->scratch_release(Ctx::kScratchEnc, st);
 }
 
+// GF(2) 64x64 matrices as 64 column images.
+static uint64_t xs_apply_host(const uint64_t *m, uint64_t v) {
+    uint64_t out = 0;
+    for (int b = 0; b < 64; b++)
+        if ((v >> b) & 1) out ^= m[b];
+    return out;
+}
+
+// The jump tables of the synthetic-input fill (built and uploaded once per
+// context): c->d_xs_jumps.
+int xs_jump_tables(Ctx *c) {
+    if (!c->d_xs_jumps.ptr) {
+        // step^(W 2^b) for b = 0..31, W = kXorshiftLaneWords
+        std::vector<uint64_t> step(64), m(64), t(64), jumps(64 * 32);
+        for (int b = 0; b < 64; b++) {
+            uint64_t x = 1ull << b;
+            x ^= x << 13;
+            x ^= x >> 7;
+            x ^= x << 17;
+            step[b] = x;
+        }
+        m = step;  // step^(2^k), squared until 2^k = W
+        for (uint32_t w = 1; w < kXorshiftLaneWords; w *= 2) {
+            for (int b = 0; b < 64; b++) t[b] = xs_apply_host(m.data(), m[b]);
+            m = t;
+        }
+        for (int k = 0; k < 32; k++) {
+            std::copy(m.begin(), m.end(), jumps.begin() + 64 * k);
+            for (int b = 0; b < 64; b++) t[b] = xs_apply_host(m.data(), m[b]);
+            m = t;
+        }
+        // each jump as 8 byte tables: entry v of table q of jump k is the
+        // XOR of the columns 8 q + i of jumps[k] for the set bits i of v
+        std::vector<uint64_t> tables((size_t)32 * kXsJumpTableWords);
+        for (int k = 0; k < 32; k++)
+            for (int q = 0; q < 8; q++)
+                for (int v = 0; v < 256; v++) {
+                    uint64_t x = 0;
+                    for (int i = 0; i < 8; i++)
+                        if ((v >> i) & 1) x ^= jumps[64 * k + 8 * q + i];
+                    tables[(size_t)k * kXsJumpTableWords + 256 * q + v] = x;
+                }
+        if (c->d_xs_jumps.reserve(tables.size() * 8) != P2G_OK)
+            return c->fail(P2G_ERR_OUT_OF_MEMORY, "xorshift jump table");
+        hipError_t e = hipMemcpy(c->d_xs_jumps.ptr, tables.data(), tables.size() * 8,
+                                 hipMemcpyHostToDevice);
+        if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "upload xorshift jumps", e);
+    }
+    return P2G_OK;
+}
+
 }  // namespace p2g
 
 // =====================================================================
@@ -384,14 +435,6 @@ const char *p2g_last_error(const p2g_ctx *ctx) {
     return ctx ? reinterpret_cast<const Ctx *>(ctx)->err.c_str() : "null context";
 }
 
-// GF(2) 64x64 matrices as 64 column images.
-static uint64_t xs_apply_host(const uint64_t *m, uint64_t v) {
-    uint64_t out = 0;
-    for (int b = 0; b < 64; b++)
-        if ((v >> b) & 1) out ^= m[b];
-    return out;
-}
-
 int p2g_dev_fill_xorshift64(p2g_ctx *ctx, uint8_t *dst, uint64_t nbytes, uint64_t seed,
                             void *stream) {
     return guarded(reinterpret_cast<Ctx *>(ctx), [&]() -> int {
@@ -401,43 +444,8 @@ int p2g_dev_fill_xorshift64(p2g_ctx *ctx, uint8_t *dst, uint64_t nbytes, uint64_
         if (!dst || reinterpret_cast<uintptr_t>(dst) % 16 != 0)
             return c->fail(P2G_ERR_INVALID_ARGUMENT, "destination must be 16-byte aligned");
         hipSetDevice(c->device);
-        if (!c->d_xs_jumps.ptr) {
-            // step^(W 2^b) for b = 0..31, W = kXorshiftLaneWords
-            std::vector<uint64_t> step(64), m(64), t(64), jumps(64 * 32);
-            for (int b = 0; b < 64; b++) {
-                uint64_t x = 1ull << b;
-                x ^= x << 13;
-                x ^= x >> 7;
-                x ^= x << 17;
-                step[b] = x;
-            }
-            m = step;  // step^(2^k), squared until 2^k = W
-            for (uint32_t w = 1; w < kXorshiftLaneWords; w *= 2) {
-                for (int b = 0; b < 64; b++) t[b] = xs_apply_host(m.data(), m[b]);
-                m = t;
-            }
-            for (int k = 0; k < 32; k++) {
-                std::copy(m.begin(), m.end(), jumps.begin() + 64 * k);
-                for (int b = 0; b < 64; b++) t[b] = xs_apply_host(m.data(), m[b]);
-                m = t;
-            }
-            // each jump as 8 byte tables: entry v of table q of jump k is the
-            // XOR of the columns 8 q + i of jumps[k] for the set bits i of v
-            std::vector<uint64_t> tables((size_t)32 * kXsJumpTableWords);
-            for (int k = 0; k < 32; k++)
-                for (int q = 0; q < 8; q++)
-                    for (int v = 0; v < 256; v++) {
-                        uint64_t x = 0;
-                        for (int i = 0; i < 8; i++)
-                            if ((v >> i) & 1) x ^= jumps[64 * k + 8 * q + i];
-                        tables[(size_t)k * kXsJumpTableWords + 256 * q + v] = x;
-                    }
-            if (c->d_xs_jumps.reserve(tables.size() * 8) != P2G_OK)
-                return c->fail(P2G_ERR_OUT_OF_MEMORY, "xorshift jump table");
-            hipError_t e = hipMemcpy(c->d_xs_jumps.ptr, tables.data(), tables.size() * 8,
-                                     hipMemcpyHostToDevice);
-            if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "upload xorshift jumps", e);
-        }
+        const int rc = xs_jump_tables(c);
+        if (rc != P2G_OK) return rc;
         hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c->stream;
         hipError_t e = launch_xorshift_fill(dst, nbytes, seed,
                                             static_cast<const uint64_t *>(c->d_xs_jumps.ptr), st);

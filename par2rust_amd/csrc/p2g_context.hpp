@@ -150,6 +150,8 @@ struct PlanItem {
 };
 int check_batch_job(Ctx *c, const p2g_batch_job &J, std::vector<SliceDesc> &descs);
 int build_plan(Ctx *c, const std::vector<PlanItem> &items, bool use_graph, BatchPlan *b);
+// Builds and uploads the synthetic-input jump tables (c->d_xs_jumps) once.
+int xs_jump_tables(Ctx *c);
 int launch_plan(BatchPlan *b, hipStream_t st);
 int wait_plan(BatchPlan *b);  // waits for the last launch, reads its flags and stamps
 void free_batch_plan(BatchPlan *b);

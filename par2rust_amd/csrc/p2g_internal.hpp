@@ -291,6 +291,19 @@ constexpr uint32_t kXorshiftLaneWords = 128;
 constexpr uint32_t kXsJumpTableWords = 8 * 256;
 hipError_t launch_xorshift_fill(uint8_t *d_dst, uint64_t nbytes, uint64_t seed,
                                 const uint64_t *d_jumps, hipStream_t stream);
+// Up to kXsFillFiles files in one launch, passed by value (kernel
+// arguments): file k is nbytes[k] bytes of the stream from seed[k] at dst[k]
+// (16-byte aligned), its lanes numbered from lane0[k]; lane0[n] is the total.
+constexpr uint32_t kXsFillFiles = 48;
+struct XsFillFiles {
+    uint8_t *dst[kXsFillFiles];
+    uint64_t nbytes[kXsFillFiles];
+    uint64_t seed[kXsFillFiles];
+    uint64_t lane0[kXsFillFiles + 1];
+    uint32_t n;
+};
+hipError_t launch_xorshift_fill_files(const XsFillFiles &f, const uint64_t *d_jumps,
+                                      hipStream_t stream);
 
 // Copies the nparts partial slices described by d_parts (ptr, valid, and in
 // `tail` the staging slot) into zero-padded S-byte slots of d_staging.

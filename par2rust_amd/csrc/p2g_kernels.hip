@@ -2659,12 +2659,10 @@ __device__ __forceinline__ uint64_t xs_apply(const uint64_t *__restrict__ tables
     return out;
 }
 
-__global__ __launch_bounds__(256) void k_xorshift_fill(uint8_t *__restrict__ dst, uint64_t nbytes,
-                                                       uint64_t seed,
-                                                       const uint64_t *__restrict__ jumps,
-                                                       uint32_t lanes) {
-    const uint32_t j = blockIdx.x * 256 + threadIdx.x;
-    if (j >= lanes) return;
+// Lane j of one file: words [j W, (j + 1) W) of the stream from `seed`.
+__device__ __forceinline__ void xs_fill_lane(uint8_t *__restrict__ dst, uint64_t nbytes,
+                                             uint64_t seed, const uint64_t *__restrict__ jumps,
+                                             uint32_t j) {
     uint64_t s = seed;
     for (int b = 0; b < 32; b++)
         if ((j >> b) & 1u) s = xs_apply(jumps + (size_t)kXsJumpTableWords * b, s);
@@ -2693,6 +2691,14 @@ __global__ __launch_bounds__(256) void k_xorshift_fill(uint8_t *__restrict__ dst
     }
 }
 
+__global__ __launch_bounds__(256) void k_xorshift_fill(uint8_t *__restrict__ dst, uint64_t nbytes,
+                                                       uint64_t seed,
+                                                       const uint64_t *__restrict__ jumps,
+                                                       uint32_t lanes) {
+    const uint32_t j = blockIdx.x * 256 + threadIdx.x;
+    if (j < lanes) xs_fill_lane(dst, nbytes, seed, jumps, j);
+}
+
 hipError_t launch_xorshift_fill(uint8_t *d_dst, uint64_t nbytes, uint64_t seed,
                                 const uint64_t *d_jumps, hipStream_t stream) {
     const uint64_t words = (nbytes + 7) / 8;
@@ -2701,6 +2707,31 @@ hipError_t launch_xorshift_fill(uint8_t *d_dst, uint64_t nbytes, uint64_t seed,
     if (lanes > 0xFFFFFFFFull) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_xorshift_fill, dim3((uint32_t)((lanes + 255) / 256)), dim3(256), 0, stream,
                        d_dst, nbytes, seed, d_jumps, (uint32_t)lanes);
+    return hipGetLastError();
+}
+
+// Several files in one launch: the files' lanes are numbered one after the
+// other (file f from lane0[f]), so short files fill side by side instead of
+// one small launch each, whose serial jump lookups set its time.
+__global__ __launch_bounds__(256) void k_xorshift_fill_files(XsFillFiles f,
+                                                             const uint64_t *__restrict__ jumps) {
+    const uint64_t g = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (g >= f.lane0[f.n]) return;
+    uint32_t lo = 0, hi = f.n;  // the file with lane0[k] <= g < lane0[k + 1]
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (f.lane0[mid] <= g) lo = mid;
+        else hi = mid;
+    }
+    xs_fill_lane(f.dst[lo], f.nbytes[lo], f.seed[lo], jumps, (uint32_t)(g - f.lane0[lo]));
+}
+
+hipError_t launch_xorshift_fill_files(const XsFillFiles &f, const uint64_t *d_jumps,
+                                      hipStream_t stream) {
+    const uint64_t lanes = f.lane0[f.n];
+    if (lanes == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_xorshift_fill_files, dim3((uint32_t)((lanes + 255) / 256)), dim3(256), 0,
+                       stream, f, d_jumps);
     return hipGetLastError();
 }
 

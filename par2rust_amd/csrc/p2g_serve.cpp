@@ -174,6 +174,17 @@ static uint64_t a256(uint64_t x) { return (x + 255) / 256 * 256; }
 // the memory it needs takes to come free.
 int Server::admit() {
     const double now = now_ms();
+    // the seeded files of the requests admitted now, filled by as few
+    // launches as hold them (one small launch per file had spent 7% of a
+    // serving run's device time in serial jump lookups)
+    XsFillFiles fills{};
+    auto flush = [&]() -> int {
+        if (fills.n == 0) return P2G_OK;
+        const hipError_t e = launch_xorshift_fill_files(
+            fills, static_cast<const uint64_t *>(c->d_xs_jumps.ptr), stream);
+        fills.n = 0;
+        return e == hipSuccess ? P2G_OK : c->fail(P2G_ERR_DEVICE, "xorshift fill kernel", e);
+    };
     for (auto it = queued.begin(); it != queued.end();) {
         Request *q = *it;
         uint64_t off = 0;
@@ -189,15 +200,25 @@ int Server::admit() {
         q->files.resize(q->lens.size());
         for (size_t f = 0; f < q->lens.size(); f++) {
             q->files[f] = {p, q->lens[f]};
-            hipError_t e = hipSuccess;
-            if (!q->host.empty() && q->host[f])
-                e = hipMemcpyAsync(p, q->host[f], q->lens[f], hipMemcpyHostToDevice, stream);
-            else if (q->lens[f])
-                e = p2g_dev_fill_xorshift64(reinterpret_cast<p2g_ctx *>(c), p, q->lens[f],
-                                            q->seeds.empty() ? 0 : q->seeds[f], stream) == P2G_OK
-                        ? hipSuccess
-                        : hipErrorUnknown;
-            if (e != hipSuccess) return c->fail(P2G_ERR_DEVICE, "request source upload", e);
+            if (!q->host.empty() && q->host[f]) {
+                const hipError_t e =
+                    hipMemcpyAsync(p, q->host[f], q->lens[f], hipMemcpyHostToDevice, stream);
+                if (e != hipSuccess) {
+                    flush();
+                    return c->fail(P2G_ERR_DEVICE, "request source upload", e);
+                }
+            } else if (q->lens[f]) {
+                if (fills.n == kXsFillFiles) {
+                    const int rc = flush();
+                    if (rc != P2G_OK) return rc;
+                }
+                const uint32_t k = fills.n++;
+                const uint64_t lanes = ((q->lens[f] + 7) / 8 + kXorshiftLaneWords - 1) / kXorshiftLaneWords;
+                fills.dst[k] = p;
+                fills.nbytes[k] = q->lens[f];
+                fills.seed[k] = q->seeds.empty() ? 0 : q->seeds[f];
+                fills.lane0[k + 1] = fills.lane0[k] + lanes;
+            }
             p += a256(q->lens[f]);
         }
         q->job.files = q->files.data();
@@ -218,7 +239,7 @@ int Server::admit() {
         q->t_admit = now_ms();
         active.push_back(q);
     }
-    return P2G_OK;
+    return flush();
 }
 
 int Server::launch_step(int k) {
@@ -384,6 +405,11 @@ int p2g_server_open(p2g_ctx *ctx, uint64_t hbm_budget, double step_ms, p2g_serve
         s->c = c;
         if (step_ms > 0) s->step_ms = step_ms;
         s->bypass_ms = 4 * s->step_ms;
+        // the fill tables of seeded requests, built before any step is queued
+        if (const int rc = xs_jump_tables(c); rc != P2G_OK) {
+            delete s;
+            return rc;
+        }
         s->pool.size = a256(hbm_budget);
         void *p = nullptr;
         hipError_t e = hipMalloc(&p, s->pool.size);

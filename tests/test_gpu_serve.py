@@ -182,6 +182,27 @@ def test_server_device_filled_requests(ctx):
     srv.close()
 
 
+def test_server_device_filled_many(ctx):
+    # many seeded requests admitted together: their 60 files are filled by
+    # shared launches (up to 48 files each), across request boundaries, with
+    # sizes from a few bytes to tens of KiB and not multiples of 8
+    srv = ctx.server(1 << 30)
+    want = {}
+    for k in range(30):
+        lens = [1 + (k * 7919) % 50000, 16 * k + 3]
+        S, R = 4096, 1 + k % 5
+        seeds = [100 + 2 * k, 101 + 2 * k]
+        rid = srv.submit(lens, S, R, bytes(16), seeds=seeds)
+        want[rid] = ([data(n, sd) for n, sd in zip(lens, seeds)], S, R)
+    done, t = set(), time.time()
+    while len(done) < len(want):
+        done.update(srv.poll(50.0))
+        assert time.time() - t < 60, "server made no progress"
+    for rid, (files, S, R) in want.items():
+        assert srv.download(rid) == alone(files, S, R, bytes(16)), rid
+    srv.close()
+
+
 def test_server_errors(ctx):
     import par2rust_amd as p2
 
