@@ -110,6 +110,23 @@ __host__ __device__ inline uint32_t next_band_width(uint32_t done, uint32_t rema
     return ramp < tail ? ramp : tail;
 }
 
+// Unit v of a band of w column groups x rows row blocks, in tiles of
+// kPassTileRows row blocks x tc column groups (the last tiles of each
+// direction ragged), tile after tile and row-block-major inside a tile:
+// its row block *y and column group *x within the band.
+constexpr uint32_t kPassTileRows = 4;
+__host__ __device__ inline void band_tile_unit(uint32_t v, uint32_t w, uint32_t rows, uint32_t tc,
+                                               uint32_t *y, uint32_t *x) {
+    const uint32_t rt = v / (kPassTileRows * w), rows_left = rows - rt * kPassTileRows;
+    const uint32_t rows_in = rows_left < kPassTileRows ? rows_left : kPassTileRows;
+    const uint32_t v1 = v - rt * kPassTileRows * w;
+    const uint32_t ct = v1 / (tc * rows_in), cols_left = w - ct * tc;
+    const uint32_t cols_in = cols_left < tc ? cols_left : tc;
+    const uint32_t v2 = v1 - ct * tc * rows_in;
+    *y = rt * kPassTileRows + v2 / cols_in;
+    *x = ct * tc + v2 % cols_in;
+}
+
 // Band widths of a batched piece: one band when W covers every column
 // group (a short row, whose packet hashes gain nothing from early bands),
 // else next_band_width.

@@ -74,6 +74,20 @@ int main() {
         CHECK(p.n_units == p.row_blocks * p.col_groups);
     }
     CHECK(planned > 1000);
+    // the fused pass's unit order inside a band: every (row block, column
+    // group) of the band exactly once, for any tile width
+    for (uint32_t rows = 1; rows <= 40; rows += 3)
+        for (uint32_t w = 1; w <= 70; w += 7)
+            for (uint32_t tc : {1u, 2u, 8u, 9u, 33u}) {
+                std::vector<uint8_t> seen(rows * w, 0);
+                for (uint32_t v = 0; v < rows * w; v++) {
+                    uint32_t y = 0, x = 0;
+                    band_tile_unit(v, w, rows, tc, &y, &x);
+                    CHECK(y < rows && x < w);
+                    if (y < rows && x < w) seen[y * w + x]++;
+                }
+                for (uint8_t k : seen) CHECK(k == 1);
+            }
     // batched groups
     for (int t = 0; t < 5000; t++) {
         BatchGroupWork w;
