@@ -1935,28 +1935,23 @@ struct PassArgs {
 // and a unit that costs more delays only the workgroup that took it; a
 // slice-hash workgroup whose hashes are done takes units too.  Queue
 // slots increase, so each workgroup's units come in increasing order (the
-// band walks rely on it).  Which slots of a round an XCD owns:
-//  * contiguous: [off_x, off_x + cnt_x) (off_x = encode workgroups of lower
-//    XCDs), so units adjacent in order (one row block's column groups,
-//    which share A fragments) run on one XCD and share its L2 copy of them;
-//    the batched pass, whose small-slice jobs stream large A fragments
-//    through few column groups;
-//  * strided: x, x + 8, x + 16, ..., the units workgroup index order deals
-//    to XCD x, which puts the row blocks of a column group that meet in one
-//    round (bands 64 column groups wide) on one XCD; the fused pass, whose
-//    source columns cost as much as its A fragments.
+// band walks rely on it).  XCD x owns the contiguous slots [off_x, off_x +
+// cnt_x) of a round (off_x = encode workgroups of lower XCDs), so units
+// adjacent in order run on one XCD and share its L2 copy of their operands:
+// one row block's column groups in the batched pass (whose small-slice jobs
+// stream large A fragments through few column groups), a tile of row blocks
+// and column groups in the fused pass.
 struct UnitQueue {
     uint32_t *ctr;
-    uint32_t e_blocks, cnt, off, step;
-    __device__ __forceinline__ UnitQueue(uint32_t *sync, uint32_t blk, uint32_t e, bool contiguous) {
+    uint32_t e_blocks, cnt, off;
+    __device__ __forceinline__ UnitQueue(uint32_t *sync, uint32_t blk, uint32_t e) {
         const uint32_t x = blk % kPassXcds;
         ctr = sync + kPassQueues + x;
         e_blocks = e;
         // 0 only on an XCD without encode workgroups (grids of fewer than 8),
         // whose hash workgroups then find no units
         cnt = e / kPassXcds + (x < e % kPassXcds ? 1u : 0u);
-        off = contiguous ? x * (e / kPassXcds) + min(x, e % kPassXcds) : x;
-        step = contiguous ? 1u : kPassXcds;
+        off = x * (e / kPassXcds) + min(x, e % kPassXcds);
     }
     // The next unit for the calling workgroup (all threads call it): thread
     // 0 takes a slot and broadcasts it through `slot`; readfirstlane makes the
@@ -1966,7 +1961,7 @@ struct UnitQueue {
         if (cnt == 0) return 0xFFFFFFFFu;
         if (threadIdx.x == 0) {
             const uint32_t k = __hip_atomic_fetch_add(ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            *slot = (k / cnt) * e_blocks + off + step * (k % cnt);
+            *slot = (k / cnt) * e_blocks + off + k % cnt;
         }
         __syncthreads();
         const uint32_t u = __builtin_amdgcn_readfirstlane(*(volatile uint32_t *)slot);
@@ -2101,7 +2096,7 @@ __global__ __launch_bounds__(kMfThreads, 1) void k_create_pass(PassArgs a) {
     // band (the queues hand out units in increasing order).
     __shared__ uint32_t s_unit;
     const uint32_t e_blocks = gridDim.x - a.h3_blocks - a.h2_blocks;
-    const UnitQueue queue(a.sync, blockIdx.x, e_blocks, true);
+    const UnitQueue queue(a.sync, blockIdx.x, e_blocks);
     // Inside a band, units go tile by tile: tiles of kPassTileRows row blocks
     // x tc column groups, about one XCD's run of queue slots per round, so
     // the units an XCD runs at once share their A fragments (per row block)
@@ -2482,7 +2477,7 @@ __global__ __launch_bounds__(kMfThreads, 1) void k_create_batch(BatchArgs a) {
     // scalar (the unit from the queue, the tables), so the barriers inside
     // are reached by the whole workgroup.
     __shared__ uint32_t s_unit;
-    const UnitQueue queue(a.sync, blk, e_blocks, true);
+    const UnitQueue queue(a.sync, blk, e_blocks);
     uint32_t pi = 0;
     const SliceDesc *loaded = nullptr;  // first descriptor of the chunk in lds.desc
     uint32_t band = 0, b_lo = 0, b_hi = 0, unit_end = 0;
