@@ -41,6 +41,14 @@ void BatchPlan::drop_graph() {
     graph = nullptr;
 }
 
+void BatchPlan::free_retired() {
+    for (void *p : retired_dev) hipFree(p);
+    for (void *p : retired_host) hipHostFree(p);
+    retired_dev.clear();
+    retired_host.clear();
+    retired_bytes = 0;
+}
+
 BatchPlan::~BatchPlan() {
     if (c) hipSetDevice(c->device);
     if (done) hipEventSynchronize(done);
@@ -50,6 +58,7 @@ BatchPlan::~BatchPlan() {
     for (DevBuf *b : {&d_descs, &d_tb, &d_staging, &d_parts, &d_pieces, &d_sjobs, &d_sync, &d_done,
                       &d_err, &own_afrag})
         b->release();
+    free_retired();
 }
 
 int BatchPlan::record(hipStream_t st) {
@@ -83,14 +92,44 @@ static void packet_prefix_words(const uint8_t set_id[16], uint32_t out[8]) {
 
 static uint64_t align_up(uint64_t x, uint64_t a) { return (x + a - 1) / a * a; }
 
-// Buffers of a plan grow with room to spare (half again, at most 1 GiB more):
-// a plan rebuilt for steps of varying size (the serving loop) then rarely
-// reallocates, and a reallocation waits for the whole device.
-static int grow(DevBuf &d, size_t n) {
-    return n <= d.cap ? P2G_OK : d.reserve(n + std::min<size_t>(n / 2, 1ull << 30));
+// Buffers of a plan grow with room to spare (half again, at most 1 GiB more),
+// so a plan rebuilt for steps of varying size (the serving loop) rarely
+// reallocates.  Freeing the old buffer waits for the whole device, unless
+// the plan keeps it for later (keep_retired), up to kRetiredMax bytes kept.
+constexpr uint64_t kRetiredMax = 4ull << 30;
+
+// Whether the old buffer (cap bytes) may wait in the plan; if not, what the
+// plan kept is freed now, after the device is idle.
+static bool keep_old(BatchPlan *B, size_t cap) {
+    if (!B->keep_retired) return false;
+    if (B->retired_bytes + cap <= kRetiredMax) return true;
+    hipDeviceSynchronize();
+    B->free_retired();
+    return false;
 }
-static int grow(PinBuf &h, size_t n) {
-    return n <= h.cap ? P2G_OK : h.reserve(n + std::min<size_t>(n / 2, 1ull << 30));
+static int grow(BatchPlan *B, DevBuf &d, size_t n) {
+    if (n <= d.cap) return P2G_OK;
+    const size_t want = n + std::min<size_t>(n / 2, 1ull << 30);
+    if (!d.ptr || !keep_old(B, d.cap)) return d.reserve(want);
+    void *p = nullptr;
+    if (hipMalloc(&p, want) != hipSuccess) return P2G_ERR_OUT_OF_MEMORY;
+    B->retired_dev.push_back(d.ptr);
+    B->retired_bytes += d.cap;
+    d.ptr = p;
+    d.cap = want;
+    return P2G_OK;
+}
+static int grow(BatchPlan *B, PinBuf &h, size_t n) {
+    if (n <= h.cap) return P2G_OK;
+    const size_t want = n + std::min<size_t>(n / 2, 1ull << 30);
+    if (!h.ptr || !keep_old(B, h.cap)) return h.reserve(want);
+    void *p = nullptr;
+    if (hipHostMalloc(&p, want, hipHostMallocDefault) != hipSuccess) return P2G_ERR_OUT_OF_MEMORY;
+    B->retired_host.push_back(h.ptr);
+    B->retired_bytes += h.cap;
+    h.ptr = p;
+    h.cap = want;
+    return P2G_OK;
 }
 
 // A table for the next launch to upload: the device buffer is sized now,
@@ -98,7 +137,7 @@ static int grow(PinBuf &h, size_t n) {
 template <typename T>
 static int stage(Ctx *c, BatchPlan *B, DevBuf &d, const std::vector<T> &v, size_t off,
                  const char *what) {
-    if (grow(d, std::max<size_t>(v.size() * sizeof(T), 16)) != P2G_OK)
+    if (grow(B, d, std::max<size_t>(v.size() * sizeof(T), 16)) != P2G_OK)
         return c->fail(P2G_ERR_OUT_OF_MEMORY, what);
     if (v.empty()) return P2G_OK;
     memcpy(static_cast<uint8_t *>(B->h_stage.ptr) + off, v.data(), v.size() * sizeof(T));
@@ -202,9 +241,9 @@ int build_plan(Ctx *c, const std::vector<PlanItem> &items, bool use_graph, Batch
         n_slices += jd[j].size();
     }
     if (n_slices > 0x7FFFFFFFull) return c->fail(P2G_ERR_INVALID_ARGUMENT, "too many slices");
-    if (grow(B->d_staging, std::max<uint64_t>(stage_bytes, 256)) != P2G_OK ||
-        grow(B->d_descs, n_slices * sizeof(SliceDesc)) != P2G_OK ||
-        grow(B->d_tb, n_slices * 64) != P2G_OK)
+    if (grow(B, B->d_staging, std::max<uint64_t>(stage_bytes, 256)) != P2G_OK ||
+        grow(B, B->d_descs, n_slices * sizeof(SliceDesc)) != P2G_OK ||
+        grow(B, B->d_tb, n_slices * 64) != P2G_OK)
         return c->fail(P2G_ERR_OUT_OF_MEMORY, "batch descriptor tables");
     uint8_t *staging = static_cast<uint8_t *>(B->d_staging.ptr);
     const SliceDesc *d_descs = static_cast<const SliceDesc *>(B->d_descs.ptr);
@@ -243,10 +282,10 @@ int build_plan(Ctx *c, const std::vector<PlanItem> &items, bool use_graph, Batch
 
     // pieces and groups
     const uint32_t band_default = c->band_cols ? c->band_cols : 64u;
-    uint64_t budget = kBatchAfragMin;
+    uint64_t budget = B->afrag_budget ? B->afrag_budget : kBatchAfragMin;
     if (const char *v = getenv("P2G_BATCH_AFRAG_BYTES")) {
         budget = std::max<uint64_t>(1, strtoull(v, nullptr, 10));
-    } else {
+    } else if (!B->afrag_budget) {
         size_t free_b = 0, total_b = 0;
         if (hipMemGetInfo(&free_b, &total_b) == hipSuccess)
             budget = std::min(kBatchAfragMax, std::max<uint64_t>(kBatchAfragMin, free_b / 4));
@@ -322,7 +361,7 @@ int build_plan(Ctx *c, const std::vector<PlanItem> &items, bool use_graph, Batch
         spans[k].s_hi = (uint32_t)(std::upper_bound(job_group.begin(), job_group.end(), k) - job_group.begin());
     }
 
-    if (grow(*B->afrag, std::max<uint64_t>(afrag_max, 256)) != P2G_OK)
+    if (grow(B, *B->afrag, std::max<uint64_t>(afrag_max, 256)) != P2G_OK)
         return c->fail(P2G_ERR_OUT_OF_MEMORY, "batch A fragments");
     uint8_t *afrag_base = static_cast<uint8_t *>(B->afrag->ptr);
 
@@ -382,7 +421,7 @@ int build_plan(Ctx *c, const std::vector<PlanItem> &items, bool use_graph, Batch
     const size_t o_sjobs = align_up(o_pieces + pieces.size() * sizeof(BatchPiece), 256);
     const size_t stage_total = o_sjobs + sjobs.size() * sizeof(BatchSlices);
     B->pending.clear();
-    if (grow(B->h_stage, std::max<size_t>(stage_total, 256)) != P2G_OK)
+    if (grow(B, B->h_stage, std::max<size_t>(stage_total, 256)) != P2G_OK)
         return c->fail(P2G_ERR_OUT_OF_MEMORY, "batch table staging");
     int rc = stage(c, B, B->d_descs, all, o_descs, "batch descriptors");
     if (rc == P2G_OK) rc = stage(c, B, B->d_parts, parts, o_parts, "batch staging table");
@@ -390,9 +429,9 @@ int build_plan(Ctx *c, const std::vector<PlanItem> &items, bool use_graph, Batch
     if (rc == P2G_OK) rc = stage(c, B, B->d_sjobs, sjobs, o_sjobs, "batch slice jobs");
     if (rc != P2G_OK) return rc;
     const bool new_err = B->d_err.ptr == nullptr;
-    if (grow(B->d_sync, (size_t)sync_words_total * 4) != P2G_OK ||
-        grow(B->d_done, (size_t)(nitems + 1) * 8) != P2G_OK || B->d_err.reserve(256) != P2G_OK ||
-        grow(B->h_back, 8 + (size_t)(nitems + 1) * 8 + spans.size() * 8 * sizeof(unsigned long long)) != P2G_OK)
+    if (grow(B, B->d_sync, (size_t)sync_words_total * 4) != P2G_OK ||
+        grow(B, B->d_done, (size_t)(nitems + 1) * 8) != P2G_OK || B->d_err.reserve(256) != P2G_OK ||
+        grow(B, B->h_back, 8 + (size_t)(nitems + 1) * 8 + spans.size() * 8 * sizeof(unsigned long long)) != P2G_OK)
         return c->fail(P2G_ERR_OUT_OF_MEMORY, "batch sync words");
     if (new_err) {
         hipError_t ez = hipMemset(B->d_err.ptr, 0, 256);

@@ -44,9 +44,21 @@ struct BatchPlan {
     hipEvent_t done = nullptr;
     bool launched = false;
     std::vector<unsigned long long> h_done;  // completion stamps of the last launch, then start
+    // A fragments of one group, in bytes; 0: a quarter of the free device
+    // memory within [16, 64] GiB (P2G_BATCH_AFRAG_BYTES overrides either).
+    uint64_t afrag_budget = 0;
+    // Freeing device or pinned memory waits for every kernel on the device.
+    // A plan rebuilt while another plan's launch runs (the serving steps)
+    // sets keep_retired: a table that outgrows its buffer then moves to a new
+    // one and the old one waits here, freed by free_retired (when the caller
+    // knows the device idle) or with the plan.
+    bool keep_retired = false;
+    std::vector<void *> retired_dev, retired_host;
+    uint64_t retired_bytes = 0;
 
     int record(hipStream_t st);
     void drop_graph();
+    void free_retired();
     ~BatchPlan();
 };
 

@@ -40,6 +40,11 @@ namespace p2g {
 
 constexpr uint32_t kServeSliceLanes = 24 * kBatchSliceLanes;  // slice-hash lanes of a batched launch
 constexpr uint32_t kServeMinCols = 32;  // column groups (16 KiB) of the smallest column piece
+// A fragments of a step: one buffer for both steps' plans, allocated at open
+// (the first allocation of this size after other large frees has taken ~2 s,
+// which inside the loop held every request that arrived meanwhile); a step
+// whose fragments exceed it runs in more than one launch group.
+constexpr uint64_t kServeAfragBytes = 16ull << 30;
 
 // Estimated device time of one recovery row of a job and of one slice's
 // hashes, from the context's measured rates (CostRates): a row block (16
@@ -372,6 +377,11 @@ int Server::retire(int k) {
         }
     }
     st.items.clear();
+    // buffers the plans outgrew go back while no step is queued (a free
+    // waits for the whole device)
+    if (!steps[0].busy && !steps[1].busy)
+        for (Step &s : steps)
+            if (!s.plan.retired_dev.empty() || !s.plan.retired_host.empty()) s.plan.free_retired();
     return rc;
 }
 
@@ -424,7 +434,20 @@ int p2g_server_open(p2g_ctx *ctx, uint64_t hbm_budget, double step_ms, p2g_serve
             delete s;
             return c->fail(P2G_ERR_DEVICE, "server stream", e);
         }
-        for (Step &st : s->steps) st.plan.afrag = &s->afrag;
+        // the steps' A fragments, smaller when the device is short of memory
+        uint64_t afrag_bytes = kServeAfragBytes;
+        size_t free_b = 0, total_b = 0;
+        if (hipMemGetInfo(&free_b, &total_b) == hipSuccess)
+            afrag_bytes = std::max<uint64_t>(256ull << 20, std::min<uint64_t>(afrag_bytes, free_b / 2));
+        if (s->afrag.reserve(afrag_bytes) != P2G_OK) {
+            delete s;
+            return c->fail(P2G_ERR_OUT_OF_MEMORY, "server fragment buffer");
+        }
+        for (Step &st : s->steps) {
+            st.plan.afrag = &s->afrag;
+            st.plan.afrag_budget = afrag_bytes;
+            st.plan.keep_retired = true;
+        }
         *out = reinterpret_cast<p2g_server *>(s);
         return P2G_OK;
     });
