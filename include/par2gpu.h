@@ -276,7 +276,8 @@ typedef struct {
 } p2g_request;
 
 /* hbm_budget: bytes of device memory for admitted requests (allocated at
- * open); step_ms: target device time of one step (0: 35 ms). */
+ * open, with the steps' 16 GiB A-fragment buffer, or half the free memory
+ * if that is less); step_ms: target device time of one step (0: 35 ms). */
 int p2g_server_open(p2g_ctx *ctx, uint64_t hbm_budget, double step_ms, p2g_server **out);
 /* How long a queued request that finds no memory may be overtaken by later
  * requests that fit, from when it first found none (default: 4 step
