@@ -920,6 +920,7 @@ constexpr int kWaitVmcnt8 = 0x0F78;
 constexpr int kWaitVmcnt12 = 0x0F7C;
 constexpr int kWaitVmcnt8Lds = 0x0078;  // vmcnt(8) lgkmcnt(0)
 constexpr int kWaitVmcnt0 = 0x0070;  // vmcnt(0) lgkmcnt(0)
+constexpr int kWaitLgkm0 = 0xC07F;   // lgkmcnt(0) alone (vmcnt 63, expcnt 7)
 
 // Addresses of the two source words of K step g of this lane (whole slices).
 __device__ __forceinline__ void mf_word_addrs(const uint4 *s_desc, uint32_t g, uint32_t h,
@@ -1259,7 +1260,9 @@ hipError_t launch_encode_mfma(const SliceDesc *d_inputs, uint32_t n_inputs, cons
 // MD5 (RFC 1321)
 // ======================================================================
 
-__constant__ uint32_t kMd5K[64] = {
+// constexpr (not __constant__): indexed by unrolled round numbers, the
+// constants become instruction literals instead of 64 scalar registers.
+constexpr uint32_t kMd5K[64] = {
     0xd76aa478u, 0xe8c7b756u, 0x242070dbu, 0xc1bdceeeu, 0xf57c0fafu, 0x4787c62au, 0xa8304613u, 0xfd469501u,
     0x698098d8u, 0x8b44f7afu, 0xffff5bb1u, 0x895cd7beu, 0x6b901122u, 0xfd987193u, 0xa679438eu, 0x49b40821u,
     0xf61e2562u, 0xc040b340u, 0x265e5a51u, 0xe9b6c7aau, 0xd62f105du, 0x02441453u, 0xd8a1e681u, 0xe7d3fbc8u,
@@ -1292,6 +1295,33 @@ __device__ __forceinline__ void md5_compress(uint32_t h[4], const uint32_t x[16]
         d = c;
         c = b;
         b = b + rotl32(a + f + (x[g] + kMd5K[i]), sh[(i >> 4) * 4 + (i & 3)]);
+        a = t;
+    }
+    h[0] += a; h[1] += b; h[2] += c; h[3] += d;
+}
+
+// The message schedule of MD5: round i reads word md5_word(i) of the block.
+__host__ __device__ constexpr int md5_word(int i) {
+    return i < 16 ? i : i < 32 ? (5 * i + 1) & 15 : i < 48 ? (3 * i + 5) & 15 : (7 * i) & 15;
+}
+
+// One block from its 64 round inputs w[i] = x[md5_word(i)] + K[i], made
+// beforehand by another wave: 4 VALU per round (v_bitop3_b32, v_add3_u32,
+// v_alignbit_b32, v_add_u32) instead of md5_compress's 5, and no loads.
+__device__ __forceinline__ void md5_compress_w(uint32_t h[4], const uint32_t w[64]) {
+    constexpr int sh[16] = {7, 12, 17, 22, 5, 9, 14, 20, 4, 11, 16, 23, 6, 10, 15, 21};
+    uint32_t a = h[0], b = h[1], c = h[2], d = h[3];
+#pragma unroll
+    for (int i = 0; i < 64; i++) {
+        uint32_t f;
+        if (i < 16) f = __builtin_amdgcn_bitop3_b32(b, c, d, 0xCA);
+        else if (i < 32) f = __builtin_amdgcn_bitop3_b32(b, c, d, 0xE4);
+        else if (i < 48) f = __builtin_amdgcn_bitop3_b32(b, c, d, 0x96);
+        else f = __builtin_amdgcn_bitop3_b32(b, c, d, 0x39);
+        const uint32_t t = d;
+        d = c;
+        c = b;
+        b = b + rotl32(a + f + w[i], sh[(i >> 4) * 4 + (i & 3)]);
         a = t;
     }
     h[0] += a; h[1] += b; h[2] += c; h[3] += d;
@@ -1802,13 +1832,12 @@ struct PacketPrefix {
 // The packet hash covers set_id || type || le32(exp) || S data bytes:
 // len = 36 + S.  Block 0 holds the 36-byte prefix and data bytes 0..27;
 // block k >= 1 holds data bytes 64 k - 36 .. 64 k + 27; blocks k >= nfull =
-// len / 64 carry the padding.  md5_packet_block compresses any block k from
+// len / 64 carry the padding.  md5_packet_words assembles any block k from
 // the prefix, the data and the padding.
-__device__ __forceinline__ void md5_packet_block(uint32_t h[4], const PacketPrefix &pre,
+__device__ __forceinline__ void md5_packet_words(uint32_t x[16], const PacketPrefix &pre,
                                                  uint32_t exp, const uint8_t *data, uint64_t len,
                                                  uint32_t k, uint32_t nb) {
     const uint64_t base = (uint64_t)k * 64u;
-    uint32_t x[16];
 #pragma unroll
     for (int w = 0; w < 16; w++) {
         const uint64_t o = base + 4u * w;
@@ -1819,6 +1848,13 @@ __device__ __forceinline__ void md5_packet_block(uint32_t h[4], const PacketPref
         else v = md5_pad_word(o, len, w, k, nb);
         x[w] = v;
     }
+}
+
+__device__ __forceinline__ void md5_packet_block(uint32_t h[4], const PacketPrefix &pre,
+                                                 uint32_t exp, const uint8_t *data, uint64_t len,
+                                                 uint32_t k, uint32_t nb) {
+    uint32_t x[16];
+    md5_packet_words(x, pre, exp, data, len, k, nb);
     md5_compress(h, x);
 }
 
@@ -1826,9 +1862,11 @@ __device__ __forceinline__ void md5_packet_block(uint32_t h[4], const PacketPref
 // 16-byte aligned and S % 16 == 0.  Block k is data words 16k - 9 .. 16k + 6:
 // the last word of the previous block's aligned 64 bytes, carried, and the
 // first 15 words of data[64k - 32, 64k + 32), loaded as four 16-byte loads
-// (which end at or before byte S, as S is a multiple of 16).
-__device__ __forceinline__ void md5_packet_bulk16(uint32_t h[4], const uint8_t *data, uint32_t k0,
-                                                  uint32_t nblk) {
+// (which end at or before byte S, as S is a multiple of 16).  step(x) takes
+// each block's 16 words in order.
+template <typename Step>
+__device__ __forceinline__ void md5_packet_bulk16_steps(const uint8_t *data, uint32_t k0,
+                                                        uint32_t nblk, Step &&step) {
     if (nblk == 0) return;
     uint32_t carry = gload(reinterpret_cast<const uint32_t *>(data + 4 * (16 * (size_t)k0 - 9)));
     md5_stream_steps(reinterpret_cast<const uint4 *>(data + 64 * (size_t)k0 - 32), nblk,
@@ -1838,8 +1876,13 @@ __device__ __forceinline__ void md5_packet_bulk16(uint32_t h[4], const uint8_t *
 #pragma unroll
                          for (int w = 1; w < 16; w++) x[w] = y[w - 1];
                          carry = y[15];
-                         md5_compress(h, x);
+                         step(x);
                      });
+}
+
+__device__ __forceinline__ void md5_packet_bulk16(uint32_t h[4], const uint8_t *data, uint32_t k0,
+                                                  uint32_t nblk) {
+    md5_packet_bulk16_steps(data, k0, nblk, [&](const uint32_t *x) { md5_compress(h, x); });
 }
 
 __global__ __launch_bounds__(64) void k_recovery_md5(const uint8_t *__restrict__ rec,
@@ -1996,38 +2039,217 @@ __device__ __forceinline__ bool pass_wait(const uint32_t *cnt, uint32_t target, 
     return true;
 }
 
-// Packet r (exponent r): the blocks of each band as it completes, then the
-// padding.  Blocks [1, bulk_end) are whole data blocks (md5_bulk).
-__device__ __forceinline__ void pass_packet_lane(const PassArgs &a, uint32_t r) {
+// Packet r (exponent r): the 16 message words of every block in order, each
+// band's blocks as the band completes, then the padding, to emit(x).  Blocks
+// [1, bulk_end) are whole data blocks (streamed).  False when a band wait
+// gave up; stops early once `ok` is false.
+template <typename Emit>
+__device__ __forceinline__ bool pass_packet_walk(const PassArgs &a, uint32_t r, const bool &ok,
+                                                 Emit &&emit) {
     const uint8_t *data = a.rec + (size_t)r * a.S;
     const bool a16 = a.S % 16 == 0 && reinterpret_cast<uintptr_t>(a.rec) % 16 == 0;
     const uint64_t len = 36ull + a.S;
     const uint32_t nb = md5_blocks(len), nfull = (uint32_t)(len / 64);
     const uint32_t bulk_end = nfull > 1 ? nfull : 1;
-    uint32_t h[4];
-    md5_init(h);
-    uint32_t k = 0;  // next block to compress
+    uint32_t k = 0;  // next block
     const uint32_t *bstart = a.sync + kPassBands + a.n_bands;
-    for (uint32_t b = 0; b < a.n_bands; b++) {
+    for (uint32_t b = 0; b < a.n_bands && ok; b++) {
         const uint32_t c1 = bstart[b + 1], w = c1 - bstart[b];
         if (!pass_wait(a.sync + kPassBands + b, w * a.row_blocks, a.stall_ticks, a.err))
-            return;
+            return false;
         // data bytes [0, e) of every block are final; block k needs up to 64 k + 28
         const uint64_t e = min((uint64_t)c1 * (kMfThreads * 2), (uint64_t)a.S);
         const uint32_t avail = b + 1 == a.n_bands ? nb : e >= 28 ? (uint32_t)((e - 28) / 64) + 1 : 0;
         if (k == 0 && avail > 0) {
-            md5_packet_block(h, a.pre, r, data, len, 0, nb);
+            uint32_t x[16];
+            md5_packet_words(x, a.pre, r, data, len, 0, nb);
+            emit(x);
             k = 1;
         }
         const uint32_t stop = min(avail, bulk_end);
         if (stop > k) {
-            if (a16) md5_packet_bulk16(h, data, k, stop - k);
-            else md5_bulk<uint32_t>(h, data + 64 * (size_t)k - 36, stop - k);
+            if (a16) md5_packet_bulk16_steps(data, k, stop - k, emit);
+            else md5_stream_steps(reinterpret_cast<const uint32_t *>(data + 64 * (size_t)k - 36),
+                                  stop - k, emit);
             k = stop;
         }
     }
-    for (; k < nb; k++) md5_packet_block(h, a.pre, r, data, len, k, nb);
-    store_digest(a.pkt_md5 + (size_t)r * 16, h);
+    for (; k < nb && ok; k++) {
+        uint32_t x[16];
+        md5_packet_words(x, a.pre, r, data, len, k, nb);
+        emit(x);
+    }
+    return true;
+}
+
+// A packet workgroup hashes 64 packets with two waves.  The feeder (wave 1)
+// walks the bands and the data (pass_packet_walk) and writes each block's 64
+// round inputs x[md5_word(i)] + K[i] to a ring of kPktRing blocks in LDS; the
+// chain (wave 0) runs the rounds from them (md5_compress_w).  The chain is
+// the pass's serial path and a lone wave issues one instruction per ~4
+// cycles whatever its kind, so moving the loads and the message additions to
+// the other wave cuts its instructions per block from ~331 to ~282.  Two
+// counters in LDS pace the waves: prod (blocks written) and cons (blocks
+// whose inputs the chain has read), both counted over all packets of the
+// workgroup; the feeder writes block j once cons > j - kPktRing, the chain
+// reads it once prod > j.  A wave that gives up (a band wait that gave up,
+// or kPktAbort from the other wave, or no movement for the stall limit)
+// stores kPktAbort in its own counter, so the other's waits end too.
+constexpr uint32_t kPktRing = 4;  // blocks of 64 lanes x 64 round inputs (16 KiB each)
+constexpr uint32_t kPktAbort = 0xFFFFFFFFu;
+// word offsets in the flag area: the two counters, then one word per lane so
+// that a counter is stored by every lane of a wave in one conflict-free
+// store (lane 0 to the counter, the others to their own word)
+constexpr uint32_t kPktProd = 0, kPktCons = 1, kPktLaneWords = 64;
+
+// The counters are read and written as relaxed workgroup-scope atomics (LDS
+// reads and writes; volatile accesses through these pointers would be flat
+// memory operations); the ring's data is ordered around them by explicit
+// waits.
+__device__ __forceinline__ uint32_t pkt_load(uint32_t *f) {
+    return __hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ void pkt_store(uint32_t *f, uint32_t v) {
+    __hip_atomic_store(f, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// Waits until the counter f >= target; false when the other wave gave up or
+// f has not moved for `ticks` of the 100 MHz real-time counter.
+__device__ __forceinline__ bool pkt_wait(uint32_t *f, uint32_t target, uint32_t ticks) {
+    uint32_t seen = __builtin_amdgcn_readfirstlane(pkt_load(f));
+    if (seen >= target) return seen != kPktAbort;
+    uint64_t since = __builtin_amdgcn_s_memrealtime();
+    for (;;) {
+        __builtin_amdgcn_s_sleep(1);
+        const uint32_t now = __builtin_amdgcn_readfirstlane(pkt_load(f));
+        if (now >= target) return now != kPktAbort;
+        const uint64_t t = __builtin_amdgcn_s_memrealtime();
+        if (now != seen) {
+            seen = now;
+            since = t;
+        } else if (t - since > ticks) {
+            return false;
+        }
+    }
+}
+
+__device__ __forceinline__ uint4 *pkt_slot(uint4 *ring, uint32_t j) {
+    return ring + (size_t)(j % kPktRing) * 16 * 64 + (threadIdx.x & 63);
+}
+
+// Feeder lane of packet r: blocks [J, J + nb) of the ring.
+__device__ __forceinline__ bool pkt_feed(const PassArgs &a, uint32_t r, uint4 *ring,
+                                         uint32_t *flags, uint32_t *own, uint32_t J,
+                                         uint32_t ticks) {
+    uint32_t j = J;
+    bool ok = true;
+    const bool walked = pass_packet_walk(a, r, ok, [&](const uint32_t *x) {
+        if (!ok) return;
+        if (j >= kPktRing && !pkt_wait(flags + kPktCons, j + 1 - kPktRing, ticks)) {
+            ok = false;
+            return;
+        }
+        uint4 *slot = pkt_slot(ring, j);
+#pragma unroll
+        for (int q = 0; q < 16; q++)
+            slot[q * 64] = make_uint4(x[md5_word(4 * q)] + kMd5K[4 * q],
+                                      x[md5_word(4 * q + 1)] + kMd5K[4 * q + 1],
+                                      x[md5_word(4 * q + 2)] + kMd5K[4 * q + 2],
+                                      x[md5_word(4 * q + 3)] + kMd5K[4 * q + 3]);
+        // the block's inputs are in LDS before prod says so
+        asm volatile("" ::: "memory");
+        __builtin_amdgcn_s_waitcnt(kWaitLgkm0);
+        asm volatile("" ::: "memory");
+        pkt_store(own, j + 1);
+        j++;
+    });
+    if (walked && ok) return true;
+    if (walked) atomicOr(a.err, kPassErrWait);  // a band wait that gave up flagged it already
+    pkt_store(own, kPktAbort);
+    return false;
+}
+
+// Chain lane: the digest of blocks [J, J + nb) of the ring, stored when
+// `store`.  Block j first reads the inputs of block j + 1 (into the other of
+// two register sets) and then runs its own rounds, so the reads land while
+// the rounds run; whether block j + 1 is there is judged from prod as read
+// one block earlier (the feeder, up to kPktRing blocks ahead, has had a
+// block's time to write it), so that read does not stall either.
+__device__ __forceinline__ bool pkt_chain(const PassArgs &a, uint8_t *digest, bool store,
+                                          const uint4 *ring, uint32_t *flags, uint32_t *own,
+                                          uint32_t J, uint32_t nb, uint32_t ticks) {
+    static_assert((kPktRing & (kPktRing - 1)) == 0, "ring slots by mask");
+    const uint4 *lane_ring = ring + (threadIdx.x & 63);
+    auto read = [&](uint32_t *w, uint32_t j) {
+        const uint4 *src = lane_ring + (size_t)(j & (kPktRing - 1)) * 16 * 64;
+#pragma unroll
+        for (int q = 0; q < 16; q++) {
+            const uint4 v = src[q * 64];
+            w[4 * q] = v.x;
+            w[4 * q + 1] = v.y;
+            w[4 * q + 2] = v.z;
+            w[4 * q + 3] = v.w;
+        }
+    };
+    uint32_t h[4];
+    md5_init(h);
+    uint32_t wa[64], wb[64];
+    const uint32_t end = J + nb;
+    uint32_t j = J, p;  // p: prod as read in the previous block
+    // Block j from cur when block j + 1 follows.  `seen` is taken first on
+    // every such block: its wait also covers the reads of cur, made before
+    // p's, so the rounds need no waits of their own.  seen - (j + 2) >=
+    // ~(j + 2) (unsigned) holds exactly when seen < j + 2 or seen is
+    // kPktAbort.  The store of cons = j + 1 depends on the rounds' result,
+    // so it cannot move ahead of them (block j's inputs are then used).
+#define PKT_STEP(CUR, NXT)                                                                    \
+    do {                                                                                      \
+        const uint32_t seen = __builtin_amdgcn_readfirstlane(p);                              \
+        if (seen - (j + 2) >= ~(j + 2) && !pkt_wait(flags + kPktProd, j + 2, ticks))          \
+            goto failed;                                                                      \
+        asm volatile("" ::: "memory");                                                        \
+        read(NXT, j + 1);                                                                     \
+        p = pkt_load(flags + kPktProd);                                                       \
+        md5_compress_w(h, CUR);                                                               \
+        uint32_t c = j + 1;                                                                   \
+        asm volatile("" : "+v"(c) : "v"(h[1]));                                               \
+        pkt_store(own, c);                                                                    \
+        j++;                                                                                  \
+    } while (0)
+    // the last block: no successor to read
+#define PKT_LAST(CUR)                                                                         \
+    do {                                                                                      \
+        md5_compress_w(h, CUR);                                                               \
+        uint32_t c = j + 1;                                                                   \
+        asm volatile("" : "+v"(c) : "v"(h[1]));                                               \
+        pkt_store(own, c);                                                                    \
+    } while (0)
+    if (!pkt_wait(flags + kPktProd, J + 1, ticks)) goto failed;
+    asm volatile("" ::: "memory");
+    read(wa, J);
+    p = pkt_load(flags + kPktProd);
+    for (;;) {
+        if (j + 1 == end) {
+            PKT_LAST(wa);
+            break;
+        }
+        PKT_STEP(wa, wb);
+        if (j + 1 == end) {
+            PKT_LAST(wb);
+            break;
+        }
+        PKT_STEP(wb, wa);
+    }
+#undef PKT_STEP
+#undef PKT_LAST
+    if (store) store_digest(digest, h);
+    return true;
+failed:
+    // a feeder that gave up flagged the error itself; a stall here flags it
+    if (__builtin_amdgcn_readfirstlane(pkt_load(flags + kPktProd)) != kPktAbort)
+        atomicOr(a.err, kPassErrWait);
+    pkt_store(own, kPktAbort);
+    return false;
 }
 
 // Real-time stamps of the pass (one thread per workgroup calls these): the
@@ -2044,7 +2266,13 @@ __device__ __forceinline__ void pass_stamp_end(const PassArgs &a, int role) {
 
 template <bool FULL>
 __global__ __launch_bounds__(kMfThreads, 1) void k_create_pass(PassArgs a) {
-    __shared__ MfLds lds;
+    // the encode's descriptors and A buffers (and the slice role's CRC
+    // tables); the packet role's ring of round inputs shares the space
+    __shared__ union {
+        MfLds mf;
+        uint4 pkt[kPktRing * 16 * 64];
+    } lds_u;
+    MfLds &lds = lds_u.mf;
     // the A ring of mf_unit_dma: four separate objects (see there)
     __shared__ uint4 s_a0[kMfRing], s_a1[kMfRing], s_a2[kMfRing], s_a3[kMfRing];
     const uint32_t tid = threadIdx.x;
@@ -2055,9 +2283,32 @@ __global__ __launch_bounds__(kMfThreads, 1) void k_create_pass(PassArgs a) {
     // LDS the encode would use; the packet-hash workgroups keep their CU's
     // memory path to themselves (they must keep pace with the encode).
     if (role.role == kPassPacket) {
-        if (tid >= 64) return;
+        // waves 0 (chain) and 1 (feeder) of pkt_chain / pkt_feed; the ring
+        // takes lds, unused by this role, and the counters s_a0
+        static_assert(kMfRing * 4 >= 3 * kPktLaneWords, "packet counters");
+        uint4 *ring = lds_u.pkt;
+        uint32_t *flags = reinterpret_cast<uint32_t *>(s_a0);
+        if (tid < 2) flags[tid] = 0;
+        __syncthreads();
+        if (tid >= 128) return;
+        // the wave index as a scalar: each wave's code is then uniform
+        const uint32_t lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+        // each lane stores its wave's counter: lane 0 to the counter, the
+        // others to words of their own
+        uint32_t *own = lane == 0 ? flags + (wave ? kPktProd : kPktCons)
+                                  : flags + (1 + wave) * kPktLaneWords + lane;
+        // the chain's waits cover the feeder's, which reset on any band progress
+        const uint32_t ticks = min(a.stall_ticks, 0x3FFFFFFFu) * 4u;
+        const uint32_t nb = md5_blocks(36ull + a.S);
         if (tid == 0) pass_stamp_start(a, kPassPacket);
-        for (uint32_t r = role.index * 64 + tid; r < a.R; r += a.h3_blocks * 64) pass_packet_lane(a, r);
+        uint32_t J = 0;
+        for (uint32_t r0 = role.index * 64; r0 < a.R; r0 += a.h3_blocks * 64, J += nb) {
+            const uint32_t r = min(r0 + lane, a.R - 1);
+            const bool ok = wave ? pkt_feed(a, r, ring, flags, own, J, ticks)
+                                 : pkt_chain(a, a.pkt_md5 + (size_t)r * 16, r0 + lane < a.R, ring,
+                                             flags, own, J, nb, ticks);
+            if (!ok) break;
+        }
         if (tid == 0) pass_stamp_end(a, kPassPacket);
         return;
     }
@@ -2342,7 +2593,7 @@ __device__ __forceinline__ uint32_t packet_blocks_within(uint64_t e) {
 }
 
 // Packet r of piece P (exponent P.first_exp + r), band by band as in
-// pass_packet_lane.  A lane with store = false hashes along (its wave waits
+// pass_packet_walk, on one wave.  A lane with store = false hashes along (its wave waits
 // together) and writes nothing.  A piece of column groups [cg0, cg0 +
 // col_groups) continues the row's hash from P.md5_state when cg0 > 0 (the
 // earlier pieces hashed every block within column group cg0's start) and,
