@@ -2088,13 +2088,14 @@ __device__ __forceinline__ bool pass_packet_walk(const PassArgs &a, uint32_t r, 
 // chain (wave 0) runs the rounds from them (md5_compress_w).  The chain is
 // the pass's serial path and a lone wave issues one instruction per ~4
 // cycles whatever its kind, so moving the loads and the message additions to
-// the other wave cuts its instructions per block from ~331 to ~282.  Two
-// counters in LDS pace the waves: prod (blocks written) and cons (blocks
-// whose inputs the chain has read), both counted over all packets of the
-// workgroup; the feeder writes block j once cons > j - kPktRing, the chain
-// reads it once prod > j.  A wave that gives up (a band wait that gave up,
-// or kPktAbort from the other wave, or no movement for the stall limit)
-// stores kPktAbort in its own counter, so the other's waits end too.
+// the other wave cuts its instructions per block from ~331 to ~300 (260
+// round VALU).  Two counters in LDS pace the waves: prod (blocks written)
+// and cons (blocks whose inputs the chain has used), both counted over all
+// packets of the workgroup; the feeder writes block j once cons > j -
+// kPktRing, the chain reads it once prod > j.  A wave that gives up (a band
+// wait that gave up, kPktAbort from the other wave, or no movement for the
+// wait limit) stores kPktAbort in its own counter, so the other's waits end
+// too.
 constexpr uint32_t kPktRing = 4;  // blocks of 64 lanes x 64 round inputs (16 KiB each)
 constexpr uint32_t kPktAbort = 0xFFFFFFFFu;
 // word offsets in the flag area: the two counters, then one word per lane so
