@@ -2247,7 +2247,7 @@ __device__ __forceinline__ bool pkt_chain(const PassArgs &a, uint8_t *digest, bo
     return true;
 failed:
     // a feeder that gave up flagged the error itself; a stall here flags it
-    if (__builtin_amdgcn_readfirstlane(pkt_load(flags + kPktProd)) != kPktAbort)
+    if ((uint32_t)__builtin_amdgcn_readfirstlane(pkt_load(flags + kPktProd)) != kPktAbort)
         atomicOr(a.err, kPassErrWait);
     pkt_store(own, kPktAbort);
     return false;
